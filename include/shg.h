@@ -1,0 +1,131 @@
+/*
+ * shg.h — C ABI of the MI355X focused-Gaussian-beam SHG split-step solver.
+ *
+ * The reference program has no library interface; its solve sits behind
+ *   (1) the CLI  `SHG <P W> <waist um> <T1 C> <T2 C> <save_only_last> <save_temperature>`
+ *       (src/SHG.cpp:37-50), and
+ *   (2) the in-process C++ call
+ *       Solver::run(Power, waist, focalpoint, Temp, T_inf, Tpeltier1, Tpeltier2,
+ *                   save_only_last, save_temperature)            (src/headers/Solver.h:152,279)
+ *       over raw host arrays Efields::Pump/Signal (src/headers/Efields.h:150).
+ * Every entry point below states which part of that path it replaces.
+ *
+ * Conventions
+ *   - Plain C linkage, POD structs, no exceptions cross the boundary.
+ *   - Every function returns SHG_OK (0) or a negative SHG_E* code; the message of
+ *     the last failure on the calling thread is returned by shg_last_error().
+ *   - Complex planes are interleaved complex64 (re, im float pairs, layout of
+ *     std::complex<float> / fftwf_complex), indexed IDX(x,y) = y*nx + x, x fastest
+ *     (src/headers/Common.h:27-29).  Host buffers are owned by the caller and copied.
+ *   - A context owns all of its device memory and one HIP stream.  Calls on one
+ *     context are not reentrant; use one context per host thread.
+ *   - Units are the reference's: um, ps, V, W, degC.  Fields are in V/um.
+ */
+#ifndef SHG_H
+#define SHG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHG_OK              0
+#define SHG_E_INVALID     (-1)   /* bad argument or unsupported parameter */
+#define SHG_E_DEVICE      (-2)   /* HIP runtime error */
+#define SHG_E_NOMEM       (-3)   /* device allocation failed */
+#define SHG_E_STATE       (-4)   /* call out of order (e.g. forward before set_fields) */
+#define SHG_E_NONFINITE   (-5)   /* NaN/Inf found in a field */
+#define SHG_E_NOCONVERGE  (-6)   /* Jacobi hit its sweep cap before the tolerance */
+
+#define SHG_CRYSTAL_SPPLT   0    /* MgO:sPPLT, src/headers/Crystal.h:55-71 */
+#define SHG_CRYSTAL_PPLN    1    /* MgO:PPLN (Gayer et al. 2008); absent from the reference */
+
+#define SHG_SCHEME_REFERENCE 0   /* RK4(dz) - L(dz) - RK4(dz), src/headers/Solver.h:236-242 */
+#define SHG_SCHEME_STRANG    1   /* RK4(dz/2) - L(dz) - RK4(dz/2) */
+
+/* Every constant the reference hard-codes (src/SHG.cpp:7-20, src/headers/Crystal.h:18-53)
+ * plus the argv values (src/SHG.cpp:37-50).  shg_default_params() fills the
+ * reference's values; the solver rounds them to float exactly as the reference's
+ * `const real_t` declarations do. */
+typedef struct shg_params {
+    int32_t nx, ny, nz;        /* transverse grid (power of two, 64..512, nx == ny) and z points */
+    int32_t crystal;           /* SHG_CRYSTAL_* */
+    int32_t thermal;           /* 0: plain SSFM, 1: THERMAL (heat equation + dk(T) feedback) */
+    int32_t scheme;            /* SHG_SCHEME_* */
+    double  lx, ly, lz;        /* crystal box [um]; dx = lx/(nx-1), dz = lz/(nz-1) */
+    double  lambda_p;          /* pump wavelength [um]; SH at lambda_p/2 */
+    double  deff;              /* [um/V] */
+    double  qpm_period;        /* grating period Lambda [um] */
+    double  alpha_p, alpha_s;  /* linear absorption [1/um] */
+    double  beta_s;            /* SH two-photon absorption [um/W] */
+    double  rho_p, rho_s;      /* walk-off angles [rad] */
+    double  k_th;              /* thermal conductivity [W/(um K)] */
+    double  h_conv;            /* heat-transfer coefficient [W/(um^2 K)] */
+    double  power;             /* pump power [W]            (argv[1]) */
+    double  waist;             /* pump waist [um]           (argv[2]) */
+    double  focus;             /* focal position [um]       (reference: 0.5*lz) */
+    double  temp;              /* crystal temperature [C]   (argv[3]) */
+    double  t_inf;             /* ambient temperature [C]   (reference: 25) */
+    double  t_peltier1;        /* oven under z < L/2 [C]    (argv[3]) */
+    double  t_peltier2;        /* oven under z >= L/2 [C]   (argv[4]) */
+    /* THERMAL solver controls (src/headers/Solver.h:295-336) */
+    int32_t heat_nz;           /* heat-grid z points; 0 = nz (the reference's coupling) */
+    int32_t jacobi_check;      /* residual check interval; reference 5000 */
+    int32_t max_outer;         /* outer fixed-point iterations; reference 5 */
+    int32_t jacobi_max;        /* cap on sweeps per outer iteration; 0 = no cap */
+    double  jacobi_tol;        /* mean squared update tolerance; 0 = reference 5e-4/(nx*ny*heat_nz) */
+} shg_params;
+
+typedef struct shg_ctx shg_ctx;
+
+/* Reference constants and the SHG.sh default run (src/SHG.sh:31-37). */
+void shg_default_params(shg_params *p);
+
+/* Last error message on this thread ("" if none). */
+const char *shg_last_error(void);
+
+/* Replaces `new Efields/Tfield/PhaseMatching` + the constant setup in Solver::run
+ * (src/headers/Solver.h:283-287, 343-347): validates the grid, evaluates the crystal
+ * constants with the reference's float/double mix, builds the propagators, the
+ * per-step phase table and the twiddles, and allocates the device state. */
+int shg_create(const shg_params *p, shg_ctx **out);
+void shg_destroy(shg_ctx *ctx);
+
+/* Reference initial pump (setPump, src/headers/Efields.h:74-104) evaluated on the
+ * host in the reference's arithmetic, written to `fh` (nx*ny complex64).  `t_centre`
+ * is the crystal temperature used for n_p (reference: Tfinal at the grid centre). */
+int shg_make_pump(const shg_params *p, double t_centre, float *fh);
+
+/* Replaces setInputPump/setNoisyField (src/headers/Efields.h:174-184): copies the
+ * caller's z = 0 fields (FH = pump, SH = signal) into the context. */
+int shg_set_fields(shg_ctx *ctx, const float *fh, const float *sh);
+
+/* Replaces Solver::CWES (src/headers/Solver.h:245-276) for the plain build: marches
+ * the fields through the crystal on the GPU.  Afterwards the context holds the state
+ * the reference writes as Pump_out / Signal_out (after nz-1 steps, z = lz). */
+int shg_forward(shg_ctx *ctx);
+
+/* Copies the current device fields to caller buffers (nx*ny complex64 each). */
+int shg_get_fields(shg_ctx *ctx, float *fh, float *sh);
+
+/* FP64 powers: P_j = 0.5*eps0*c*n_j*sum|A_j|^2*dx*dy for the z = 0 pump and the
+ * current SH, and eta = P_SH / P_FH(z=0).  New: the reference never reduces powers. */
+int shg_powers(shg_ctx *ctx, double *p_fh_in, double *p_sh_out, double *eta);
+
+/* Milliseconds of GPU time of the last shg_forward (HIP events on the context stream). */
+int shg_last_forward_ms(shg_ctx *ctx, double *ms);
+
+/* Number of transverse cell-steps executed by one shg_forward (nx*ny*steps). */
+int64_t shg_cell_steps(const shg_ctx *ctx);
+
+/* Measurement hook (no reference counterpart): average device time of one row pass and
+ * one column pass, each launched `reps` times back to back.  Clobbers the current
+ * fields; the next shg_forward starts again from the shg_set_fields state. */
+int shg_profile_kernels(shg_ctx *ctx, int reps, double *row_ms, double *col_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHG_H */

@@ -1,0 +1,178 @@
+"""MI355X-native focused-Gaussian-beam SHG split-step solver.
+
+The compute path is libshg.so (hand-written HIP for gfx950 behind the C ABI of
+include/shg.h).  This module is a thin ctypes binding that mirrors the reference's
+entry point, ``Solver::run(Power, waist, focalpoint, Temp, T_inf, Tpeltier1,
+Tpeltier2, save_only_last, save_temperature)`` (src/headers/Solver.h:152,279), and
+exposes the context calls for tests and benchmarks.  There is no CPU fallback: if the
+library is missing, importing the solver raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import (ShgParams, default_params, config1_params, config2_params,  # noqa: F401
+                     SHG_CRYSTAL_SPPLT, SHG_CRYSTAL_PPLN, SHG_SCHEME_REFERENCE, SHG_SCHEME_STRANG)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libshg.so")
+_lib = None
+
+SHG_OK = 0
+ERRORS = {-1: "SHG_E_INVALID", -2: "SHG_E_DEVICE", -3: "SHG_E_NOMEM", -4: "SHG_E_STATE",
+          -5: "SHG_E_NONFINITE", -6: "SHG_E_NOCONVERGE"}
+
+
+class ShgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Loads libshg.so (built in-tree by build.py).  Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run `python -m shg_focused_gaussian_beams_amd.build` "
+                          "(the HIP library is the only compute path)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER(ShgParams)
+    vp = ctypes.c_void_p
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    sigs = {
+        "shg_default_params": (None, [P]),
+        "shg_last_error": (ctypes.c_char_p, []),
+        "shg_create": (ctypes.c_int, [P, ctypes.POINTER(vp)]),
+        "shg_destroy": (None, [vp]),
+        "shg_make_pump": (ctypes.c_int, [P, ctypes.c_double, fp]),
+        "shg_set_fields": (ctypes.c_int, [vp, fp, fp]),
+        "shg_forward": (ctypes.c_int, [vp]),
+        "shg_get_fields": (ctypes.c_int, [vp, fp, fp]),
+        "shg_powers": (ctypes.c_int, [vp, dp, dp, dp]),
+        "shg_last_forward_ms": (ctypes.c_int, [vp, dp]),
+        "shg_cell_steps": (ctypes.c_int64, [vp]),
+        "shg_profile_kernels": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != SHG_OK:
+        raise ShgError(rc, load_library().shg_last_error().decode())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def make_pump(params, t_centre=None):
+    """The reference's initial Gaussian pump (setPump, src/headers/Efields.h:74-104),
+    evaluated in its float arithmetic by the library's host code."""
+    lib = load_library()
+    out = np.zeros((params.ny, params.nx), np.complex64)
+    _check(lib.shg_make_pump(ctypes.byref(params), params.temp if t_centre is None else t_centre, _fptr(out)))
+    return out
+
+
+class Context:
+    """One solver context: device state, stream and precomputed tables."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.params = params.copy()
+        h = ctypes.c_void_p()
+        _check(self.lib.shg_create(ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.shg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_fields(self, fh, sh):
+        fh = np.ascontiguousarray(fh, np.complex64)
+        sh = np.ascontiguousarray(sh, np.complex64)
+        n = self.params.nx * self.params.ny
+        if fh.size != n or sh.size != n:
+            raise ValueError(f"fields must hold {n} cells")
+        _check(self.lib.shg_set_fields(self.h, _fptr(fh), _fptr(sh)))
+
+    def forward(self):
+        _check(self.lib.shg_forward(self.h))
+
+    def get_fields(self):
+        shape = (self.params.ny, self.params.nx)
+        fh = np.empty(shape, np.complex64)
+        sh = np.empty(shape, np.complex64)
+        _check(self.lib.shg_get_fields(self.h, _fptr(fh), _fptr(sh)))
+        return fh, sh
+
+    def powers(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.shg_powers(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def last_forward_ms(self):
+        v = ctypes.c_double()
+        _check(self.lib.shg_last_forward_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def cell_steps(self):
+        return int(self.lib.shg_cell_steps(self.h))
+
+    def profile_kernels(self, reps=50):
+        r, c = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.shg_profile_kernels(self.h, reps, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+
+class Solver:
+    """Mirror of the reference's ``Solver`` (src/headers/Solver.h:120-153).
+
+    ``run`` has the reference's argument list and meaning.  Instead of writing .dat
+    files it returns the output fields (the state the reference saves as Pump_out /
+    Signal_out) and the FP64 powers; the SHG command-line program writes the files.
+    ``signal`` replaces the reference's clock-seeded N(0, 1e-20) SH noise
+    (src/headers/Efields.h:129-144); by default it is that noise with a fixed seed.
+    """
+
+    def __init__(self, params=None):
+        self.params = (params or default_params()).copy()
+
+    def run(self, Power, waist, focalpoint, Temp, T_inf, Tpeltier1, Tpeltier2,
+            save_only_last=True, save_temperature=False, signal=None, seed=0):
+        p = self.params.copy(power=Power, waist=waist, focus=focalpoint, temp=Temp, t_inf=T_inf,
+                             t_peltier1=Tpeltier1, t_peltier2=Tpeltier2)
+        if p.thermal:
+            raise ShgError(-1, "THERMAL runs are not available in this build")
+        pump = make_pump(p, Temp)
+        if signal is None:
+            rng = np.random.default_rng(seed)
+            signal = (rng.normal(0, 1e-20, pump.shape) + 1j * rng.normal(0, 1e-20, pump.shape)).astype(np.complex64)
+        with Context(p) as ctx:
+            ctx.set_fields(pump, signal)
+            ctx.forward()
+            fh, sh = ctx.get_fields()
+            powers = ctx.powers()
+        return fh, sh, powers

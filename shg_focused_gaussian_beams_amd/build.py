@@ -1,0 +1,42 @@
+"""Builds the HIP library (libshg.so) and the SHG command-line program in-tree.
+
+hipcc compiles for gfx950 only.  Packed FP32 instructions are disabled
+(-target-feature -packed-fp32-ops) for every HIP translation unit.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libshg.so")
+CLI = os.path.join(HERE, "SHG")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function"]
+SOURCES = ["ssfm_kernels.hip", "shg_capi.cpp"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = []
+    for s in SOURCES:
+        srcs += ["-x", "hip", os.path.join(CSRC, s)]
+    _run([hipcc, *HIP_FLAGS, "-shared", "-o", LIB + ".tmp", *srcs])
+    os.replace(LIB + ".tmp", LIB)
+    cli_src = os.path.join(CSRC, "shg_cli.cpp")
+    if os.path.exists(cli_src):
+        _run(["g++", "-O2", "-std=c++17", "-o", CLI, cli_src, "-L" + HERE, "-lshg",
+              "-Wl,-rpath,$ORIGIN", "-I" + os.path.join(HERE, "..", "include")])
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

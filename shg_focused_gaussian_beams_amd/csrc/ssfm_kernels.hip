@@ -1,0 +1,219 @@
+// ssfm_kernels.hip — the split-step z-march on gfx950.
+//
+// State layout in HBM: one float4 per transverse cell, {FH.re, FH.im, SH.re, SH.im},
+// row-major with x fastest (the reference's IDX, src/headers/Common.h:27-29), so one
+// 16-byte load gives both fields of a cell.
+//
+// One z-step of the reference (Solver::SSFM, src/headers/Solver.h:236-242) is
+//     RK4_s ; F_bwd ; x H ; F_fwd ; /(NX*NY) ; RK4_s
+// with the 2D FFTs split as row transforms R and column transforms C.  Since H is
+// diagonal in the 2D spectrum, the march becomes two kernels per step:
+//     row pass  :  R_fwd (end of step s-1) -> RK4_{s-1} -> RK4_s -> R_bwd
+//     col pass  :  C_bwd -> x H/(NX*NY) -> C_fwd
+// so every step reads and writes the field pair twice (once per pass), and all FFT,
+// propagator, scaling and RK4 work happens in registers/LDS in between.
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.h"
+#include "ssfm_kernels.h"
+
+namespace shg {
+
+// Right-hand side of the coupled-wave equations at one cell (dAdz, plain build,
+// src/headers/Solver.h:65-83), with the step constants folded:
+//   dP = c1 * S conj(P) - lossP * P,          c1 = i kappa_p e^{-i theta}
+//   dS = c2 * P^2 - (lossS + tpa |S|^2) * S,  c2 = i kappa_s e^{+i theta}
+__device__ __forceinline__ void nl_rhs(float2 P, float2 S, float4 c, const NlConst &k, float2 &dP, float2 &dS) {
+    const float ur = S.x * P.x + S.y * P.y, ui = S.y * P.x - S.x * P.y;
+    dP.x = c.x * ur - c.y * ui - k.lossP * P.x;
+    dP.y = c.x * ui + c.y * ur - k.lossP * P.y;
+    const float vr = P.x * P.x - P.y * P.y, vi = 2.0f * P.x * P.y;
+    const float g = k.lossS + k.tpa * (S.x * S.x + S.y * S.y);
+    dS.x = c.z * vr - c.w * vi - g * S.x;
+    dS.y = c.z * vi + c.w * vr - g * S.y;
+}
+
+// solverRK4 (src/headers/Solver.h:193-233): stages A + {.5,.5,1} k (not scaled by the
+// step), final A += h/6 (k1 + 2k2 + 2k3 + k4).
+__device__ __forceinline__ void rk4(float2 &P, float2 &S, float4 c, const NlConst &k) {
+    float2 k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
+    nl_rhs(P, S, c, k, k1p, k1s);
+    nl_rhs(make_float2(P.x + 0.5f * k1p.x, P.y + 0.5f * k1p.y), make_float2(S.x + 0.5f * k1s.x, S.y + 0.5f * k1s.y), c,
+           k, k2p, k2s);
+    nl_rhs(make_float2(P.x + 0.5f * k2p.x, P.y + 0.5f * k2p.y), make_float2(S.x + 0.5f * k2s.x, S.y + 0.5f * k2s.y), c,
+           k, k3p, k3s);
+    nl_rhs(make_float2(P.x + k3p.x, P.y + k3p.y), make_float2(S.x + k3s.x, S.y + k3s.y), c, k, k4p, k4s);
+    P.x += k.h6 * (k1p.x + 2.0f * k2p.x + 2.0f * k3p.x + k4p.x);
+    P.y += k.h6 * (k1p.y + 2.0f * k2p.y + 2.0f * k3p.y + k4p.y);
+    S.x += k.h6 * (k1s.x + 2.0f * k2s.x + 2.0f * k3s.x + k4s.x);
+    S.y += k.h6 * (k1s.y + 2.0f * k2s.y + 2.0f * k3s.y + k4s.y);
+}
+
+template <int N> struct RowCfg {
+    static constexpr int E = FftPlan<N>::E;
+    static constexpr int T = N / E;               // threads per row
+    static constexpr int THREADS = 64;            // one wave per workgroup
+    static constexpr int ROWS = THREADS / T;      // rows per workgroup
+    static constexpr int L = LdsLen<N>::value;
+};
+
+template <int N> struct ColCfg {
+    static constexpr int E = FftPlan<N>::E;
+    static constexpr int T = N / E;               // threads per column
+    static constexpr int CW = N == 512 ? 4 : 8;   // columns per workgroup
+    static constexpr int THREADS = CW * T;
+    static constexpr int L = LdsLen<N>::value;
+};
+
+__device__ __forceinline__ void block_sync() { __syncthreads(); }
+
+// Row pass over rows [blockIdx.x*ROWS, +ROWS) of one N x N field pair.
+template <int N>
+__global__ __launch_bounds__(64) void k_row_pass(float4 *__restrict__ U, const float2 *__restrict__ twg, RowArgs a) {
+    using C = RowCfg<N>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[2][C::ROWS][C::L];
+    __shared__ float2 tw[N];
+    const int lane = threadIdx.x;
+    const int t = lane % T, rr = lane / T;
+    const int row = blockIdx.x * C::ROWS + rr;
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    float4 *Ur = U + (size_t)row * N;
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = Ur[t + j * T];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { block_sync(); };
+    if (a.flags & ROW_FWD) fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+    if (a.flags & ROW_NPREV) {
+#pragma unroll
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], a.cprev, a.nl);
+    }
+    if (a.flags & ROW_NCUR) {
+#pragma unroll
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], a.ccur, a.nl);
+    }
+    if (a.flags & ROW_BWD) fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) Ur[t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// Column pass over columns [blockIdx.x*CW, +CW): C_bwd, x H/(N*N), C_fwd.
+// G holds the separable propagator: Gx_p[N], Gy_p[N], Gx_s[N], Gy_s[N] (Gy carries 1/N^2).
+template <int N>
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(float4 *__restrict__ U,
+                                                                 const float2 *__restrict__ twg,
+                                                                 const float2 *__restrict__ G) {
+    using C = ColCfg<N>;
+    constexpr int E = C::E, T = C::T, CW = C::CW;
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x;
+    const int c = tid % CW, t = tid / CW;
+    const int x = blockIdx.x * CW + c;
+    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = U[(size_t)(t + j * T) * N + x];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { block_sync(); };
+    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    const float2 gxp = G[x], gxs = G[2 * N + x];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const int y = t + j * T;
+        fa[j] = c_mul(c_mul(gxp, G[N + y]), fa[j]);
+        fb[j] = c_mul(c_mul(gxs, G[3 * N + y]), fb[j]);
+    }
+    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++)
+        U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// FP64 sums of |FH|^2 and |SH|^2: per-block partials, then one block folds them.
+// Deterministic (fixed reduction tree).
+__global__ __launch_bounds__(256) void k_power_partial(const float4 *__restrict__ U, int64_t n, double2 *part) {
+    double sp = 0.0, ss = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = U[i];
+        sp += (double)v.x * v.x + (double)v.y * v.y;
+        ss += (double)v.z * v.z + (double)v.w * v.w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sp += __shfl_down(sp, off, 64);
+        ss += __shfl_down(ss, off, 64);
+    }
+    __shared__ double2 w[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) w[wv] = make_double2(sp, ss);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double2 r = w[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) { r.x += w[k].x; r.y += w[k].y; }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_power_final(const double2 *__restrict__ part, int nparts, double2 *out) {
+    __shared__ double2 s[256];
+    double2 r = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nparts; i += 256) { r.x += part[i].x; r.y += part[i].y; }
+    s[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { s[threadIdx.x].x += s[threadIdx.x + w].x; s[threadIdx.x].y += s[threadIdx.x + w].y; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.  The caller guarantees the buffers hold n*n cells, n*... twiddles etc.
+
+template <int N> static hipError_t row_launch(float4 *U, const float2 *tw, const RowArgs &a, hipStream_t st) {
+    using C = RowCfg<N>;
+    hipLaunchKernelGGL(k_row_pass<N>, dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, tw, a);
+    return hipGetLastError();
+}
+template <int N> static hipError_t col_launch(float4 *U, const float2 *tw, const float2 *G, hipStream_t st) {
+    using C = ColCfg<N>;
+    hipLaunchKernelGGL(k_col_pass<N>, dim3(N / C::CW), dim3(C::THREADS), 0, st, U, tw, G);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_pass(int n, float4 *U, const float2 *tw, const RowArgs &a, hipStream_t st) {
+    switch (n) {
+        case 64: return row_launch<64>(U, tw, a, st);
+        case 128: return row_launch<128>(U, tw, a, st);
+        case 256: return row_launch<256>(U, tw, a, st);
+        case 512: return row_launch<512>(U, tw, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_col_pass(int n, float4 *U, const float2 *tw, const float2 *G, hipStream_t st) {
+    switch (n) {
+        case 64: return col_launch<64>(U, tw, G, st);
+        case 128: return col_launch<128>(U, tw, G, st);
+        case 256: return col_launch<256>(U, tw, G, st);
+        case 512: return col_launch<512>(U, tw, G, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_power(const float4 *U, int64_t ncell, double2 *part, int nparts, double2 *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_power_partial, dim3(nparts), dim3(256), 0, st, U, ncell, part);
+    hipLaunchKernelGGL(k_power_final, dim3(1), dim3(256), 0, st, part, nparts, out);
+    return hipGetLastError();
+}
+
+}  // namespace shg

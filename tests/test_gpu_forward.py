@@ -1,0 +1,90 @@
+"""GPU parity of the plain z-march (libshg.so through the C ABI) against the oracle.
+
+Bar (stated by the task for floating point): output fields within relative L2 1e-4 of
+the FP32 restatement of the reference (oracle32), SH efficiency within 1e-4 relative.
+The distance to the all-FP64 restatement (oracle64) is printed for every case; it is
+larger where the reference's float phase-mismatch cancellation dominates.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from shg_focused_gaussian_beams_amd.params import default_params, config1_params, SHG_SCHEME_STRANG
+
+TOL_FIELD = 1e-4
+TOL_ETA = 1e-4
+
+
+def _inputs(p, seed=1, noise=1e-6):
+    import shg_focused_gaussian_beams_amd as shg
+    pump = shg.make_pump(p)
+    rng = np.random.default_rng(seed)
+    amp = noise * np.abs(pump).max()
+    pump = (pump + amp * (rng.standard_normal(pump.shape) + 1j * rng.standard_normal(pump.shape))).astype(np.complex64)
+    sh = np.zeros_like(pump)
+    return pump, sh
+
+
+def _run_gpu(shg, p, fh0, sh0):
+    with shg.Context(p) as ctx:
+        ctx.set_fields(fh0, sh0)
+        ctx.forward()
+        fh, sh = ctx.get_fields()
+        pw = ctx.powers()
+    return fh, sh, pw
+
+
+CASES = {
+    "ref64_nz20": lambda: default_params(nz=20),
+    "cfg1_n64_nz40": lambda: config1_params(nx=64, ny=64, nz=40),
+    "cfg1_n128_nz200": lambda: config1_params(),
+    "cfg1_n256_nz30": lambda: config1_params(nx=256, ny=256, nz=30, lx=400.0, ly=400.0),
+    "cfg1_n512_nz8": lambda: config1_params(nx=512, ny=512, nz=8, lx=500.0, ly=500.0),
+    "cfg1_strang_n128_nz50": lambda: config1_params(nz=50, scheme=SHG_SCHEME_STRANG),
+    "hot_n128_nz60": lambda: config1_params(nz=60, power=50.0, waist=20.0, lx=160.0, ly=160.0,
+                                             alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(CASES))
+def test_forward_matches_oracle32(gpu, case):
+    p = CASES[case]()
+    print(f"\ncase={case} n={p.nx} nz={p.nz}", flush=True)
+    fh0, sh0 = _inputs(p)
+    fh, sh, (p_in, p_out, eta) = _run_gpu(gpu, p, fh0, sh0)
+    rfh, rsh = orc.cwes(p, fh0, sh0, prec=32)
+    dfh, dsh = orc.cwes(p, fh0, sh0, prec=64)
+    e_fh, e_sh = orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh)
+    eta_ref = orc.power(p, 1, rsh) / orc.power(p, 0, fh0)
+    e_eta = abs(eta - eta_ref) / eta_ref
+    print(f"  vs oracle32: FH {e_fh:.2e} SH {e_sh:.2e} eta {eta:.6e} (ref {eta_ref:.6e}, rel {e_eta:.2e})")
+    print(f"  vs oracle64: FH {orc.rel_l2(fh, dfh):.2e} SH {orc.rel_l2(sh, dsh):.2e}")
+    assert np.isfinite(fh).all() and np.isfinite(sh).all()
+    assert e_fh <= TOL_FIELD, e_fh
+    assert e_sh <= TOL_FIELD, e_sh
+    assert e_eta <= TOL_ETA, e_eta
+
+
+@pytest.mark.gpu
+def test_forward_is_repeatable(gpu):
+    p = config1_params(nz=30)
+    fh0, sh0 = _inputs(p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(fh0, sh0)
+        ctx.forward()
+        a = ctx.get_fields()
+        ctx.forward()
+        b = ctx.get_fields()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.gpu
+def test_input_power_matches_closed_form(gpu):
+    # A well-sampled Gaussian carries P: sum|A0|^2 dx dy = 2P/(eps0 c n).
+    p = config1_params(nx=256, ny=256, lx=400.0, ly=400.0, nz=4)
+    fh0 = gpu.make_pump(p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(fh0, np.zeros_like(fh0))
+        p_in, _, _ = ctx.powers()
+    assert abs(p_in - p.power) / p.power < 2e-2, p_in
