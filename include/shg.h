@@ -101,10 +101,23 @@ int shg_make_pump(const shg_params *p, double t_centre, float *fh);
  * caller's z = 0 fields (FH = pump, SH = signal) into the context. */
 int shg_set_fields(shg_ctx *ctx, const float *fh, const float *sh);
 
-/* Replaces Solver::CWES (src/headers/Solver.h:245-276) for the plain build: marches
- * the fields through the crystal on the GPU.  Afterwards the context holds the state
- * the reference writes as Pump_out / Signal_out (after nz-1 steps, z = lz). */
+/* Replaces Solver::CWES (src/headers/Solver.h:245-276): marches the shg_set_fields
+ * fields through the crystal on the GPU.  Afterwards the context holds the state the
+ * reference writes as Pump_out / Signal_out (after nz-1 steps, z = lz).  A plain
+ * context executes nz-1 steps (the reference's last step has no observable effect); a
+ * THERMAL context executes all nz steps with the current temperature, updating the heat
+ * source after every step as the reference does. */
 int shg_forward(shg_ctx *ctx);
+
+/* Restores the device fields to the last shg_set_fields state (z = 0). */
+int shg_reset_fields(shg_ctx *ctx);
+
+/* Advances the current device fields by z-steps [first_step, first_step + nsteps) of the
+ * reference's z-loop (Solver::SSFM per step, src/headers/Solver.h:236-242, 265), with
+ * first_step + nsteps <= nz - 1.  Used for the reference's per-slice output mode
+ * (save_only_last = 0, src/headers/Solver.h:257-262), which saves the fields before
+ * every step. */
+int shg_march(shg_ctx *ctx, int first_step, int nsteps);
 
 /* Copies the current device fields to caller buffers (nx*ny complex64 each). */
 int shg_get_fields(shg_ctx *ctx, float *fh, float *sh);
@@ -118,6 +131,41 @@ int shg_last_forward_ms(shg_ctx *ctx, double *ms);
 
 /* Number of transverse cell-steps executed by one shg_forward (nx*ny*steps). */
 int64_t shg_cell_steps(const shg_ctx *ctx);
+
+/* ----- THERMAL build (src/headers/Solver.h:289-340, src/headers/Tfield.h) ------------- */
+
+typedef struct shg_thermal_stats {
+    int32_t outer_iterations;  /* global iterations run (reference: at most 5) */
+    int32_t reserved;
+    int64_t sweeps_last;       /* Jacobi sweeps of the last outer iteration */
+    int64_t sweeps_total;
+    double  residual_last;     /* mean squared update at the last residual check (FP64) */
+    double  jacobi_ms;         /* device time of all Jacobi sweeps */
+    double  march_ms;          /* device time of all z-marches */
+} shg_thermal_stats;
+
+/* Replaces the THERMAL branch of Solver::run (src/headers/Solver.h:289-340) on a context
+ * created with thermal = 1: temperature initialised to `temp` with the bottom ovens
+ * (Tfield.h:31-68) and Q = 0, then up to max_outer iterations of [Jacobi to the tolerance
+ * with the residual checked every jacobi_check sweeps and the reference's stagnation and
+ * early-stop rules; dk(T) and the z-march with the heat source updated after every step].
+ * The SH input is the one given to shg_set_fields; the pump is rebuilt each iteration by
+ * setPump at the grid-centre temperature when pump_from_temperature != 0 (as the
+ * reference does), else the shg_set_fields pump is reused.  Afterwards shg_get_fields
+ * returns the Pump_out/Signal_out state of the last iteration. */
+int shg_thermal_run(shg_ctx *ctx, int pump_from_temperature, shg_thermal_stats *stats);
+
+/* `nsweeps` Jacobi sweeps of heatEquationTopOpen (src/headers/Tfield.h:113-221) on the
+ * current temperature and heat source; *mean_sq_update (if non-null) receives
+ * sum((Tf-Ti)^2)/(nx*ny*heat_nz) of the last sweep in FP64 (reducedSum, Tfield.h:13-28). */
+int shg_jacobi(shg_ctx *ctx, int64_t nsweeps, double *mean_sq_update);
+
+/* Temperature and heat source (nx*ny*heat_nz floats, IDX(x,y,z) = z*nx*ny + y*nx + x).
+ * shg_set_temperature also rebuilds the propagators at the grid-centre temperature. */
+int shg_get_temperature(shg_ctx *ctx, float *T);
+int shg_set_temperature(shg_ctx *ctx, const float *T);
+int shg_get_heat_source(shg_ctx *ctx, float *Q);
+int shg_set_heat_source(shg_ctx *ctx, const float *Q);
 
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass, each launched `reps` times back to back.  Clobbers the current

@@ -24,6 +24,16 @@ ERRORS = {-1: "SHG_E_INVALID", -2: "SHG_E_DEVICE", -3: "SHG_E_NOMEM", -4: "SHG_E
           -5: "SHG_E_NONFINITE", -6: "SHG_E_NOCONVERGE"}
 
 
+class ThermalStats(ctypes.Structure):
+    _fields_ = [("outer_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("sweeps_last", ctypes.c_int64), ("sweeps_total", ctypes.c_int64),
+                ("residual_last", ctypes.c_double), ("jacobi_ms", ctypes.c_double),
+                ("march_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class ShgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -51,11 +61,19 @@ def load_library(path=LIB_PATH):
         "shg_make_pump": (ctypes.c_int, [P, ctypes.c_double, fp]),
         "shg_set_fields": (ctypes.c_int, [vp, fp, fp]),
         "shg_forward": (ctypes.c_int, [vp]),
+        "shg_reset_fields": (ctypes.c_int, [vp]),
+        "shg_march": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_get_fields": (ctypes.c_int, [vp, fp, fp]),
         "shg_powers": (ctypes.c_int, [vp, dp, dp, dp]),
         "shg_last_forward_ms": (ctypes.c_int, [vp, dp]),
         "shg_cell_steps": (ctypes.c_int64, [vp]),
         "shg_profile_kernels": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
+        "shg_thermal_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ThermalStats)]),
+        "shg_jacobi": (ctypes.c_int, [vp, ctypes.c_int64, dp]),
+        "shg_get_temperature": (ctypes.c_int, [vp, fp]),
+        "shg_set_temperature": (ctypes.c_int, [vp, fp]),
+        "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
+        "shg_set_heat_source": (ctypes.c_int, [vp, fp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -121,6 +139,12 @@ class Context:
     def forward(self):
         _check(self.lib.shg_forward(self.h))
 
+    def reset_fields(self):
+        _check(self.lib.shg_reset_fields(self.h))
+
+    def march(self, first_step, nsteps):
+        _check(self.lib.shg_march(self.h, first_step, nsteps))
+
     def get_fields(self):
         shape = (self.params.ny, self.params.nx)
         fh = np.empty(shape, np.complex64)
@@ -140,6 +164,43 @@ class Context:
 
     def cell_steps(self):
         return int(self.lib.shg_cell_steps(self.h))
+
+    # THERMAL
+    def heat_shape(self):
+        hz = self.params.heat_nz if self.params.heat_nz > 0 else self.params.nz
+        return (hz, self.params.ny, self.params.nx)
+
+    def thermal_run(self, pump_from_temperature=True):
+        st = ThermalStats()
+        _check(self.lib.shg_thermal_run(self.h, 1 if pump_from_temperature else 0, ctypes.byref(st)))
+        return st.as_dict()
+
+    def jacobi(self, nsweeps, residual=True):
+        r = ctypes.c_double()
+        _check(self.lib.shg_jacobi(self.h, nsweeps, ctypes.byref(r) if residual else None))
+        return r.value
+
+    def get_temperature(self):
+        t = np.empty(self.heat_shape(), np.float32)
+        _check(self.lib.shg_get_temperature(self.h, _fptr(t)))
+        return t
+
+    def set_temperature(self, T):
+        T = np.ascontiguousarray(T, np.float32)
+        if T.size != int(np.prod(self.heat_shape())):
+            raise ValueError("temperature must hold nx*ny*heat_nz values")
+        _check(self.lib.shg_set_temperature(self.h, _fptr(T)))
+
+    def get_heat_source(self):
+        q = np.empty(self.heat_shape(), np.float32)
+        _check(self.lib.shg_get_heat_source(self.h, _fptr(q)))
+        return q
+
+    def set_heat_source(self, Q):
+        Q = np.ascontiguousarray(Q, np.float32)
+        if Q.size != int(np.prod(self.heat_shape())):
+            raise ValueError("heat source must hold nx*ny*heat_nz values")
+        _check(self.lib.shg_set_heat_source(self.h, _fptr(Q)))
 
     def profile_kernels(self, reps=50):
         r, c = ctypes.c_double(), ctypes.c_double()
@@ -164,15 +225,17 @@ class Solver:
             save_only_last=True, save_temperature=False, signal=None, seed=0):
         p = self.params.copy(power=Power, waist=waist, focus=focalpoint, temp=Temp, t_inf=T_inf,
                              t_peltier1=Tpeltier1, t_peltier2=Tpeltier2)
-        if p.thermal:
-            raise ShgError(-1, "THERMAL runs are not available in this build")
         pump = make_pump(p, Temp)
         if signal is None:
             rng = np.random.default_rng(seed)
             signal = (rng.normal(0, 1e-20, pump.shape) + 1j * rng.normal(0, 1e-20, pump.shape)).astype(np.complex64)
         with Context(p) as ctx:
             ctx.set_fields(pump, signal)
-            ctx.forward()
+            if p.thermal:
+                self.thermal_stats = ctx.thermal_run(pump_from_temperature=True)
+                self.temperature = ctx.get_temperature() if save_temperature else None
+            else:
+                ctx.forward()
             fh, sh = ctx.get_fields()
             powers = ctx.powers()
         return fh, sh, powers

@@ -15,7 +15,7 @@ CLI = os.path.join(HERE, "SHG")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
-SOURCES = ["ssfm_kernels.hip", "shg_capi.cpp"]
+SOURCES = ["ssfm_kernels.hip", "thermal_kernels.hip", "shg_capi.cpp"]
 
 
 def _run(cmd):

@@ -172,5 +172,44 @@ inline std::vector<float> step_coefficients(const Consts &k, float temp, int nst
     return c;
 }
 
+// Heat-grid z-spacing: the reference couples the heat grid to the field grid (dz); a
+// heat grid of hz slices spans the same crystal length.
+inline float heat_dz(const Consts &k, int hz) { return hz == k.nz ? k.dz : k.lz / (float)(unsigned)(hz - 1); }
+
+// Field step s -> heat slice (one to one when hz == nz).
+inline int heat_slice(int s, int nz, int hz) { return hz == nz ? s : (int)(((int64_t)s * hz) / nz); }
+
+// Jacobi weights of heatEquationTopOpen (src/headers/Tfield.h:118-125) with the
+// reference's typing: float products/quotients, the edge denominators and 0.5*T_inf*Bi
+// terms in double, V/H/D rounded to float.
+template <class JC> inline JC jacobi_const(const Consts &k, float hdz, float k_th, float h_conv, float t_inf) {
+    JC c{};
+    const float dx = k.dx, dy = k.dy, dz = hdz;
+    const float Bix = h_conv * dx / k_th, Biy = h_conv * dy / k_th, Biz = h_conv * dz / k_th;
+    c.ax = dx * dx / (2.0f * dy * dy); c.ay = dy * dy / (2.0f * dx * dx); c.az = dz * dz / (2.0f * dx * dx);
+    c.bx = dx * dx / (2.0f * dz * dz); c.by = dy * dy / (2.0f * dz * dz); c.bz = dz * dz / (2.0f * dy * dy);
+    c.av = dz / (2.0f * dx); c.bv = dx / (2.0f * dz); c.cv = dx * dz / (4.0f * dy * dy);
+    c.V = (double)(float)(1.0 / ((double)(c.av + c.bv + 2.0f * c.cv) + 0.5 * (double)(Bix + Biz)));
+    c.ah = dz / (2.0f * dy); c.bh = dy / (2.0f * dz); c.ch = dy * dz / (4.0f * dx * dx);
+    c.H = (double)(float)(1.0 / ((double)(c.ah + c.bh + 2.0f * c.ch) + 0.5 * (double)(Biy + Biz)));
+    c.ad = dx / (2.0f * dy); c.bd = dy / (2.0f * dx); c.cd = dx * dy / (4.0f * dz * dz);
+    c.D = (double)(float)(1.0 / ((double)(c.ad + c.bd + 2.0f * c.cd) + 0.5 * (double)(Bix + Biy)));
+    c.AV = dy * dz / dx; c.BV = dx * dz / dy; c.CV = dx * dy / dz;
+    const float BiD = Bix * dy + Biy * dz + Biz * dx;
+    c.GV = 1.0f / (c.AV + c.BV + c.CV + BiD);
+    c.L2 = 1.0f / (2.0f / (dx * dx) + 2.0f / (dy * dy) + 2.0f / (dz * dz));
+    c.dxx = dx * dx; c.dyy = dy * dy; c.dzz = dz * dz; c.kth = k_th;
+    c.fdenx = 1.0f + 2.0f * c.ax + 2.0f * c.bx + Bix;
+    c.fdeny = 1.0f + 2.0f * c.ay + 2.0f * c.by + Biy;
+    c.fdenz = 1.0f + 2.0f * c.az + 2.0f * c.bz + Biz;
+    c.bixT = Bix * t_inf; c.biyT = Biy * t_inf; c.bizT = Biz * t_inf;
+    c.tinfBiD = t_inf * BiD;
+    const double hv = 0.5 * (double)t_inf;
+    c.sv = hv * (double)(Bix + Biz);
+    c.sh = hv * (double)(Biy + Biz);
+    c.sd = hv * (double)(Bix + Biy);
+    return c;
+}
+
 }  // namespace phys
 }  // namespace shg

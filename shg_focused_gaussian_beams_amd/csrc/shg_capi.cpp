@@ -1,16 +1,21 @@
 // shg_capi.cpp — the C ABI (include/shg.h): contexts, device state and the z-march
-// schedule.  Host code only; device code lives in ssfm_kernels.hip.
+// schedules of the plain and THERMAL builds.  Host code only; device code lives in
+// ssfm_kernels.hip and thermal_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/shg.h"
 #include "physics.h"
 #include "ssfm_kernels.h"
+#include "thermal_kernels.h"
 
 namespace {
 
@@ -27,24 +32,39 @@ int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess) return fail(SHG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-bool pow2_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512; }
+#define RC_TRY(expr)           \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
+
+bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512; }
 
 }  // namespace
 
 struct shg_ctx {
     shg_params p{};
     int n = 0;
-    int steps = 0;                   // z-steps executed by shg_forward (plain: nz - 1)
+    int steps = 0;                   // z-steps executed by shg_forward (plain nz-1, THERMAL nz)
+    int hz = 0;                      // heat-grid slices (THERMAL)
     hipStream_t stream = nullptr;
-    float4 *U = nullptr, *U0 = nullptr;
+    float4 *U = nullptr, *U0 = nullptr, *Uout = nullptr;
     float2 *tw = nullptr, *G = nullptr;
     double2 *part = nullptr, *red = nullptr;
     int nparts = 0;
-    std::vector<float> coef;         // (c1, c2) per step, 4 floats
+    std::vector<float> coef;         // plain: (c1, c2) per step, 4 floats
     shg::NlConst nl{};
+    // THERMAL state
+    float *T[2] = {nullptr, nullptr};
+    int tcur = 0;                    // index of the buffer holding the latest temperature
+    float *Q = nullptr, *DKint = nullptr;
+    double *jpart = nullptr, *jres = nullptr;
+    int njpart = 0;
+    shg::JacobiConst jc{};
+    shg::ThermConst tc{};
+    float t_centre = 0.f;            // temperature used for the propagators and powers
     bool have_fields = false;
     double p_fh_in = 0.0;
-    double sum_fh_in = 0.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
     hipGraphExec_t gexec = nullptr;
@@ -58,9 +78,18 @@ float4 coef_at(const shg_ctx *c, int s) {
     return make_float4(c->coef[4 * s], c->coef[4 * s + 1], c->coef[4 * s + 2], c->coef[4 * s + 3]);
 }
 
-// Enqueues steps [first, first + count) of the plain z-march on the context stream.
-// The field is in real space before and after.
-int enqueue_march(shg_ctx *c, int first, int count) {
+int upload_propagators(shg_ctx *c, float temp) {
+    const shg::phys::Consts k(c->p);
+    const auto G = shg::phys::propagator_factors(k, temp, (double)k.dz);
+    HIP_TRY(hipMemcpyAsync(c->G, G.data(), G.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->t_centre = temp;
+    return SHG_OK;
+}
+
+// Plain build: enqueues steps [first, first + count) on the context stream.  The field
+// is in real space before and after.
+int enqueue_march_plain(shg_ctx *c, int first, int count) {
     if (count <= 0) return SHG_OK;
     shg::RowArgs a{};
     a.nl = c->nl;
@@ -82,6 +111,41 @@ int enqueue_march(shg_ctx *c, int first, int count) {
     return SHG_OK;
 }
 
+shg::ThermStep therm_step(const shg_ctx *c, int s) {
+    const shg::phys::Consts k(c->p);
+    shg::ThermStep st{};
+    const float incs[3] = {0.0f, 0.25f, 0.5f};     // 0.5 * {0, .5, 1} (src/headers/Solver.h:201-212)
+    for (int i = 0; i < 3; i++) st.zfac[i] = (double)(k.dz * incs[i]) + (s + 1.0) * (double)k.dz;
+    st.sp1 = s + 1.0;
+    st.slice = shg::phys::heat_slice(s, c->p.nz, c->hz);
+    return st;
+}
+
+// THERMAL build: all nz steps of Solver::CWES (src/headers/Solver.h:245-276) with the
+// heat-source update after every step; Uout receives the fields before the last step.
+int enqueue_march_thermal(shg_ctx *c) {
+    const int nz = c->p.nz;
+    shg::ThermRowArgs a{};
+    a.k = c->tc;
+    a.cur = therm_step(c, 0);
+    a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
+    float *T = c->T[c->tcur];
+    HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
+    for (int s = 0; s < nz; s++) {
+        HIP_TRY(shg::launch_col_pass(c->n, c->U, c->tw, c->G, c->stream));
+        a.prev = therm_step(c, s);
+        if (s < nz - 1) {
+            a.cur = therm_step(c, s + 1);
+            a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
+            if (s + 1 == nz - 1) a.flags |= shg::TROW_SNAP;
+        } else {
+            a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV;
+        }
+        HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
+    }
+    return SHG_OK;
+}
+
 int reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh) {
     HIP_TRY(shg::launch_power(U, (int64_t)c->n * c->n, c->part, c->nparts, c->red, c->stream));
     double2 h;
@@ -92,23 +156,106 @@ int reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh) {
     return SHG_OK;
 }
 
-double power_of(const shg_ctx *c, int which, double sum) {
+double power_of(const shg_ctx *c, int which, double sum, double temp) {
     const shg::phys::Consts k(c->p);
     const double eps0 = 8.8541878128e-6, cl = 299.792458;
     const double lam = which == 0 ? (double)k.lp : (double)k.ls;
-    const double n = shg::phys::index_d(c->p.crystal, lam, c->p.temp);
+    const double n = shg::phys::index_d(c->p.crystal, lam, temp);
     return 0.5 * eps0 * cl * n * sum * (double)k.dx * (double)k.dy;
 }
 
 void release(shg_ctx *c) {
     if (!c) return;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->tw, (void *)c->G, (void *)c->part, (void *)c->red})
+    for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
+                      (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
+                      (void *)c->jpart, (void *)c->jres})
         if (ptr) (void)hipFree(ptr);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int set_u_from_host(shg_ctx *c, float4 *dst, const float *fh, const float *sh) {
+    const size_t ncell = (size_t)c->n * c->n;
+    float *h = c->host_stage.data();
+    for (size_t i = 0; i < ncell; i++) {
+        h[4 * i] = fh[2 * i];
+        h[4 * i + 1] = fh[2 * i + 1];
+        h[4 * i + 2] = sh[2 * i];
+        h[4 * i + 3] = sh[2 * i + 1];
+    }
+    HIP_TRY(hipMemcpyAsync(dst, h, ncell * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHG_OK;
+}
+
+// Jacobi sweeps on the current temperature; with `want_resid` the mean squared update
+// (FP64) of the last sweep is returned in *resid.
+int jacobi_run(shg_ctx *c, int64_t nsweeps, bool want_resid, double *resid) {
+    const size_t n3 = (size_t)c->n * c->n * c->hz;
+    for (int64_t i = 0; i < nsweeps; i++) {
+        const bool r = want_resid && i == nsweeps - 1;
+        float *Ti = c->T[c->tcur], *Tf = c->T[c->tcur ^ 1];
+        HIP_TRY(shg::launch_jacobi(Tf, Ti, c->Q, c->n, c->n, c->hz, c->jc, r ? c->jpart : nullptr, c->njpart, c->stream));
+        c->tcur ^= 1;
+        if (r) {
+            HIP_TRY(shg::launch_sum_partials(c->jpart, c->njpart, c->jres, c->stream));
+            double h;
+            HIP_TRY(hipMemcpyAsync(&h, c->jres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            *resid = h / (double)n3;
+        }
+    }
+    return SHG_OK;
+}
+
+int thermal_init_temperature(shg_ctx *c) {
+    const int n = c->n, hz = c->hz;
+    const size_t n2 = (size_t)n * n;
+    std::vector<float> t(n2 * hz, (float)c->p.temp);          // initTemperature (Tfield.h:31-45)
+    const float nzf = (float)(unsigned)hz;                      // initBottomOvensTemperature (:48-68)
+    for (int z = 0; z < hz; z++)
+        for (int x = 0; x < n; x++)
+            t[(size_t)z * n2 + x] = (z < nzf / 2.0) ? (float)c->p.t_peltier1 : (float)c->p.t_peltier2;
+    for (int b = 0; b < 2; b++)
+        HIP_TRY(hipMemcpyAsync(c->T[b], t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(c->Q, 0, t.size() * sizeof(float), c->stream));   // initQ0 (:261-274)
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->tcur = 0;
+    return SHG_OK;
+}
+
+int timed_march(shg_ctx *c) {
+    if (!c->p.thermal && c->use_graph && !c->gexec) {
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_march_plain(c, 0, c->steps);
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (rc) return rc;
+        if (ee != hipSuccess) return fail(SHG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(SHG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    HIP_TRY(hipEventRecord(c->e0, c->stream));
+    if (c->p.thermal) {
+        RC_TRY(enqueue_march_thermal(c));
+    } else if (c->use_graph) {
+        HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        RC_TRY(enqueue_march_plain(c, 0, c->steps));
+    }
+    HIP_TRY(hipEventRecord(c->e1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+    c->last_ms = ms;
+    return SHG_OK;
 }
 
 }  // namespace
@@ -141,7 +288,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     if (!p || !out) return fail(SHG_E_INVALID, "shg_create: null argument");
     *out = nullptr;
     if (p->nx != p->ny) return fail(SHG_E_INVALID, "shg_create: nx must equal ny");
-    if (!pow2_supported(p->nx)) return fail(SHG_E_INVALID, "shg_create: nx must be one of 64, 128, 256, 512");
+    if (!size_supported(p->nx)) return fail(SHG_E_INVALID, "shg_create: nx must be one of 64, 128, 256, 512");
     if (p->nz < 2) return fail(SHG_E_INVALID, "shg_create: nz must be >= 2");
     if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return fail(SHG_E_INVALID, "shg_create: box lengths must be > 0");
     if (!(p->lambda_p > 0)) return fail(SHG_E_INVALID, "shg_create: lambda_p must be > 0");
@@ -149,12 +296,17 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         return fail(SHG_E_INVALID, "shg_create: unknown crystal");
     if (p->scheme != SHG_SCHEME_REFERENCE && p->scheme != SHG_SCHEME_STRANG)
         return fail(SHG_E_INVALID, "shg_create: unknown scheme");
-    if (p->thermal) return fail(SHG_E_INVALID, "shg_create: THERMAL contexts use shg_thermal_create");
+    if (p->thermal) {
+        const int hz = p->heat_nz > 0 ? p->heat_nz : p->nz;
+        if (hz < 2 || hz > p->nz) return fail(SHG_E_INVALID, "shg_create: heat_nz must be in [2, nz]");
+        if (!(p->k_th > 0)) return fail(SHG_E_INVALID, "shg_create: k_th must be > 0");
+    }
 
     shg_ctx *c = new shg_ctx();
     c->p = *p;
     c->n = p->nx;
-    c->steps = p->nz - 1;
+    c->steps = p->thermal ? p->nz : p->nz - 1;
+    c->hz = p->thermal ? (p->heat_nz > 0 ? p->heat_nz : p->nz) : 0;
     const char *g = std::getenv("SHG_NO_GRAPH");
     c->use_graph = (g && g[0] == '1') ? 0 : 1;
     const size_t ncell = (size_t)c->n * c->n;
@@ -162,12 +314,11 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return bad(fail(SHG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    c->nparts = 256;
     if (hipMalloc(&c->U, ncell * sizeof(float4)) != hipSuccess || hipMalloc(&c->U0, ncell * sizeof(float4)) != hipSuccess ||
         hipMalloc(&c->tw, (size_t)c->n * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&c->G, 4 * (size_t)c->n * sizeof(float2)) != hipSuccess)
-        return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-    c->nparts = 256;
-    if (hipMalloc(&c->part, c->nparts * sizeof(double2)) != hipSuccess || hipMalloc(&c->red, sizeof(double2)) != hipSuccess)
+        hipMalloc(&c->G, 4 * (size_t)c->n * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&c->part, c->nparts * sizeof(double2)) != hipSuccess || hipMalloc(&c->red, sizeof(double2)) != hipSuccess)
         return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
     if (hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
         return bad(fail(SHG_E_DEVICE, "shg_create: hipEventCreate failed"));
@@ -175,18 +326,39 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     const shg::phys::Consts k(*p);
     const float T = (float)p->temp;
     const auto tw = shg::phys::twiddle_table(c->n);
-    const auto G = shg::phys::propagator_factors(k, T, (double)k.dz);
-    c->coef = shg::phys::step_coefficients(k, T, c->steps);
+    if (hipMemcpy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     const float h = p->scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
-    const float ns = k.n_s(T);
     c->nl.lossP = (float)(0.5 * (double)k.alpha_p);
     c->nl.lossS = (float)(0.5 * (double)k.alpha_s);
-    c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)ns));
+    c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
     c->nl.h6 = h / 6.0f;
-    if (hipMemcpy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->G, G.data(), G.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-        return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     c->host_stage.resize(4 * ncell);
+    if (!p->thermal) {
+        c->coef = shg::phys::step_coefficients(k, T, c->steps);
+    } else {
+        const size_t n3 = ncell * c->hz;
+        c->njpart = shg::jacobi_blocks(c->n, c->n, c->hz);
+        if (hipMalloc(&c->Uout, ncell * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
+            hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
+            return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
+        const float hdz = shg::phys::heat_dz(k, c->hz);
+        c->jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p->k_th, (float)p->h_conv, (float)p->t_inf);
+        shg::ThermConst &tc = c->tc;
+        tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
+        tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
+        tc.tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
+        tc.q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
+        tc.q_s = (double)k.alpha_s * 0.5 * (double)k.c * (double)k.eps0;
+        tc.q_i = 0.5 * (double)k.c * (double)k.eps0;
+        tc.beta_s = k.beta_s;
+        tc.crystal = p->crystal;
+        if (int rc = thermal_init_temperature(c)) return bad(rc);
+    }
+    if (int rc = upload_propagators(c, T)) return bad(rc);
     *out = c;
     return SHG_OK;
 }
@@ -204,21 +376,12 @@ int shg_make_pump(const shg_params *p, double t_centre, float *fh) {
 int shg_set_fields(shg_ctx *c, const float *fh, const float *sh) {
     if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_set_fields: null argument");
     const size_t ncell = (size_t)c->n * c->n;
-    float *h = c->host_stage.data();
-    for (size_t i = 0; i < ncell; i++) {
-        h[4 * i] = fh[2 * i];
-        h[4 * i + 1] = fh[2 * i + 1];
-        h[4 * i + 2] = sh[2 * i];
-        h[4 * i + 3] = sh[2 * i + 1];
-    }
-    HIP_TRY(hipMemcpyAsync(c->U0, h, ncell * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    RC_TRY(set_u_from_host(c, c->U0, fh, sh));
     HIP_TRY(hipMemcpyAsync(c->U, c->U0, ncell * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
     double sf, ss;
-    int rc = reduce_powers(c, c->U0, &sf, &ss);
-    if (rc) return rc;
+    RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
     if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_set_fields: non-finite input field");
-    c->sum_fh_in = sf;
-    c->p_fh_in = power_of(c, 0, sf);
+    c->p_fh_in = power_of(c, 0, sf, c->t_centre);
     c->have_fields = true;
     return SHG_OK;
 }
@@ -226,34 +389,26 @@ int shg_set_fields(shg_ctx *c, const float *fh, const float *sh) {
 int shg_forward(shg_ctx *c) {
     if (!c) return fail(SHG_E_INVALID, "shg_forward: null context");
     if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward: call shg_set_fields first");
-    const size_t ncell = (size_t)c->n * c->n;
-    HIP_TRY(hipMemcpyAsync(c->U, c->U0, ncell * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    if (c->use_graph && !c->gexec) {
-        hipGraph_t graph;
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_march(c, 0, c->steps);
-        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-        if (rc) return rc;
-        if (ee != hipSuccess) return fail(SHG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-        ee = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ee != hipSuccess) {
-            c->gexec = nullptr;
-            return fail(SHG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ee));
-        }
-    }
-    HIP_TRY(hipEventRecord(c->e0, c->stream));
-    if (c->use_graph) {
-        HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
-    } else {
-        int rc = enqueue_march(c, 0, c->steps);
-        if (rc) return rc;
-    }
-    HIP_TRY(hipEventRecord(c->e1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->e1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
-    c->last_ms = ms;
+    HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    return timed_march(c);
+}
+
+int shg_reset_fields(shg_ctx *c) {
+    if (!c) return fail(SHG_E_INVALID, "shg_reset_fields: null context");
+    if (!c->have_fields) return fail(SHG_E_STATE, "shg_reset_fields: call shg_set_fields first");
+    HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHG_OK;
+}
+
+int shg_march(shg_ctx *c, int first_step, int nsteps) {
+    if (!c) return fail(SHG_E_INVALID, "shg_march: null context");
+    if (c->p.thermal) return fail(SHG_E_INVALID, "shg_march: THERMAL contexts march whole solves (shg_forward)");
+    if (!c->have_fields) return fail(SHG_E_STATE, "shg_march: call shg_set_fields first");
+    if (first_step < 0 || nsteps < 0 || first_step + nsteps > c->steps)
+        return fail(SHG_E_INVALID, "shg_march: step range outside [0, nz-1)");
+    RC_TRY(enqueue_march_plain(c, first_step, nsteps));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return SHG_OK;
 }
 
@@ -261,7 +416,8 @@ int shg_get_fields(shg_ctx *c, float *fh, float *sh) {
     if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_get_fields: null argument");
     const size_t ncell = (size_t)c->n * c->n;
     float *h = c->host_stage.data();
-    HIP_TRY(hipMemcpyAsync(h, c->U, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    const float4 *src = c->p.thermal ? c->Uout : c->U;
+    HIP_TRY(hipMemcpyAsync(h, src, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < ncell; i++) {
         fh[2 * i] = h[4 * i];
@@ -276,10 +432,9 @@ int shg_powers(shg_ctx *c, double *p_fh_in, double *p_sh_out, double *eta) {
     if (!c) return fail(SHG_E_INVALID, "shg_powers: null context");
     if (!c->have_fields) return fail(SHG_E_STATE, "shg_powers: no fields");
     double sf, ss;
-    int rc = reduce_powers(c, c->U, &sf, &ss);
-    if (rc) return rc;
+    RC_TRY(reduce_powers(c, c->p.thermal ? c->Uout : c->U, &sf, &ss));
     if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_powers: non-finite field");
-    const double psh = power_of(c, 1, ss);
+    const double psh = power_of(c, 1, ss, c->t_centre);
     if (p_fh_in) *p_fh_in = c->p_fh_in;
     if (p_sh_out) *p_sh_out = psh;
     if (eta) *eta = c->p_fh_in > 0 ? psh / c->p_fh_in : 0.0;
@@ -294,19 +449,26 @@ int shg_last_forward_ms(shg_ctx *c, double *ms) {
 
 int64_t shg_cell_steps(const shg_ctx *c) { return c ? (int64_t)c->n * c->n * c->steps : 0; }
 
-// Average device time of one row pass (the mid-march form: R_fwd, two RK4, R_bwd) and
-// one column pass, each launched `reps` times back to back between two events on the
-// context stream.  The field values are overwritten; the next shg_forward restores them.
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
-    shg::RowArgs a{};
-    a.nl = c->nl;
-    a.cprev = coef_at(c, 0);
-    a.ccur = coef_at(c, c->steps > 1 ? 1 : 0);
-    a.flags = shg::ROW_FWD | shg::ROW_NPREV | shg::ROW_NCUR | shg::ROW_BWD;
     float ms = 0.f;
     HIP_TRY(hipEventRecord(c->e0, c->stream));
-    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->U, c->tw, a, c->stream));
+    if (c->p.thermal) {
+        shg::ThermRowArgs a{};
+        a.k = c->tc;
+        a.prev = therm_step(c, 0);
+        a.cur = therm_step(c, 1);
+        a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
+        for (int r = 0; r < reps; r++)
+            HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->T[c->tcur], c->Q, c->DKint, a, c->stream));
+    } else {
+        shg::RowArgs a{};
+        a.nl = c->nl;
+        a.cprev = coef_at(c, 0);
+        a.ccur = coef_at(c, c->steps > 1 ? 1 : 0);
+        a.flags = shg::ROW_FWD | shg::ROW_NPREV | shg::ROW_NCUR | shg::ROW_BWD;
+        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->U, c->tw, a, c->stream));
+    }
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
@@ -317,6 +479,121 @@ int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
     *col_ms = ms / reps;
+    return SHG_OK;
+}
+
+// ----- THERMAL --------------------------------------------------------------
+
+int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st) {
+    if (!c) return fail(SHG_E_INVALID, "shg_thermal_run: null context");
+    if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_thermal_run: context was created with thermal = 0");
+    if (!c->have_fields) return fail(SHG_E_STATE, "shg_thermal_run: call shg_set_fields first (SH input)");
+    const shg::phys::Consts k(c->p);
+    const size_t ncell = (size_t)c->n * c->n, n3 = ncell * c->hz;
+    const int check = c->p.jacobi_check > 0 ? c->p.jacobi_check : 5000;
+    const int num_of_iter = c->p.max_outer > 0 ? c->p.max_outer : 5;
+    const double tol = c->p.jacobi_tol > 0 ? (double)(float)c->p.jacobi_tol : (double)(float)(5e-4 / (double)n3);
+    RC_TRY(thermal_init_temperature(c));
+    // the caller's z = 0 fields (SH input; FH used when pump_from_temperature == 0)
+    std::vector<float> fh0(2 * ncell), sh0(2 * ncell), stage(4 * ncell);
+    HIP_TRY(hipMemcpyAsync(stage.data(), c->U0, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < ncell; i++) {
+        fh0[2 * i] = stage[4 * i]; fh0[2 * i + 1] = stage[4 * i + 1];
+        sh0[2 * i] = stage[4 * i + 2]; sh0[2 * i + 1] = stage[4 * i + 3];
+    }
+    shg_thermal_stats s{};
+    double jac_ms = 0.0, march_ms = 0.0;
+    int global_count = 0;
+    while (global_count < num_of_iter) {
+        // Jacobi to steady state, residual checked every `check` sweeps (src/headers/Solver.h:302-325)
+        double reduced = 1.0, aux = 0.0;
+        int64_t counter = 0;
+        float ms = 0.f;
+        HIP_TRY(hipEventRecord(c->e0, c->stream));
+        while (reduced >= tol) {
+            if (counter % check == 0) {
+                double r = 0.0;
+                RC_TRY(jacobi_run(c, 1, true, &r));
+                if (counter > 0) aux = reduced;
+                reduced = r;
+                if (counter > 0 && reduced == aux) reduced = 0.5 * tol;
+                s.residual_last = r;
+                counter++;
+            } else {
+                const int64_t next_check = (counter / check + 1) * (int64_t)check;
+                int64_t burst = next_check - counter;
+                if (c->p.jacobi_max > 0) burst = std::min<int64_t>(burst, c->p.jacobi_max - counter);
+                RC_TRY(jacobi_run(c, burst, false, nullptr));
+                counter += burst;
+            }
+            if (c->p.jacobi_max > 0 && counter >= c->p.jacobi_max) break;
+        }
+        HIP_TRY(hipEventRecord(c->e1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->e1));
+        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+        jac_ms += ms;
+        s.sweeps_last = counter;
+        s.sweeps_total += counter;
+        // fields and propagators at the temperature of the grid centre, slice 0 (Solver.h:329-331)
+        float tcen = 0.f;
+        HIP_TRY(hipMemcpy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
+                          hipMemcpyDeviceToHost));
+        if (pump_from_temperature)
+            shg::phys::make_pump(k, (float)c->p.power, (float)c->p.focus, (float)c->p.waist, tcen, fh0.data());
+        RC_TRY(set_u_from_host(c, c->U, fh0.data(), sh0.data()));
+        RC_TRY(upload_propagators(c, tcen));
+        RC_TRY(timed_march(c));
+        march_ms += c->last_ms;
+        s.outer_iterations++;
+        if (counter == 1 && global_count != 0) global_count = num_of_iter;   // (Solver.h:336)
+        global_count++;
+    }
+    // input power of the last outer iteration's pump
+    RC_TRY(set_u_from_host(c, c->U0, fh0.data(), sh0.data()));
+    double sf, ss;
+    RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
+    c->p_fh_in = power_of(c, 0, sf, c->t_centre);
+    s.jacobi_ms = jac_ms;
+    s.march_ms = march_ms;
+    if (st) *st = s;
+    return SHG_OK;
+}
+
+int shg_jacobi(shg_ctx *c, int64_t nsweeps, double *mean_sq_update) {
+    if (!c || nsweeps < 0) return fail(SHG_E_INVALID, "shg_jacobi: bad argument");
+    if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_jacobi: context was created with thermal = 0");
+    double r = 0.0;
+    RC_TRY(jacobi_run(c, nsweeps, mean_sq_update != nullptr, &r));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (mean_sq_update) *mean_sq_update = r;
+    return SHG_OK;
+}
+
+int shg_get_temperature(shg_ctx *c, float *T) {
+    if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_temperature: bad argument");
+    HIP_TRY(hipMemcpy(T, c->T[c->tcur], (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost));
+    return SHG_OK;
+}
+
+int shg_set_temperature(shg_ctx *c, const float *T) {
+    if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_temperature: bad argument");
+    const size_t bytes = (size_t)c->n * c->n * c->hz * sizeof(float);
+    HIP_TRY(hipMemcpy(c->T[0], T, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->T[1], T, bytes, hipMemcpyHostToDevice));
+    c->tcur = 0;
+    return upload_propagators(c, T[(size_t)(c->n / 2) * c->n + c->n / 2]);
+}
+
+int shg_get_heat_source(shg_ctx *c, float *Q) {
+    if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_heat_source: bad argument");
+    HIP_TRY(hipMemcpy(Q, c->Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost));
+    return SHG_OK;
+}
+
+int shg_set_heat_source(shg_ctx *c, const float *Q) {
+    if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_heat_source: bad argument");
+    HIP_TRY(hipMemcpy(c->Q, Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyHostToDevice));
     return SHG_OK;
 }
 

@@ -15,55 +15,15 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_lds.h"
+#include "nl_step.h"
+#include "pass_cfg.h"
 #include "ssfm_kernels.h"
 
 namespace shg {
 
-// Right-hand side of the coupled-wave equations at one cell (dAdz, plain build,
-// src/headers/Solver.h:65-83), with the step constants folded:
-//   dP = c1 * S conj(P) - lossP * P,          c1 = i kappa_p e^{-i theta}
-//   dS = c2 * P^2 - (lossS + tpa |S|^2) * S,  c2 = i kappa_s e^{+i theta}
-__device__ __forceinline__ void nl_rhs(float2 P, float2 S, float4 c, const NlConst &k, float2 &dP, float2 &dS) {
-    const float ur = S.x * P.x + S.y * P.y, ui = S.y * P.x - S.x * P.y;
-    dP.x = c.x * ur - c.y * ui - k.lossP * P.x;
-    dP.y = c.x * ui + c.y * ur - k.lossP * P.y;
-    const float vr = P.x * P.x - P.y * P.y, vi = 2.0f * P.x * P.y;
-    const float g = k.lossS + k.tpa * (S.x * S.x + S.y * S.y);
-    dS.x = c.z * vr - c.w * vi - g * S.x;
-    dS.y = c.z * vi + c.w * vr - g * S.y;
-}
-
-// solverRK4 (src/headers/Solver.h:193-233): stages A + {.5,.5,1} k (not scaled by the
-// step), final A += h/6 (k1 + 2k2 + 2k3 + k4).
 __device__ __forceinline__ void rk4(float2 &P, float2 &S, float4 c, const NlConst &k) {
-    float2 k1p, k1s, k2p, k2s, k3p, k3s, k4p, k4s;
-    nl_rhs(P, S, c, k, k1p, k1s);
-    nl_rhs(make_float2(P.x + 0.5f * k1p.x, P.y + 0.5f * k1p.y), make_float2(S.x + 0.5f * k1s.x, S.y + 0.5f * k1s.y), c,
-           k, k2p, k2s);
-    nl_rhs(make_float2(P.x + 0.5f * k2p.x, P.y + 0.5f * k2p.y), make_float2(S.x + 0.5f * k2s.x, S.y + 0.5f * k2s.y), c,
-           k, k3p, k3s);
-    nl_rhs(make_float2(P.x + k3p.x, P.y + k3p.y), make_float2(S.x + k3s.x, S.y + k3s.y), c, k, k4p, k4s);
-    P.x += k.h6 * (k1p.x + 2.0f * k2p.x + 2.0f * k3p.x + k4p.x);
-    P.y += k.h6 * (k1p.y + 2.0f * k2p.y + 2.0f * k3p.y + k4p.y);
-    S.x += k.h6 * (k1s.x + 2.0f * k2s.x + 2.0f * k3s.x + k4s.x);
-    S.y += k.h6 * (k1s.y + 2.0f * k2s.y + 2.0f * k3s.y + k4s.y);
+    rk4_cell(P, S, c, c, c, k.lossP, k.lossS, k.tpa, k.h6);
 }
-
-template <int N> struct RowCfg {
-    static constexpr int E = FftPlan<N>::E;
-    static constexpr int T = N / E;               // threads per row
-    static constexpr int THREADS = 64;            // one wave per workgroup
-    static constexpr int ROWS = THREADS / T;      // rows per workgroup
-    static constexpr int L = LdsLen<N>::value;
-};
-
-template <int N> struct ColCfg {
-    static constexpr int E = FftPlan<N>::E;
-    static constexpr int T = N / E;               // threads per column
-    static constexpr int CW = N == 512 ? 4 : 8;   // columns per workgroup
-    static constexpr int THREADS = CW * T;
-    static constexpr int L = LdsLen<N>::value;
-};
 
 __device__ __forceinline__ void block_sync() { __syncthreads(); }
 

@@ -1,0 +1,59 @@
+// thermal_kernels.h — launch interface of the THERMAL-mode kernels (thermal_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ssfm_kernels.h"
+
+namespace shg {
+
+// Weights of one Jacobi sweep of heatEquationTopOpen (src/headers/Tfield.h:118-125),
+// evaluated on the host with the reference's float/double typing.
+struct JacobiConst {
+    float dxx, dyy, dzz, L2, kth;                 // inner points
+    float ax, bx, fdenx, bixT;                    // x faces
+    float ay, by, fdeny, biyT;                    // top face
+    float az, bz, fdenz, bizT;                    // z faces
+    float av, bv, cv, ah, bh, ch, ad, bd, cd;     // edges
+    float AV, BV, CV, GV, tinfBiD;                // vertices
+    double V, H, D, sv, sh, sd;                   // double parts of the edge formulas
+};
+
+// Per-cell physics of the THERMAL nonlinear step (src/headers/Solver.h:44-63) and setQ
+// (src/headers/Tfield.h:277-290).
+struct ThermConst {
+    float pi, dq, lp, ls, inv_period;             // dk = 2 pi (2 n_p/lp - n_s/ls + 1/Lambda)
+    float lossP, lossS, h6;                       // 0.5 alpha_p, 0.5 alpha_s, RK4 step / 6
+    double tpa_base;                              // beta_s * 0.5 * eps0 * c   (times n_s(T))
+    double q_p, q_s, q_i;                         // alpha_p*0.5*c*eps0, alpha_s*0.5*c*eps0, 0.5*c*eps0
+    float beta_s;
+    int crystal;
+};
+
+// Step-dependent scalars of one thermal RK4 application at field step s: the phase is
+// DKint_s * zfac[k] / (s + 1) with zfac[k] = dz*inc_k + (s+1)*dz, inc in {0, .25, .5}.
+struct ThermStep {
+    double zfac[3];
+    double sp1;                                   // s + 1
+    int slice;                                    // heat slice of step s
+};
+
+enum ThermRowFlags : int {
+    TROW_FWD = 1, TROW_NPREV = 2, TROW_QPREV = 4, TROW_SNAP = 8, TROW_NCUR = 16, TROW_BWD = 32, TROW_FIRST = 64
+};
+
+struct ThermRowArgs {
+    ThermStep prev, cur;
+    ThermConst k;
+    int flags;
+};
+
+hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
+                         double *part, int nparts, hipStream_t st);
+hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
+hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q,
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st);
+// Grid size the Jacobi launcher uses (number of residual partials it writes).
+int jacobi_blocks(int nx, int ny, int hz);
+
+}  // namespace shg

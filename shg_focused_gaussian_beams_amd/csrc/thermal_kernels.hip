@@ -1,0 +1,260 @@
+// thermal_kernels.hip — THERMAL mode on gfx950: the steady-state heat-equation Jacobi
+// sweep, and the row pass whose nonlinear step reads the temperature, accumulates the
+// phase mismatch and writes the heat source.
+//
+// Arithmetic that the reference evaluates in float (the Jacobi weights and sums, the
+// refractive index and the phase mismatch) is evaluated here with FP contraction off
+// and the same float/double typing, so a sweep and a dk(T) match the reference's
+// float results bit for bit; the mismatch dk is a difference of O(4) terms whose
+// rounding is amplified about a thousandfold.
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.h"
+#include "nl_step.h"
+#include "pass_cfg.h"
+#include "thermal_kernels.h"
+#include "../../include/shg.h"
+
+namespace shg {
+
+// ---------------------------------------------------------------------------
+// One Jacobi sweep Tf <- J(Ti, Q) of heatEquationTopOpen (src/headers/Tfield.h:113-221).
+// Thread per voxel; the y = 0 plane (Dirichlet ovens) is never written.  With RESID the
+// FP64 sum of (Tf - Ti)^2 over the block goes to part[block].
+template <bool RESID>
+__global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
+                                                double *__restrict__ part) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    double r = 0.0;
+    if (x < nx && y >= 1 && y < ny) {
+        const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * ny;
+        const size_t i = (size_t)z * sz + (size_t)y * sy + x;
+        const bool xi = x > 0 && x < nx - 1, zi = z > 0 && z < hz - 1, yi = y < ny - 1;
+        const size_t ixn = x == 0 ? i + sx : i - sx;       // the one x-neighbour of an x face
+        const size_t izn = z == 0 ? i + sz : i - sz;
+        float out;
+        if (yi) {
+            if (xi && zi) {
+                out = c.L2 * ((Ti[i + sx] + Ti[i - sx]) / c.dxx + (Ti[i + sy] + Ti[i - sy]) / c.dyy +
+                              (Ti[i + sz] + Ti[i - sz]) / c.dzz + Q[i] / c.kth);
+            } else if (!xi && zi) {
+                out = (Ti[ixn] + c.ax * (Ti[i + sy] + Ti[i - sy]) + c.bx * (Ti[i + sz] + Ti[i - sz]) + c.bixT) / c.fdenx;
+            } else if (xi && !zi) {
+                out = (Ti[izn] + c.az * (Ti[i + sx] + Ti[i - sx]) + c.bz * (Ti[i + sy] + Ti[i - sy]) + c.bizT) / c.fdenz;
+            } else {
+                const float f = c.av * Ti[ixn] + c.bv * Ti[izn] + c.cv * (Ti[i + sy] + Ti[i - sy]);
+                out = (float)(((double)f + c.sv) * c.V);
+            }
+        } else {
+            if (xi && zi) {
+                out = (Ti[i - sy] + c.ay * (Ti[i + sx] + Ti[i - sx]) + c.by * (Ti[i + sz] + Ti[i - sz]) + c.biyT) / c.fdeny;
+            } else if (xi && !zi) {
+                const float f = c.ah * Ti[i - sy] + c.bh * Ti[izn] + c.ch * (Ti[i - sx] + Ti[i + sx]);
+                out = (float)(((double)f + c.sh) * c.H);
+            } else if (!xi && zi) {
+                const float f = c.ad * Ti[i - sy] + c.bd * Ti[ixn] + c.cd * (Ti[i + sz] + Ti[i - sz]);
+                out = (float)(((double)f + c.sd) * c.D);
+            } else {
+                out = (c.AV * Ti[ixn] + c.BV * Ti[i - sy] + c.CV * Ti[izn] + c.tinfBiD) * c.GV;
+            }
+        }
+        Tf[i] = out;
+        if (RESID) {
+            const double d = (double)(out - Ti[i]);
+            r = d * d;
+        }
+    }
+    if (RESID) {
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        __shared__ double w[4];
+        if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = r;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ part, int n, double *out) {
+    __shared__ double s[256];
+    double r = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) r += part[i];
+    s[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// ---------------------------------------------------------------------------
+// Refractive index with the reference's typing (src/headers/Crystal.h:55-71).
+__device__ __forceinline__ float index_dev(int crystal, float L, float T) {
+#pragma clang fp contract(off)
+    if (crystal == SHG_CRYSTAL_PPLN) {
+        const double t = (double)T, l2 = (double)L * (double)L;
+        const double f = (t - 24.5) * (t + 570.82);
+        const double p = 0.2020 + 6.113e-8 * f;
+        return (float)sqrt(5.756 + 2.860e-6 * f + (0.0983 + 4.700e-8 * f) / (l2 - p * p) +
+                           (189.32 + 1.516e-4 * f) / (l2 - 12.52 * 12.52) - 1.32e-2 * l2);
+    }
+    const float A = 4.502483f, B = 0.007294f, C = 0.185087f, D = -0.02357f;
+    const float E = 0.073423f, F = 0.199595f, G = 0.001f, H = 7.99724f;
+    const double tk = (double)T + 273.15;
+    const float b = (float)(3.483933e-8 * (tk * tk));
+    const float c = (float)(1.607839e-8 * (tk * tk));
+    const double l2 = (double)L * (double)L;
+    const float bb = B + b, cc = C + c;
+    const double s = (double)A + (double)bb / (l2 - (double)cc * (double)cc) + (double)E / (l2 - (double)F * (double)F) +
+                     (double)G / (l2 - (double)H * (double)H) + (double)D * l2;
+    return (float)sqrt(s);
+}
+
+struct CellPhys {
+    float np, ns, kp, ks, dk, tpa;
+};
+
+// n_p, n_s, the coupling magnitudes, the phase mismatch (src/headers/PhaseMatching.h:47)
+// and the TPA coefficient at temperature T.
+__device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float T) {
+#pragma clang fp contract(off)
+    CellPhys c;
+    c.np = index_dev(k.crystal, k.lp, T);
+    c.ns = index_dev(k.crystal, k.ls, T);
+    c.kp = ((1.0f * k.pi) * k.dq) / (c.np * k.lp);
+    c.ks = ((0.5f * k.pi) * k.dq) / (c.ns * k.ls);
+    c.dk = (2.0f * k.pi) * (((2.0f * c.np) / k.lp - c.ns / k.ls) + k.inv_period);
+    c.tpa = (float)(0.5 * (k.tpa_base * (double)c.ns));
+    return c;
+}
+
+// (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}), phi = (float)(DKint * zfac / (s+1)).
+__device__ __forceinline__ float4 phase_coef(const CellPhys &c, float dkint, double zfac, double sp1) {
+    const float phi = (float)(((double)dkint * zfac) / sp1);
+    float sn, cs;
+    sincosf(phi, &sn, &cs);
+    return make_float4(c.kp * sn, c.kp * cs, -c.ks * sn, c.ks * cs);
+}
+
+// THERMAL solverRK4 (src/headers/Solver.h:197-213): stage increments 0, .25, .25, .5.
+__device__ __forceinline__ void rk4_thermal(float2 &P, float2 &S, const CellPhys &c, float dkint, const ThermStep &st,
+                                            const ThermConst &k) {
+    const float4 ca = phase_coef(c, dkint, st.zfac[0], st.sp1);
+    const float4 cb = phase_coef(c, dkint, st.zfac[1], st.sp1);
+    const float4 cc = phase_coef(c, dkint, st.zfac[2], st.sp1);
+    rk4_cell(P, S, ca, cb, cc, k.lossP, k.lossS, c.tpa, k.h6);
+}
+
+// setQ (src/headers/Tfield.h:277-290): Q = ap Ip + as Is + bs Is^2 with I_j = c eps0 n_j |A_j|^2 / 2.
+__device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys &c, float2 P, float2 S) {
+#pragma clang fp contract(off)
+    const float p2 = P.x * P.x + P.y * P.y, s2 = S.x * S.x + S.y * S.y;
+    const double ip = k.q_p * (double)c.np * (double)p2;
+    const double is = k.q_s * (double)c.ns * (double)s2;
+    const float i2 = (float)(k.q_i * (double)c.ns * (double)s2);
+    const float tpa = k.beta_s * (i2 * i2);
+    return (float)(ip + is + (double)tpa);
+}
+
+// Row pass of the THERMAL z-march (flags in ThermRowFlags):
+//   FWD   R_fwd (end of the previous step's diffraction)
+//   NPREV second RK4 of step s-1 with T(slice(s-1)) and DKint_{s-1}
+//   QPREV heat source of slice(s-1) from the fields after step s-1
+//   SNAP  copy of the fields after step s-1 to Uout (the reference's Pump_out state)
+//   NCUR  DKint_s = DKint_{s-1} + dk(T(slice(s))), then the first RK4 of step s
+//   BWD   R_bwd (start of step s's diffraction)
+// DKint holds the running float prefix sum of dk over the executed steps: the reference
+// recomputes the same float sum from z' = 0 twice per step (src/headers/PhaseMatching.h:
+// 25-38, Solver.h:198-199); the sequential float additions are identical.
+template <int N>
+__global__ __launch_bounds__(64) void k_therm_row_pass(float4 *__restrict__ U, float4 *__restrict__ Uout,
+                                                       const float2 *__restrict__ twg, const float *__restrict__ T,
+                                                       float *__restrict__ Q, float *__restrict__ DKint, ThermRowArgs a) {
+    using C = RowCfg<N>;
+    constexpr int E = C::E, TT = C::T;
+    __shared__ float2 lds[2][C::ROWS][C::L];
+    __shared__ float2 tw[N];
+    const int lane = threadIdx.x;
+    const int t = lane % TT, rr = lane / TT;
+    const int row = blockIdx.x * C::ROWS + rr;
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    const size_t n2 = (size_t)N * N;
+    const size_t base = (size_t)row * N;
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = U[base + t + j * TT];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { __syncthreads(); };
+    const ThermConst &k = a.k;
+    if (a.flags & TROW_FWD) fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+#pragma unroll 2
+    for (int j = 0; j < E; j++) {
+        const size_t idx = base + t + j * TT;
+        const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
+        if (a.flags & (TROW_NPREV | TROW_QPREV)) {
+            const CellPhys cp = cell_phys(k, T[(size_t)a.prev.slice * n2 + idx]);
+            if (a.flags & TROW_NPREV) rk4_thermal(fa[j], fb[j], cp, dkprev, a.prev, k);
+            if (a.flags & TROW_QPREV) Q[(size_t)a.prev.slice * n2 + idx] = heat_source(k, cp, fa[j], fb[j]);
+        }
+        if (a.flags & TROW_SNAP) Uout[idx] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+        if (a.flags & TROW_NCUR) {
+            const CellPhys cc = cell_phys(k, T[(size_t)a.cur.slice * n2 + idx]);
+            const float dk = dkprev + cc.dk;
+            DKint[idx] = dk;
+            rk4_thermal(fa[j], fb[j], cc, dk, a.cur, k);
+        }
+    }
+    if (a.flags & TROW_BWD) fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) U[base + t + j * TT] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// ---------------------------------------------------------------------------
+int jacobi_blocks(int nx, int ny, int hz) { return (nx / 64) * ((ny + 3) / 4) * hz; }
+
+hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
+                         double *part, int nparts, hipStream_t st) {
+    if (nx % 64 != 0 || hz < 2 || ny < 2) return hipErrorInvalidValue;
+    const dim3 grid(nx / 64, (ny + 3) / 4, hz);
+    if (part) {
+        if (nparts < jacobi_blocks(nx, ny, hz)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    } else {
+        hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, nparts, out);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t therm_row_launch(float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q, float *DKint,
+                                   const ThermRowArgs &a, hipStream_t st) {
+    using C = RowCfg<N>;
+    hipLaunchKernelGGL(k_therm_row_pass<N>, dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, Uout, tw, T, Q, DKint, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q,
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st) {
+    switch (n) {
+        case 64: return therm_row_launch<64>(U, Uout, tw, T, Q, DKint, a, st);
+        case 128: return therm_row_launch<128>(U, Uout, tw, T, Q, DKint, a, st);
+        case 256: return therm_row_launch<256>(U, Uout, tw, T, Q, DKint, a, st);
+        case 512: return therm_row_launch<512>(U, Uout, tw, T, Q, DKint, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace shg

@@ -1,0 +1,92 @@
+"""CPU-side checks of the product: the C ABI library loads and exports every function
+include/shg.h declares, argument validation fails before any device call, the host-side
+reference arithmetic (initial pump) matches the oracle bit for bit, the device FFT's
+index logic is correct in a host build, and the SHG program rejects bad command lines."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+import shg_focused_gaussian_beams_amd as shg
+from shg_focused_gaussian_beams_amd.params import default_params, config1_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(shg_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(shg.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 20, names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_default_params_match_reference_constants():
+    lib = shg.load_library()
+    p = shg.ShgParams()
+    lib.shg_default_params(ctypes.byref(p))
+    q = default_params()
+    for name, _ in shg.ShgParams._fields_:
+        assert getattr(p, name) == getattr(q, name), name
+
+
+@pytest.mark.parametrize("bad", [dict(nx=64, ny=128), dict(nx=100, ny=100), dict(nz=1), dict(crystal=7),
+                                 dict(scheme=3), dict(lz=0.0), dict(thermal=1, heat_nz=500)])
+def test_create_rejects_bad_parameters_without_device(bad):
+    lib = shg.load_library()
+    p = default_params(**bad)
+    h = ctypes.c_void_p()
+    rc = lib.shg_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert lib.shg_last_error().decode().startswith("shg_create")
+
+
+@pytest.mark.parametrize("mk", [default_params, config1_params])
+def test_host_pump_is_bit_exact_with_oracle32(mk):
+    p = mk()
+    a = shg.make_pump(p)
+    ref = orc.pump(p, 32)
+    assert np.array_equal(a.astype(np.complex128), ref)
+
+
+def test_device_fft_index_logic_in_host_build(tmp_path):
+    exe = tmp_path / "fft_emul"
+    subprocess.run(["g++", "-std=c++20", "-O1", "-pthread", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "cpu_emul", "fft_emul.cpp")], check=True)
+    for n in (64, 128, 256, 512):
+        for sign in (1, -1):
+            for mode in ("row", "col"):
+                out = subprocess.run([str(exe), str(n), str(sign), mode, "11"], check=True,
+                                     capture_output=True, text=True).stdout
+                d = np.array([list(map(float, l.split())) for l in out.strip().split("\n")])
+                cw = 1 if mode == "row" else (4 if n == 512 else 8)
+                for k in (0, 2):
+                    a = (d[:, k] + 1j * d[:, k + 1]).reshape(n, cw)
+                    o = (d[:, k + 4] + 1j * d[:, k + 5]).reshape(n, cw)
+                    ref = np.fft.ifft(a, axis=0) * n if sign > 0 else np.fft.fft(a, axis=0)
+                    assert np.abs(o - ref).max() / np.abs(ref).max() < 1e-6, (n, sign, mode)
+
+
+def test_cli_rejects_bad_command_lines(tmp_path):
+    exe = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG")
+    if not os.path.exists(exe):
+        pytest.skip("SHG program not built")
+    r = subprocess.run([exe, "30", "29"], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([exe, "30", "29", "55", "55", "1", "0", "--nope=1"], capture_output=True, text=True,
+                       cwd=tmp_path)
+    assert r.returncode == 1 and "unknown option" in r.stderr
