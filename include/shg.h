@@ -167,6 +167,61 @@ int shg_set_temperature(shg_ctx *ctx, const float *T);
 int shg_get_heat_source(shg_ctx *ctx, float *Q);
 int shg_set_heat_source(shg_ctx *ctx, const float *Q);
 
+/* ----- Gradients (new: the reference has none) --------------------------------------- */
+
+typedef struct shg_adjoint_result {
+    double  eta;                    /* P_SH(out) / P_FH(z = 0) of the forward solve */
+    double  d_eta_d_focus;          /* through the reference pump setPump (Efields.h:74-104), per um */
+    double  d_eta_d_temp;           /* total, per degC: crystal path + pump path */
+    double  d_eta_d_temp_crystal;   /* crystal path only (coupling, phase mismatch, TPA, propagator,
+                                       the n of both powers) with the z = 0 fields held fixed */
+    double  forward_ms;             /* device time of the checkpointed forward */
+    double  adjoint_ms;             /* device time of the taped recomputation + adjoint march */
+    int32_t checkpoint_interval;    /* K: a checkpoint every K steps */
+    int32_t segments;
+} shg_adjoint_result;
+
+/* Discrete adjoint of the plain z-march (Solver::CWES, src/headers/Solver.h:245-276) for
+ * J = eta.  Checkpoints every `checkpoint_interval` steps (<= 0: ceil(sqrt(nz-1))), with
+ * per-segment recomputation.  d_eta_d_fh (nx*ny complex64, may be null) receives the
+ * gradient with respect to the z = 0 pump field, dJ = Re sum conj(g) dA.  The focus and
+ * the pump part of the temperature gradient assume the z = 0 pump is setPump at the
+ * context's power, waist, focus and temp (plus additive noise). */
+int shg_adjoint(shg_ctx *ctx, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res);
+
+/* ----- Batched sweeps ---------------------------------------------------------------
+ * Replace the reference's sweep driver, which runs one `./SHG` process per parameter
+ * point (src/SHG.sh:39-81): `count` independent plain instances (each its own grid size
+ * 64..512, nz, waist, power, temperature, focus, ...) march in one device schedule
+ * captured in a hipGraph.  Instances are addressed by their index in `params`. */
+typedef struct shg_batch shg_batch;
+
+int shg_batch_create(const shg_params *params, int count, shg_batch **out);
+void shg_batch_destroy(shg_batch *b);
+
+/* z = 0 fields of every instance on the device: the reference pump (setPump,
+ * src/headers/Efields.h:74-104) plus complex Gaussian noise of standard deviation
+ * noise_rel * |A(0,0)| per component (seeded, reproducible), SH = 0. */
+int shg_batch_init_fields(shg_batch *b, double noise_rel, uint64_t seed);
+
+/* Caller-supplied z = 0 fields of one instance (nx*ny complex64 each). */
+int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *sh);
+
+/* Marches every instance through its crystal; afterwards each holds its Pump_out /
+ * Signal_out state (after its own nz-1 steps). */
+int shg_batch_run(shg_batch *b);
+
+/* Current fields (after shg_batch_run: the outputs) and the z = 0 fields of one instance. */
+int shg_batch_get_fields(shg_batch *b, int index, float *fh, float *sh);
+int shg_batch_get_input(shg_batch *b, int index, float *fh, float *sh);
+
+/* FP64 powers and efficiency per instance (arrays of `count`; any may be null). */
+int shg_batch_powers(shg_batch *b, double *p_fh_in, double *p_sh_out, double *eta);
+
+/* Sum over instances of nx*ny*(nz-1), and the device time of the last shg_batch_run. */
+int64_t shg_batch_cell_steps(const shg_batch *b);
+int shg_batch_last_ms(shg_batch *b, double *ms);
+
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass, each launched `reps` times back to back.  Clobbers the current
  * fields; the next shg_forward starts again from the shg_set_fields state. */

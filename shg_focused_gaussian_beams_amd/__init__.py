@@ -34,6 +34,16 @@ class ThermalStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class AdjointResult(ctypes.Structure):
+    _fields_ = [("eta", ctypes.c_double), ("d_eta_d_focus", ctypes.c_double),
+                ("d_eta_d_temp", ctypes.c_double), ("d_eta_d_temp_crystal", ctypes.c_double),
+                ("forward_ms", ctypes.c_double), ("adjoint_ms", ctypes.c_double),
+                ("checkpoint_interval", ctypes.c_int32), ("segments", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class ShgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -74,6 +84,17 @@ def load_library(path=LIB_PATH):
         "shg_set_temperature": (ctypes.c_int, [vp, fp]),
         "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_set_heat_source": (ctypes.c_int, [vp, fp]),
+        "shg_adjoint": (ctypes.c_int, [vp, ctypes.c_int, fp, ctypes.POINTER(AdjointResult)]),
+        "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
+        "shg_batch_destroy": (None, [vp]),
+        "shg_batch_init_fields": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint64]),
+        "shg_batch_set_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "shg_batch_run": (ctypes.c_int, [vp]),
+        "shg_batch_get_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "shg_batch_get_input": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "shg_batch_powers": (ctypes.c_int, [vp, dp, dp, dp]),
+        "shg_batch_cell_steps": (ctypes.c_int64, [vp]),
+        "shg_batch_last_ms": (ctypes.c_int, [vp, dp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -165,6 +186,14 @@ class Context:
     def cell_steps(self):
         return int(self.lib.shg_cell_steps(self.h))
 
+    def adjoint(self, checkpoint_interval=0, want_field=True):
+        """Efficiency and its gradients (discrete adjoint).  Returns (result dict,
+        d eta / d FH(z=0) as complex64 (ny, nx) or None)."""
+        r = AdjointResult()
+        g = np.empty((self.params.ny, self.params.nx), np.complex64) if want_field else None
+        _check(self.lib.shg_adjoint(self.h, checkpoint_interval, _fptr(g) if want_field else None, ctypes.byref(r)))
+        return r.as_dict(), g
+
     # THERMAL
     def heat_shape(self):
         hz = self.params.heat_nz if self.params.heat_nz > 0 else self.params.nz
@@ -206,6 +235,78 @@ class Context:
         r, c = ctypes.c_double(), ctypes.c_double()
         _check(self.lib.shg_profile_kernels(self.h, reps, ctypes.byref(r), ctypes.byref(c)))
         return r.value, c.value
+
+
+class Batch:
+    """A batched sweep: many independent plain instances marched in one device schedule
+    (replaces the one-process-per-point loop of src/SHG.sh:39-81)."""
+
+    def __init__(self, params_list):
+        self.lib = load_library()
+        self.params = [p.copy() for p in params_list]
+        arr = (ShgParams * len(self.params))(*self.params)
+        h = ctypes.c_void_p()
+        _check(self.lib.shg_batch_create(arr, len(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.shg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init_fields(self, noise_rel=1e-6, seed=1):
+        _check(self.lib.shg_batch_init_fields(self.h, noise_rel, seed))
+
+    def set_fields(self, i, fh, sh):
+        fh = np.ascontiguousarray(fh, np.complex64)
+        sh = np.ascontiguousarray(sh, np.complex64)
+        n = self.params[i].nx * self.params[i].ny
+        if fh.size != n or sh.size != n:
+            raise ValueError(f"instance {i}: fields must hold {n} cells")
+        _check(self.lib.shg_batch_set_fields(self.h, i, _fptr(fh), _fptr(sh)))
+
+    def run(self):
+        _check(self.lib.shg_batch_run(self.h))
+
+    def _get(self, fn, i):
+        shape = (self.params[i].ny, self.params[i].nx)
+        fh = np.empty(shape, np.complex64)
+        sh = np.empty(shape, np.complex64)
+        _check(fn(self.h, i, _fptr(fh), _fptr(sh)))
+        return fh, sh
+
+    def get_fields(self, i):
+        return self._get(self.lib.shg_batch_get_fields, i)
+
+    def get_input(self, i):
+        return self._get(self.lib.shg_batch_get_input, i)
+
+    def powers(self):
+        n = len(self.params)
+        a, b, c = (np.zeros(n) for _ in range(3))
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(self.lib.shg_batch_powers(self.h, a.ctypes.data_as(dp), b.ctypes.data_as(dp), c.ctypes.data_as(dp)))
+        return a, b, c
+
+    def cell_steps(self):
+        return int(self.lib.shg_batch_cell_steps(self.h))
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        _check(self.lib.shg_batch_last_ms(self.h, ctypes.byref(v)))
+        return v.value
 
 
 class Solver:

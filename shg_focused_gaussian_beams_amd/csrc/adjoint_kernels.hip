@@ -1,0 +1,388 @@
+// adjoint_kernels.hip — discrete adjoint of the plain z-march on gfx950.
+//
+// Forward (one checkpoint segment, steps [s0, s1)) with a tape:
+//   row pass writes the real-space state before each step's first RK4 (A_s) and before
+//   its second RK4 (A''_s); column pass writes the 2D spectrum it multiplies by H.
+// Adjoint (reverse step order), with lambda the gradient dJ = Re sum conj(lambda) dA:
+//   row pass  : R_fwd -> RK4_s^T at A_s -> RK4_{s-1}^T at A''_{s-1} -> R_bwd
+//   col pass  : C_bwd -> x conj(H)/N^2 -> C_fwd   (L^H = (1/N^2) F_fwd conj(H) F_bwd)
+// RK4^T recomputes the forward stages from the taped state and runs them backwards; the
+// per-cell maps are not holomorphic (conj(P), |S|^2), so each stage uses the real adjoint
+// zbar = conj(dg/dz) gbar + (dg/dconj z) conj(gbar).  FP64 partial sums per block give the
+// sensitivities of J to the step coefficients (c1, c2, tpa) and to the propagator (for
+// the temperature gradient).
+#include <hip/hip_runtime.h>
+
+#include "adjoint_kernels.h"
+#include "fft_lds.h"
+#include "nl_step.h"
+#include "pass_cfg.h"
+
+namespace shg {
+
+__device__ __forceinline__ float2 cj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 axpy2(float s, float2 a, float2 b) { return make_float2(b.x + s * a.x, b.y + s * a.y); }
+__device__ __forceinline__ float2 scl2(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+
+struct AdjAcc {
+    double c1r, c1i, c2r, c2i, tpa;
+};
+
+__device__ __forceinline__ void acc_zero(AdjAcc &a) { a.c1r = a.c1i = a.c2r = a.c2i = a.tpa = 0.0; }
+
+// Adjoint of the right-hand side f(P, S) at (P, S) applied to (gP, gS).
+__device__ __forceinline__ void rhs_adj(float2 P, float2 S, float2 gP, float2 gS, float4 c, const NlConst &k,
+                                        float2 &Pb, float2 &Sb, AdjAcc &acc) {
+    const float2 c1 = f2(c.x, c.y), c2 = f2(c.z, c.w);
+    const float s2 = S.x * S.x + S.y * S.y;
+    // Pb = -lossP gP + c1 S conj(gP) + 2 conj(c2) conj(P) gS
+    Pb = add2(add2(scl2(-k.lossP, gP), c_mul(c_mul(c1, S), cj(gP))), scl2(2.0f, c_mul(c_mul(cj(c2), cj(P)), gS)));
+    // Sb = conj(c1) P gP - (lossS + 2 tpa |S|^2) gS - tpa S^2 conj(gS)
+    Sb = add2(add2(c_mul(c_mul(cj(c1), P), gP), scl2(-(k.lossS + 2.0f * k.tpa * s2), gS)),
+              scl2(-k.tpa, c_mul(c_mul(S, S), cj(gS))));
+    // coefficient sensitivities: c1b += conj(S conj P) gP, c2b += conj(P^2) gS, tpab += -|S|^2 Re(conj(gS) S)
+    const float2 u = c_mul(cj(S), P);
+    const float2 w = c_mul(cj(c_mul(P, P)), gS);
+    const float2 v = c_mul(u, gP);
+    acc.c1r += (double)v.x; acc.c1i += (double)v.y;
+    acc.c2r += (double)w.x; acc.c2i += (double)w.y;
+    acc.tpa += -(double)s2 * ((double)gS.x * S.x + (double)gS.y * S.y);
+}
+
+// RK4^T at forward input (P, S): (lP, lS) is the adjoint of the output on entry and of the
+// input on exit.
+__device__ __forceinline__ void rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, float4 c, const NlConst &k,
+                                        AdjAcc &acc) {
+    float2 k1p, k1s, k2p, k2s, k3p, k3s;
+    nl_rhs(P, S, c, k.lossP, k.lossS, k.tpa, k1p, k1s);
+    const float2 y2p = axpy2(0.5f, k1p, P), y2s = axpy2(0.5f, k1s, S);
+    nl_rhs(y2p, y2s, c, k.lossP, k.lossS, k.tpa, k2p, k2s);
+    const float2 y3p = axpy2(0.5f, k2p, P), y3s = axpy2(0.5f, k2s, S);
+    nl_rhs(y3p, y3s, c, k.lossP, k.lossS, k.tpa, k3p, k3s);
+    const float2 y4p = add2(P, k3p), y4s = add2(S, k3s);
+    const float h6 = k.h6;
+    float2 ybp = lP, ybs = lS;
+    float2 kb1p = scl2(h6, lP), kb1s = scl2(h6, lS);
+    float2 kb2p = scl2(2.0f * h6, lP), kb2s = scl2(2.0f * h6, lS);
+    float2 kb3p = kb2p, kb3s = kb2s;
+    const float2 kb4p = kb1p, kb4s = kb1s;
+    float2 rp, rs;
+    rhs_adj(y4p, y4s, kb4p, kb4s, c, k, rp, rs, acc);
+    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
+    kb3p = add2(kb3p, rp); kb3s = add2(kb3s, rs);
+    rhs_adj(y3p, y3s, kb3p, kb3s, c, k, rp, rs, acc);
+    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
+    kb2p = axpy2(0.5f, rp, kb2p); kb2s = axpy2(0.5f, rs, kb2s);
+    rhs_adj(y2p, y2s, kb2p, kb2s, c, k, rp, rs, acc);
+    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
+    kb1p = axpy2(0.5f, rp, kb1p); kb1s = axpy2(0.5f, rs, kb1s);
+    rhs_adj(P, S, kb1p, kb1s, c, k, rp, rs, acc);
+    lP = add2(ybp, rp);
+    lS = add2(ybs, rs);
+}
+
+// Wave-64 sum of the five accumulators; lane 0 writes them to `out`.
+__device__ __forceinline__ void wave_store(AdjAcc a, double *out) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a.c1r += __shfl_down(a.c1r, off, 64);
+        a.c1i += __shfl_down(a.c1i, off, 64);
+        a.c2r += __shfl_down(a.c2r, off, 64);
+        a.c2i += __shfl_down(a.c2i, off, 64);
+        a.tpa += __shfl_down(a.tpa, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        out[0] = a.c1r; out[1] = a.c1i; out[2] = a.c2r; out[3] = a.c2i; out[4] = a.tpa;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward row pass with tape (single instance, modes as RowMode without AUTO).
+// START at s0: tape_a[0] = input, then RK4_s0, R_bwd.
+// MID at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s, tape_a[s+1-s0] = A_{s+1}, RK4_{s+1}, R_bwd.
+// END at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s  (real-space state of the segment end).
+// A null tape writes nothing (checkpoint-only forward: START copies its input to ckpt).
+template <int N>
+__global__ __launch_bounds__(64) void k_row_tape(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
+                                                 int step, int mode, TapeArgs tp) {
+    using C = RowCfg<N>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[2][C::ROWS][C::L];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[0];
+    const int lane = threadIdx.x;
+    const int t = lane % T, rr = lane / T;
+    const int row = blockIdx.x * C::ROWS + rr;
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    const size_t n2 = (size_t)N * N, base = (size_t)row * N;
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = d.U[base + t + j * T];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { __syncthreads(); };
+    const int ls = step - tp.s0;
+    if (mode == ROW_START) {
+        float4 *dst = tp.a ? tp.a : tp.ckpt;
+        if (dst)
+#pragma unroll
+            for (int j = 0; j < E; j++) dst[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    } else {
+        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        const float4 c = d.coef[step];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            if (tp.a2) tp.a2[(size_t)ls * n2 + base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+            rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
+            if (mode == ROW_MID && tp.a)
+                tp.a[(size_t)(ls + 1) * n2 + base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+        }
+    }
+    if (mode != ROW_END) {
+        const float4 c = d.coef[mode == ROW_START ? step : step + 1];
+#pragma unroll
+        for (int j = 0; j < E; j++) rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
+        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+    }
+#pragma unroll
+    for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// Forward column pass with tape: tape_spec[s-s0] = the 2D spectrum before x H.
+template <int N>
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_tape(const InstDesc *__restrict__ descs,
+                                                                 const float2 *__restrict__ twg, int step, TapeArgs tp) {
+    using C = ColCfg<N>;
+    constexpr int E = C::E, T = C::T, CW = C::CW;
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[0];
+    const int tid = threadIdx.x;
+    const int c = tid % CW, t = tid / CW;
+    const int x = blockIdx.x * CW + c;
+    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = d.U[(size_t)(t + j * T) * N + x];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { __syncthreads(); };
+    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    const size_t soff = (size_t)(step - tp.s0) * N * N;
+    const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const int y = t + j * T;
+        if (tp.spec) tp.spec[soff + (size_t)y * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+        fa[j] = c_mul(c_mul(gxp, d.G[N + y]), fa[j]);
+        fb[j] = c_mul(c_mul(gxs, d.G[3 * N + y]), fb[j]);
+    }
+    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++)
+        d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's).
+// ADJ_START at s: RK4_s^T at A''_s, R_bwd.
+// ADJ_MID   at s: R_fwd, RK4_s^T at A_s, RK4_{s-1}^T at A''_{s-1}, R_bwd.
+// ADJ_END   at s: R_fwd, RK4_s^T at A_s (real-space lambda of the segment start).
+// g1[(s*nblk + blk)*5] gets the sensitivities of the first RK4 of step s (at A_s),
+// g2[...] those of the second (at A''_s).
+template <int N>
+__global__ __launch_bounds__(64) void k_adj_row(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
+                                                int step, int mode, TapeArgs tp, double *__restrict__ g1,
+                                                double *__restrict__ g2) {
+    using C = RowCfg<N>;
+    constexpr int E = C::E, T = C::T;
+    __shared__ float2 lds[2][C::ROWS][C::L];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[0];
+    const int lane = threadIdx.x;
+    const int t = lane % T, rr = lane / T;
+    const int row = blockIdx.x * C::ROWS + rr;
+    const int nblk = gridDim.x;
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    const size_t n2 = (size_t)N * N, base = (size_t)row * N;
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = d.U[base + t + j * T];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { __syncthreads(); };
+    const int ls = step - tp.s0;
+    if (mode != ADJ_START) {
+        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        const float4 c = d.coef[step];
+        AdjAcc acc;
+        acc_zero(acc);
+#pragma unroll 2
+        for (int j = 0; j < E; j++) {
+            const float4 a = tp.a[(size_t)ls * n2 + base + t + j * T];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), fa[j], fb[j], c, d.nl, acc);
+        }
+        wave_store(acc, g1 + ((size_t)step * nblk + blockIdx.x) * 5);
+    }
+    if (mode != ADJ_END) {
+        const int sp = mode == ADJ_START ? step : step - 1;   // step of the second RK4
+        const float4 c = d.coef[sp];
+        AdjAcc acc;
+        acc_zero(acc);
+#pragma unroll 2
+        for (int j = 0; j < E; j++) {
+            const float4 a = tp.a2[(size_t)(sp - tp.s0) * n2 + base + t + j * T];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), fa[j], fb[j], c, d.nl, acc);
+        }
+        wave_store(acc, g2 + ((size_t)sp * nblk + blockIdx.x) * 5);
+        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+    }
+#pragma unroll
+    for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// Adjoint column pass: C_bwd, x conj(Gx Gy), C_fwd.  gH[step*nblk + blk] receives
+// sum Re(conj(lambda_hat) (Gx Gy) x_hat * i (a_p, a_s) (qx + qy)) over the block's cells,
+// the sensitivity of J to a temperature change of the propagator (d log H/dT = i a q).
+template <int N>
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_adj_col(const InstDesc *__restrict__ descs,
+                                                                const float2 *__restrict__ twg, int step, TapeArgs tp,
+                                                                const float2 *__restrict__ qxy, float a_p, float a_s,
+                                                                double *__restrict__ gH) {
+    using C = ColCfg<N>;
+    constexpr int E = C::E, T = C::T, CW = C::CW;
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    __shared__ double red[ColCfg<N>::THREADS / 64];
+    const InstDesc d = descs[0];
+    const int tid = threadIdx.x;
+    const int c = tid % CW, t = tid / CW;
+    const int x = blockIdx.x * CW + c;
+    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
+    float2 fa[E], fb[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float4 v = d.U[(size_t)(t + j * T) * N + x];
+        fa[j] = make_float2(v.x, v.y);
+        fb[j] = make_float2(v.z, v.w);
+    }
+    auto sync = [] { __syncthreads(); };
+    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    const size_t soff = (size_t)(step - tp.s0) * N * N;
+    const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
+    const float qx = qxy[x].x;
+    double gsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const int y = t + j * T;
+        const float q = qx + qxy[y].y;
+        const float2 hp = c_mul(gxp, d.G[N + y]), hs = c_mul(gxs, d.G[3 * N + y]);
+        const float4 xs = tp.spec[soff + (size_t)y * N + x];
+        // Re(conj(l) * h * xs * i a q) = -a q Im(conj(l) h xs)
+        const float2 tp_ = c_mul(c_mul(cj(fa[j]), hp), f2(xs.x, xs.y));
+        const float2 ts_ = c_mul(c_mul(cj(fb[j]), hs), f2(xs.z, xs.w));
+        gsum += -(double)(a_p * q) * (double)tp_.y - (double)(a_s * q) * (double)ts_.y;
+        fa[j] = c_mul(cj(hp), fa[j]);
+        fb[j] = c_mul(cj(hs), fb[j]);
+    }
+    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++)
+        d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = gsum;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < C::THREADS / 64; w++) s += red[w];
+        gH[(size_t)step * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// lambda at the output plane: dJ/dA_out with J = P_SH / P_in, P_SH = cs sum |S|^2:
+// lambda_P = 0, lambda_S = 2 cs / P_in * S.
+__global__ __launch_bounds__(256) void k_adj_seed(const float4 *__restrict__ U, float4 *__restrict__ L, int64_t n,
+                                                  float scale) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = U[i];
+    L[i] = make_float4(0.0f, 0.0f, scale * v.z, scale * v.w);
+}
+
+// ---------------------------------------------------------------------------
+template <int N> static int row_blocks() { return N / RowCfg<N>::ROWS; }
+template <int N> static int col_blocks() { return N / ColCfg<N>::CW; }
+
+int adj_row_blocks(int n) {
+    switch (n) {
+        case 64: return row_blocks<64>();
+        case 128: return row_blocks<128>();
+        case 256: return row_blocks<256>();
+        case 512: return row_blocks<512>();
+        default: return 0;
+    }
+}
+int adj_col_blocks(int n) {
+    switch (n) {
+        case 64: return col_blocks<64>();
+        case 128: return col_blocks<128>();
+        case 256: return col_blocks<256>();
+        case 512: return col_blocks<512>();
+        default: return 0;
+    }
+}
+
+#define SHG_DISPATCH(n, CALL)                \
+    switch (n) {                             \
+        case 64: { constexpr int NN = 64; CALL; } break;   \
+        case 128: { constexpr int NN = 128; CALL; } break; \
+        case 256: { constexpr int NN = 256; CALL; } break; \
+        case 512: { constexpr int NN = 512; CALL; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
+hipError_t launch_row_tape(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
+                           hipStream_t st) {
+    if (mode != ROW_START && mode != ROW_MID && mode != ROW_END) return hipErrorInvalidValue;
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_row_tape<NN>, dim3(row_blocks<NN>()), dim3(64), 0, st, d, tw, step, mode, tp));
+    return hipGetLastError();
+}
+
+hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, hipStream_t st) {
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(ColCfg<NN>::THREADS), 0, st, d, tw,
+                                       step, tp));
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
+                          double *g1, double *g2, hipStream_t st) {
+    if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(row_blocks<NN>()), dim3(64), 0, st, d, tw, step, mode, tp, g1,
+                                       g2));
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
+                          float a_p, float a_s, double *gH, hipStream_t st) {
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(ColCfg<NN>::THREADS), 0, st, d, tw,
+                                       step, tp, qxy, a_p, a_s, gH));
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, L, n, scale);
+    return hipGetLastError();
+}
+
+}  // namespace shg

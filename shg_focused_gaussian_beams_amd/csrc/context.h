@@ -1,0 +1,61 @@
+// context.h — the solver context behind the opaque `shg_ctx` of include/shg.h (private
+// to the library's translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/shg.h"
+#include "ssfm_kernels.h"
+#include "thermal_kernels.h"
+
+// Device buffers of the adjoint (allocated on first use, kept with the context).
+struct AdjointBuffers {
+    int k = 0, nseg = 0;                       // checkpoint interval and segment count they were sized for
+    float4 *lam = nullptr;                     // adjoint state
+    float4 *ckpt = nullptr;                    // nseg checkpoints
+    float4 *a = nullptr, *a2 = nullptr, *spec = nullptr;   // segment tape
+    double *g1 = nullptr, *g2 = nullptr, *gH = nullptr;    // FP64 parameter-sensitivity partials
+    float2 *qxy = nullptr;                     // 2 pi^2 fx^2, 2 pi^2 fy^2 per index
+    shg::InstDesc *dlam = nullptr;             // instance table whose U is `lam`
+};
+
+struct shg_ctx {
+    shg_params p{};
+    int n = 0;
+    int steps = 0;                   // z-steps executed by shg_forward (plain nz-1, THERMAL nz)
+    int hz = 0;                      // heat-grid slices (THERMAL)
+    hipStream_t stream = nullptr;
+    float4 *U = nullptr, *U0 = nullptr, *Uout = nullptr;
+    float2 *tw = nullptr, *G = nullptr;
+    double2 *part = nullptr, *red = nullptr;
+    int nparts = 0;
+    std::vector<float> coef;         // plain: (c1, c2) per step, 4 floats
+    float4 *d_coef = nullptr;        // the same on the device
+    shg::InstDesc *d_desc = nullptr; // one-entry instance table for the passes
+    shg::NlConst nl{};
+    // THERMAL state
+    float *T[2] = {nullptr, nullptr};
+    int tcur = 0;                    // index of the buffer holding the latest temperature
+    float *Q = nullptr, *DKint = nullptr;
+    double *jpart = nullptr, *jres = nullptr;
+    int njpart = 0;
+    shg::JacobiConst jc{};
+    shg::ThermConst tc{};
+    float t_centre = 0.f;            // temperature used for the propagators and powers
+    bool have_fields = false;
+    double p_fh_in = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0.0;
+    hipGraphExec_t gexec = nullptr;
+    int use_graph = 1;
+    std::vector<float> host_stage;   // 4 floats per cell
+    AdjointBuffers adj;
+};
+
+// Shared helpers of the library's translation units (defined in shg_capi.cpp).
+int shg_internal_fail(int code, const std::string &msg);
+int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count);
+int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
+double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);

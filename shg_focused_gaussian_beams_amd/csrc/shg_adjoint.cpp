@@ -1,0 +1,239 @@
+// shg_adjoint.cpp — gradients of the SH efficiency of a plain context (C ABI in
+// include/shg.h).  New: the reference has no derivatives.
+//
+// J = eta = P_SH(out) / P_FH(z = 0).  The z-march is differentiated discretely: a
+// checkpointed forward (state copied every K steps), then per segment, last first, a
+// forward recomputation that tapes the RK4 inputs and the column-pass spectra, and the
+// adjoint march through the segment (adjoint_kernels.hip).  Parameter sensitivities:
+//   - the z = 0 pump field: lambda_0 plus the direct P_in term;
+//   - the focal position: through the reference pump setPump (src/headers/Efields.h:74-104)
+//     differentiated analytically;
+//   - the crystal temperature: the step coefficients c1, c2 (kappa and the phase z*dk),
+//     the TPA coefficient, the propagator (k_j = 2 pi n_j / lambda_j), the pump (n_p in
+//     Ap0 and zR) and the n_j of the two powers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../../include/shg.h"
+#include "adjoint_kernels.h"
+#include "context.h"
+#include "physics.h"
+
+namespace {
+
+#define AHIP_TRY(expr)                                                                                               \
+    do {                                                                                                             \
+        hipError_t e_ = (expr);                                                                                      \
+        if (e_ != hipSuccess) return shg_internal_fail(SHG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define ARC_TRY(expr)          \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
+
+using cd = std::complex<double>;
+
+int ensure_buffers(shg_ctx *c, int K, int nseg) {
+    AdjointBuffers &b = c->adj;
+    const size_t n2 = (size_t)c->n * c->n;
+    const int M = c->steps;
+    if (!b.lam) {
+        if (hipMalloc(&b.lam, n2 * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b.g1, (size_t)M * shg::adj_row_blocks(c->n) * 5 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.g2, (size_t)M * shg::adj_row_blocks(c->n) * 5 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.gH, (size_t)M * shg::adj_col_blocks(c->n) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.qxy, (size_t)c->n * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess)
+            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+        // 2 pi^2 f^2 on the fftshifted index grid (DC at 0), x and y spacings
+        const shg::phys::Consts k(c->p);
+        std::vector<float> q(2 * (size_t)c->n);
+        const double PI = 3.14159265358979323846;
+        for (int i = 0; i < c->n; i++) {
+            const int orig = (i + c->n / 2) % c->n;
+            const double fx = (orig - 0.5 * c->n) / ((double)k.dx * c->n), fy = (orig - 0.5 * c->n) / ((double)k.dy * c->n);
+            q[2 * i] = (float)(2 * PI * PI * fx * fx);
+            q[2 * i + 1] = (float)(2 * PI * PI * fy * fy);
+        }
+        AHIP_TRY(hipMemcpy(b.qxy, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice));
+        shg::InstDesc d{};
+        AHIP_TRY(hipMemcpy(&d, c->d_desc, sizeof(d), hipMemcpyDeviceToHost));
+        d.U = b.lam;
+        AHIP_TRY(hipMemcpy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice));
+    }
+    if (b.k != K || b.nseg != nseg) {
+        for (float4 **ptr : {&b.ckpt, &b.a, &b.a2, &b.spec})
+            if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
+        if (hipMalloc(&b.ckpt, (size_t)nseg * n2 * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b.a, (size_t)K * n2 * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b.a2, (size_t)K * n2 * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b.spec, (size_t)K * n2 * sizeof(float4)) != hipSuccess) {
+            b.k = 0;
+            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: tape allocation failed");
+        }
+        b.k = K;
+        b.nseg = nseg;
+    }
+    return SHG_OK;
+}
+
+// dn/dT of the double-precision index (central difference; the Sellmeier form is smooth).
+double dn_dT(int crystal, double lam, double T) {
+    const double h = 1e-3;
+    return (shg::phys::index_d(crystal, lam, T + h) - shg::phys::index_d(crystal, lam, T - h)) / (2 * h);
+}
+
+}  // namespace
+
+extern "C" {
+
+int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res) {
+    if (!c || !res) return shg_internal_fail(SHG_E_INVALID, "shg_adjoint: null argument");
+    if (c->p.thermal) return shg_internal_fail(SHG_E_INVALID, "shg_adjoint: plain contexts only");
+    if (!c->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_adjoint: call shg_set_fields first");
+    const int M = c->steps, n = c->n;
+    const size_t n2 = (size_t)n * n;
+    int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
+    K = std::max(1, std::min(K, M));
+    const int nseg = (M + K - 1) / K;
+    ARC_TRY(ensure_buffers(c, K, nseg));
+    AdjointBuffers &b = c->adj;
+    hipStream_t st = c->stream;
+    const int rb = shg::adj_row_blocks(n), cb = shg::adj_col_blocks(n);
+
+    // 1. checkpointed forward
+    AHIP_TRY(hipEventRecord(c->e0, st));
+    AHIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    for (int j = 0; j < nseg; j++) {
+        const int s0 = j * K, s1 = std::min(M, s0 + K);
+        AHIP_TRY(hipMemcpyAsync(b.ckpt + (size_t)j * n2, c->U, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        ARC_TRY(shg_internal_enqueue_march_plain(c, s0, s1 - s0));
+    }
+    AHIP_TRY(hipEventRecord(c->e1, st));
+    AHIP_TRY(hipEventSynchronize(c->e1));
+    float fwd_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+
+    // efficiency and the adjoint seed
+    const shg::phys::Consts k(c->p);
+    const double T = c->p.temp;
+    const double eps0 = 8.8541878128e-6, cl = 299.792458;
+    const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
+    const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
+    const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
+    double sf = 0.0, ss = 0.0;
+    ARC_TRY(shg_internal_reduce_powers(c, c->U, &sf, &ss));
+    const double pin = c->p_fh_in, J = cs * ss / pin;
+
+    AHIP_TRY(hipEventRecord(c->e0, st));
+    AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
+
+    // 2. per segment, last first: taped recomputation, then the adjoint
+    const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
+    const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
+    const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
+    const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
+    for (int j = nseg - 1; j >= 0; j--) {
+        const int s0 = j * K, s1 = std::min(M, s0 + K);
+        shg::TapeArgs tp{b.a, b.a2, b.spec, nullptr, s0, 0};
+        AHIP_TRY(hipMemcpyAsync(c->U, b.ckpt + (size_t)j * n2, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s0, shg::ROW_START, tp, st));
+        for (int s = s0; s < s1; s++) {
+            AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
+            AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < s1 - 1 ? shg::ROW_MID : shg::ROW_END, tp, st));
+        }
+        AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st));
+        for (int s = s1 - 1; s >= s0; s--) {
+            AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
+            AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1, b.g2, st));
+        }
+    }
+    AHIP_TRY(hipEventRecord(c->e1, st));
+    AHIP_TRY(hipEventSynchronize(c->e1));
+    float adj_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e0, c->e1));
+
+    // 3. gradients on the host (FP64)
+    std::vector<float> lam(4 * n2), u0(4 * n2);
+    std::vector<double> g1((size_t)M * rb * 5), g2((size_t)M * rb * 5), gH((size_t)M * cb);
+    AHIP_TRY(hipMemcpy(lam.data(), b.lam, n2 * sizeof(float4), hipMemcpyDeviceToHost));
+    AHIP_TRY(hipMemcpy(u0.data(), c->U0, n2 * sizeof(float4), hipMemcpyDeviceToHost));
+    AHIP_TRY(hipMemcpy(g1.data(), b.g1, g1.size() * sizeof(double), hipMemcpyDeviceToHost));
+    AHIP_TRY(hipMemcpy(g2.data(), b.g2, g2.size() * sizeof(double), hipMemcpyDeviceToHost));
+    AHIP_TRY(hipMemcpy(gH.data(), b.gH, gH.size() * sizeof(double), hipMemcpyDeviceToHost));
+
+    // dJ/dA0 = lambda_0 (FH part) + d(1/P_in)/dA0 term
+    std::vector<cd> grad(n2);
+    const double direct = -J / pin * 2.0 * cp;
+    for (size_t i = 0; i < n2; i++) grad[i] = cd(lam[4 * i], lam[4 * i + 1]) + direct * cd(u0[4 * i], u0[4 * i + 1]);
+    if (d_eta_d_fh)
+        for (size_t i = 0; i < n2; i++) {
+            d_eta_d_fh[2 * i] = (float)grad[i].real();
+            d_eta_d_fh[2 * i + 1] = (float)grad[i].imag();
+        }
+
+    // temperature through the step coefficients and the TPA coefficient
+    const double kap = (double)k.kappa_p((float)T), kas = (double)k.kappa_s((float)T);
+    const double dkap = -kap * dnp / np, dkas = -kas * dns / ns;
+    const double ddk = 2 * M_PI * (2 * dnp / (double)k.lp - dns / (double)k.ls);
+    const double dtpa = 0.5 * (double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * dns;
+    const double dkf = (double)k.dk((float)T);
+    double dT_coef = 0.0, dT_H = 0.0;
+    float z = 0.0f;
+    for (int s = 0; s < M; s++) {
+        cd c1b(0, 0), c2b(0, 0);
+        double tb = 0.0;
+        for (int blk = 0; blk < rb; blk++) {
+            const double *x1 = &g1[((size_t)s * rb + blk) * 5], *x2 = &g2[((size_t)s * rb + blk) * 5];
+            c1b += cd(x1[0] + x2[0], x1[1] + x2[1]);
+            c2b += cd(x1[2] + x2[2], x1[3] + x2[3]);
+            tb += x1[4] + x2[4];
+        }
+        const double zs = (double)z, th = (double)(float)(z * (float)dkf);
+        const cd I(0, 1);
+        const cd dc1 = I * std::exp(-I * th) * (dkap - I * kap * zs * ddk);
+        const cd dc2 = I * std::exp(I * th) * (dkas + I * kas * zs * ddk);
+        dT_coef += (std::conj(c1b) * dc1).real() + (std::conj(c2b) * dc2).real() + tb * dtpa;
+        for (int blk = 0; blk < cb; blk++) dT_H += gH[(size_t)s * cb + blk];
+        z += k.dz;
+    }
+    const double dT_powers = J * (dns / ns - dnp / np);
+
+    // focus and temperature through the reference pump (analytic derivative of setPump)
+    const double w0 = c->p.waist, f = c->p.focus, lam0 = (double)k.lp;
+    const double zr = M_PI * np * w0 * w0 / lam0;
+    const double ap0 = std::sqrt(4.0 * c->p.power / (eps0 * cl * M_PI * np * w0 * w0));
+    const cd Mq(1.0, -f / zr);
+    double d_focus = 0.0, dT_pump = 0.0;
+    for (int iy = 0; iy < n; iy++)
+        for (int ix = 0; ix < n; ix++) {
+            const double xx = (ix - 0.5 * n) * (double)k.dx, yy = (iy - 0.5 * n) * (double)k.dy;
+            const double r2 = xx * xx + yy * yy;
+            const cd A = ap0 / Mq * std::exp(-r2 / (w0 * w0 * Mq));
+            const cd dA_dM = A * (-1.0 / Mq + r2 / (w0 * w0 * Mq * Mq));
+            const cd dA_df = dA_dM * cd(0.0, -1.0 / zr);
+            const cd dA_dn = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
+            const cd g = grad[(size_t)iy * n + ix];
+            d_focus += (std::conj(g) * dA_df).real();
+            dT_pump += (std::conj(g) * dA_dn).real() * dnp;
+        }
+
+    res->eta = J;
+    res->d_eta_d_focus = d_focus;
+    res->d_eta_d_temp_crystal = dT_coef + dT_H + dT_powers;
+    res->d_eta_d_temp = res->d_eta_d_temp_crystal + dT_pump;
+    res->forward_ms = fwd_ms;
+    res->adjoint_ms = adj_ms;
+    res->checkpoint_interval = K;
+    res->segments = nseg;
+    return SHG_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 #include "physics.h"
 #include "ssfm_kernels.h"
 #include "thermal_kernels.h"
+#include "context.h"
 
 namespace {
 
@@ -42,41 +43,10 @@ bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512;
 
 }  // namespace
 
-struct shg_ctx {
-    shg_params p{};
-    int n = 0;
-    int steps = 0;                   // z-steps executed by shg_forward (plain nz-1, THERMAL nz)
-    int hz = 0;                      // heat-grid slices (THERMAL)
-    hipStream_t stream = nullptr;
-    float4 *U = nullptr, *U0 = nullptr, *Uout = nullptr;
-    float2 *tw = nullptr, *G = nullptr;
-    double2 *part = nullptr, *red = nullptr;
-    int nparts = 0;
-    std::vector<float> coef;         // plain: (c1, c2) per step, 4 floats
-    shg::NlConst nl{};
-    // THERMAL state
-    float *T[2] = {nullptr, nullptr};
-    int tcur = 0;                    // index of the buffer holding the latest temperature
-    float *Q = nullptr, *DKint = nullptr;
-    double *jpart = nullptr, *jres = nullptr;
-    int njpart = 0;
-    shg::JacobiConst jc{};
-    shg::ThermConst tc{};
-    float t_centre = 0.f;            // temperature used for the propagators and powers
-    bool have_fields = false;
-    double p_fh_in = 0.0;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    double last_ms = 0.0;
-    hipGraphExec_t gexec = nullptr;
-    int use_graph = 1;
-    std::vector<float> host_stage;   // 4 floats per cell
-};
+int shg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
+
 
 namespace {
-
-float4 coef_at(const shg_ctx *c, int s) {
-    return make_float4(c->coef[4 * s], c->coef[4 * s + 1], c->coef[4 * s + 2], c->coef[4 * s + 3]);
-}
 
 int upload_propagators(shg_ctx *c, float temp) {
     const shg::phys::Consts k(c->p);
@@ -91,22 +61,11 @@ int upload_propagators(shg_ctx *c, float temp) {
 // is in real space before and after.
 int enqueue_march_plain(shg_ctx *c, int first, int count) {
     if (count <= 0) return SHG_OK;
-    shg::RowArgs a{};
-    a.nl = c->nl;
-    a.ccur = coef_at(c, first);
-    a.flags = shg::ROW_NCUR | shg::ROW_BWD;
-    HIP_TRY(shg::launch_row_pass(c->n, c->U, c->tw, a, c->stream));
+    HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, first, shg::ROW_START, c->stream));
     const int last = first + count - 1;
     for (int s = first; s <= last; s++) {
-        HIP_TRY(shg::launch_col_pass(c->n, c->U, c->tw, c->G, c->stream));
-        a.cprev = coef_at(c, s);
-        if (s < last) {
-            a.ccur = coef_at(c, s + 1);
-            a.flags = shg::ROW_FWD | shg::ROW_NPREV | shg::ROW_NCUR | shg::ROW_BWD;
-        } else {
-            a.flags = shg::ROW_FWD | shg::ROW_NPREV;
-        }
-        HIP_TRY(shg::launch_row_pass(c->n, c->U, c->tw, a, c->stream));
+        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
+        HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, s, s < last ? shg::ROW_MID : shg::ROW_END, c->stream));
     }
     return SHG_OK;
 }
@@ -132,7 +91,7 @@ int enqueue_march_thermal(shg_ctx *c) {
     float *T = c->T[c->tcur];
     HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
     for (int s = 0; s < nz; s++) {
-        HIP_TRY(shg::launch_col_pass(c->n, c->U, c->tw, c->G, c->stream));
+        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
         a.prev = therm_step(c, s);
         if (s < nz - 1) {
             a.cur = therm_step(c, s + 1);
@@ -168,8 +127,11 @@ void release(shg_ctx *c) {
     if (!c) return;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
+                      (void *)c->d_coef, (void *)c->d_desc,
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
-                      (void *)c->jpart, (void *)c->jres})
+                      (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt, (void *)c->adj.a,
+                      (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
+                      (void *)c->adj.qxy, (void *)c->adj.dlam})
         if (ptr) (void)hipFree(ptr);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
@@ -260,6 +222,14 @@ int timed_march(shg_ctx *c) {
 
 }  // namespace
 
+int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count) { return enqueue_march_plain(c, first, count); }
+int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh) {
+    return reduce_powers(c, U, sum_fh, sum_sh);
+}
+double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp) {
+    return power_of(c, which, sum, temp);
+}
+
 extern "C" {
 
 const char *shg_last_error(void) { return g_err.c_str(); }
@@ -334,8 +304,14 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
     c->nl.h6 = h / 6.0f;
     c->host_stage.resize(4 * ncell);
+    if (hipMalloc(&c->d_desc, sizeof(shg::InstDesc)) != hipSuccess)
+        return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
     if (!p->thermal) {
         c->coef = shg::phys::step_coefficients(k, T, c->steps);
+        if (hipMalloc(&c->d_coef, c->coef.size() * sizeof(float)) != hipSuccess)
+            return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
+        if (hipMemcpy(c->d_coef, c->coef.data(), c->coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     } else {
         const size_t n3 = ncell * c->hz;
         c->njpart = shg::jacobi_blocks(c->n, c->n, c->hz);
@@ -358,6 +334,14 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         tc.crystal = p->crystal;
         if (int rc = thermal_init_temperature(c)) return bad(rc);
     }
+    shg::InstDesc hd{};
+    hd.U = c->U;
+    hd.G = c->G;
+    hd.coef = c->d_coef;
+    hd.nl = c->nl;
+    hd.steps = c->steps;
+    if (hipMemcpy(c->d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     if (int rc = upload_propagators(c, T)) return bad(rc);
     *out = c;
     return SHG_OK;
@@ -462,19 +446,15 @@ int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
         for (int r = 0; r < reps; r++)
             HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->T[c->tcur], c->Q, c->DKint, a, c->stream));
     } else {
-        shg::RowArgs a{};
-        a.nl = c->nl;
-        a.cprev = coef_at(c, 0);
-        a.ccur = coef_at(c, c->steps > 1 ? 1 : 0);
-        a.flags = shg::ROW_FWD | shg::ROW_NPREV | shg::ROW_NCUR | shg::ROW_BWD;
-        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->U, c->tw, a, c->stream));
+        const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
+        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream));
     }
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
     *row_ms = ms / reps;
     HIP_TRY(hipEventRecord(c->e0, c->stream));
-    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->U, c->tw, c->G, c->stream));
+    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));

@@ -21,25 +21,28 @@
 
 namespace shg {
 
+__device__ __forceinline__ void block_sync() { __syncthreads(); }
+
 __device__ __forceinline__ void rk4(float2 &P, float2 &S, float4 c, const NlConst &k) {
     rk4_cell(P, S, c, c, c, k.lossP, k.lossS, k.tpa, k.h6);
 }
 
-__device__ __forceinline__ void block_sync() { __syncthreads(); }
-
-// Row pass over rows [blockIdx.x*ROWS, +ROWS) of one N x N field pair.
+// Row pass over rows [blockIdx.x*ROWS, +ROWS) of instance blockIdx.y (see RowMode).
 template <int N>
-__global__ __launch_bounds__(64) void k_row_pass(float4 *__restrict__ U, const float2 *__restrict__ twg, RowArgs a) {
+__global__ __launch_bounds__(64) void k_row_pass(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
+                                                 int step, int mode) {
     using C = RowCfg<N>;
     constexpr int E = C::E, T = C::T;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
+    const InstDesc d = descs[blockIdx.y];
+    if (mode == ROW_AUTO) mode = step + 1 < d.steps ? ROW_MID : ROW_END;
     const int lane = threadIdx.x;
     const int t = lane % T, rr = lane / T;
     const int row = blockIdx.x * C::ROWS + rr;
     for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
     __syncthreads();
-    float4 *Ur = U + (size_t)row * N;
+    float4 *Ur = d.U + (size_t)row * N;
     float2 fa[E], fb[E];
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -48,30 +51,35 @@ __global__ __launch_bounds__(64) void k_row_pass(float4 *__restrict__ U, const f
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { block_sync(); };
-    if (a.flags & ROW_FWD) fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
-    if (a.flags & ROW_NPREV) {
+    if (mode != ROW_START) {
+        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        const float4 c = d.coef[step];
 #pragma unroll
-        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], a.cprev, a.nl);
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
     }
-    if (a.flags & ROW_NCUR) {
+    if (mode != ROW_END) {
+        const float4 c = d.coef[mode == ROW_START ? step : step + 1];
 #pragma unroll
-        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], a.ccur, a.nl);
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
+        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
     }
-    if (a.flags & ROW_BWD) fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++) Ur[t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
 
-// Column pass over columns [blockIdx.x*CW, +CW): C_bwd, x H/(N*N), C_fwd.
-// G holds the separable propagator: Gx_p[N], Gy_p[N], Gx_s[N], Gy_s[N] (Gy carries 1/N^2).
+// Column pass over columns [blockIdx.x*CW, +CW) of instance blockIdx.y: C_bwd, x H/(N*N),
+// C_fwd.  G holds the separable propagator: Gx_p[N], Gy_p[N], Gx_s[N], Gy_s[N] (Gy
+// carries the 1/N^2 of FFTscale).
 template <int N>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(float4 *__restrict__ U,
-                                                                 const float2 *__restrict__ twg,
-                                                                 const float2 *__restrict__ G) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(const InstDesc *__restrict__ descs,
+                                                                 const float2 *__restrict__ twg) {
     using C = ColCfg<N>;
     constexpr int E = C::E, T = C::T, CW = C::CW;
     __shared__ float2 lds[2][C::L * CW];
     __shared__ float2 tw[N];
+    const InstDesc d = descs[blockIdx.y];
+    float4 *__restrict__ U = d.U;
+    const float2 *__restrict__ G = d.G;
     const int tid = threadIdx.x;
     const int c = tid % CW, t = tid / CW;
     const int x = blockIdx.x * CW + c;
@@ -136,36 +144,61 @@ __global__ __launch_bounds__(256) void k_power_final(const double2 *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// One block per table entry: FP64 sums of |FH|^2 and |SH|^2 of that instance.
+__global__ __launch_bounds__(256) void k_power_batch(const InstDesc *__restrict__ descs, const int *__restrict__ ncell,
+                                                     double2 *out) {
+    const float4 *U = descs[blockIdx.x].U;
+    const int n = ncell[blockIdx.x];
+    double sp = 0.0, ss = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const float4 v = U[i];
+        sp += (double)v.x * v.x + (double)v.y * v.y;
+        ss += (double)v.z * v.z + (double)v.w * v.w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sp += __shfl_down(sp, off, 64);
+        ss += __shfl_down(ss, off, 64);
+    }
+    __shared__ double2 w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = make_double2(sp, ss);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = make_double2(w[0].x + w[1].x + w[2].x + w[3].x, w[0].y + w[1].y + w[2].y + w[3].y);
+}
+
 // ---------------------------------------------------------------------------
-// Launchers.  The caller guarantees the buffers hold n*n cells, n*... twiddles etc.
+// Launchers.
 
-template <int N> static hipError_t row_launch(float4 *U, const float2 *tw, const RowArgs &a, hipStream_t st) {
+template <int N>
+static hipError_t row_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
     using C = RowCfg<N>;
-    hipLaunchKernelGGL(k_row_pass<N>, dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, tw, a);
+    hipLaunchKernelGGL(k_row_pass<N>, dim3(N / C::ROWS, ninst), dim3(C::THREADS), 0, st, d, tw, step, mode);
     return hipGetLastError();
 }
-template <int N> static hipError_t col_launch(float4 *U, const float2 *tw, const float2 *G, hipStream_t st) {
+template <int N> static hipError_t col_launch(const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
     using C = ColCfg<N>;
-    hipLaunchKernelGGL(k_col_pass<N>, dim3(N / C::CW), dim3(C::THREADS), 0, st, U, tw, G);
+    hipLaunchKernelGGL(k_col_pass<N>, dim3(N / C::CW, ninst), dim3(C::THREADS), 0, st, d, tw);
     return hipGetLastError();
 }
 
-hipError_t launch_row_pass(int n, float4 *U, const float2 *tw, const RowArgs &a, hipStream_t st) {
+hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
+    if (ninst <= 0 || ninst > 65535 || step < 0 || mode < ROW_START || mode > ROW_AUTO) return hipErrorInvalidValue;
     switch (n) {
-        case 64: return row_launch<64>(U, tw, a, st);
-        case 128: return row_launch<128>(U, tw, a, st);
-        case 256: return row_launch<256>(U, tw, a, st);
-        case 512: return row_launch<512>(U, tw, a, st);
+        case 64: return row_launch<64>(d, ninst, tw, step, mode, st);
+        case 128: return row_launch<128>(d, ninst, tw, step, mode, st);
+        case 256: return row_launch<256>(d, ninst, tw, step, mode, st);
+        case 512: return row_launch<512>(d, ninst, tw, step, mode, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_col_pass(int n, float4 *U, const float2 *tw, const float2 *G, hipStream_t st) {
+hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
+    if (ninst <= 0 || ninst > 65535) return hipErrorInvalidValue;
     switch (n) {
-        case 64: return col_launch<64>(U, tw, G, st);
-        case 128: return col_launch<128>(U, tw, G, st);
-        case 256: return col_launch<256>(U, tw, G, st);
-        case 512: return col_launch<512>(U, tw, G, st);
+        case 64: return col_launch<64>(d, ninst, tw, st);
+        case 128: return col_launch<128>(d, ninst, tw, st);
+        case 256: return col_launch<256>(d, ninst, tw, st);
+        case 512: return col_launch<512>(d, ninst, tw, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -173,6 +206,12 @@ hipError_t launch_col_pass(int n, float4 *U, const float2 *tw, const float2 *G, 
 hipError_t launch_power(const float4 *U, int64_t ncell, double2 *part, int nparts, double2 *out, hipStream_t st) {
     hipLaunchKernelGGL(k_power_partial, dim3(nparts), dim3(256), 0, st, U, ncell, part);
     hipLaunchKernelGGL(k_power_final, dim3(1), dim3(256), 0, st, part, nparts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_power_batch(const InstDesc *d, const int *ncell, int ninst, double2 *out, hipStream_t st) {
+    if (ninst <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_power_batch, dim3(ninst), dim3(256), 0, st, d, ncell, out);
     return hipGetLastError();
 }
 

@@ -78,6 +78,28 @@ def config1_params(**overrides):
     return p
 
 
+def config3_params(count=256, seed=3, sizes=(64, 128, 256, 512), nz_range=(100, 400), **overrides):
+    """Batched sweep: per instance, grid size uniform over `sizes`, w0 ~ U(20, 80) um,
+    P ~ U(1, 50) W, T ~ 27 C (QPM) + U(-2, 2), focus ~ L/2 + U(-L/4, L/4),
+    nz ~ U{nz_range}, window 8*w0; otherwise as config1 (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    L = 30000.0
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        w0 = float(rng.uniform(20.0, 80.0))
+        p = config1_params(nx=n, ny=n, nz=int(rng.integers(nz_range[0], nz_range[1] + 1)),
+                           lx=8 * w0, ly=8 * w0, waist=w0, power=float(rng.uniform(1.0, 50.0)),
+                           temp=27.0 + float(rng.uniform(-2.0, 2.0)),
+                           focus=L / 2 + float(rng.uniform(-L / 4, L / 4)))
+        p.t_peltier1 = p.t_peltier2 = p.temp
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        out.append(p)
+    return out
+
+
 def config2_params(**overrides):
     """THERMAL workload: as config1 but P = 30 W, w0 = 40 um, 256x256 grid, nz = 400,
     alpha_p = 0.1/m, alpha_s = 4/m, beta_s = 1e-13 m/W, 25 C ambient."""
