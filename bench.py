@@ -1,16 +1,20 @@
 """Throughput of the split-step (SSFM) z-march on one MI355X.
 
-A "step" of this benchmark is one full solve of the workload: the fields enter the
-crystal and are marched through all z-steps to the output plane (the state the
-reference writes as Pump_out/Signal_out).  Inputs are resident in HBM before timing.
-
-metric: SSFM cell-steps/s = nx*ny*(z-steps executed) per second of wall time.
-roofline: the fused z-step (row pass + column pass) at 32 algorithmic bytes per
-cell-step (FH + SH complex64, each read once and written once per step), timed with
-HIP events on the library's stream; peak 8 TB/s HBM.
-cpu_baseline: the FP32 CPU restatement of the reference (oracle/, -O3, per-cell n()
-as the reference evaluates it) on one host core, over a bounded number of z-steps of
-the same workload.
+Headline workload: THERMAL SSFM, 1064 -> 532 nm in MgO:sPPLT,
+P = 30 W, w0 = 40 um, 256x256 grid, nz = 400, heat grid 256x256x100, Jacobi to a mean
+squared update < 1e-6, dk(T) fed back.  A "step" of this benchmark is one THERMAL
+z-march of the field pair through the crystal at the converged temperature (nz steps:
+the last one feeds the heat source), inputs resident in HBM.
+  value  = SSFM cell-steps/s = nx*ny*nz / time per march
+  roofline = the fused THERMAL z-step (row pass + column pass): 48 algorithmic bytes
+           per cell-step (FH+SH read+write 32, T slice 4, Q slice 4, dk prefix r/w 8),
+           average launch durations from HIP events on the library stream; 8 TB/s peak.
+Also reported (same run): the THERMAL outer loop, Jacobi voxel-sweeps/s, the batched
+sweep of config 3 (256 uneven instances in one hipGraph) against one-at-a-time solves,
+forward+adjoint of config 4 against one forward, and the plain config-2-shape march.
+cpu_baseline: the FP32 CPU restatement of the reference's THERMAL march (oracle/, -O3,
+per-cell n() as the reference evaluates it) on one host core over a bounded number of
+z-steps of the same grid.
 """
 import argparse
 import json
@@ -23,24 +27,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
-BYTES_PER_CELL_STEP = 32
+B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 48, 80, 12
 
 
-def workload_params(name):
-    import shg_focused_gaussian_beams_amd as shg
-    if name == "config2-plain":
-        return shg.config2_params(thermal=0)
-    if name == "config1":
-        return shg.config1_params()
-    raise SystemExit(f"unknown workload {name}")
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(p, seed=1):
+def make_inputs(p, seed=1, noise=1e-6):
     import numpy as np
     import shg_focused_gaussian_beams_amd as shg
     pump = shg.make_pump(p)
     rng = np.random.default_rng(seed)
-    amp = 1e-6 * np.abs(pump).max()
+    amp = noise * np.abs(pump).max()
     pump = (pump + amp * (rng.standard_normal(pump.shape) + 1j * rng.standard_normal(pump.shape))).astype(np.complex64)
     return pump, np.zeros_like(pump)
 
@@ -55,37 +54,183 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(p, fh0, sh0, budget_s=12.0):
-    """oracle32 (-O3, per-cell n()) on one core over a bounded number of z-steps."""
+def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=15.0):
+    """oracle32 THERMAL march (per-cell n(), the reference's O(s) dk re-integration) on
+    one core, over the first k z-steps of the workload grid."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
-    lib = orc.load()
-    lib.oracle_set_percell_index(1)
-    try:
-        t0 = time.perf_counter()
-        orc.cwes(p, fh0, sh0, prec=32, nsteps=1)
-        one = time.perf_counter() - t0
-        k = max(1, min(p.nz - 1, int(budget_s / max(one, 1e-6))))
-        t0 = time.perf_counter()
-        orc.cwes(p, fh0, sh0, prec=32, nsteps=k)
-        dt = time.perf_counter() - t0
-    finally:
-        lib.oracle_set_percell_index(0)
+    hz = p.heat_nz
+    t0 = time.perf_counter()
+    q = p.copy(nz=2, heat_nz=2)
+    orc.thermal_cwes(q, T[[0, -1]], fh0, sh0)
+    one = (time.perf_counter() - t0) / 2
+    k = max(2, min(p.nz, int(budget_s / max(one, 1e-6))))
+    q = p.copy(nz=k, heat_nz=max(2, min(k, hz)), lz=p.lz * (k - 1) / (p.nz - 1))
+    idx = [min(hz - 1, int(i * hz / k)) for i in range(q.heat_nz)]
+    t0 = time.perf_counter()
+    orc.thermal_cwes(q, T[idx], fh0, sh0)
+    dt = time.perf_counter() - t0
     cells = p.nx * p.ny * k
     return {"value": cells / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle32 -O3 (per-cell n(), own radix-2 FFT) on {p.nx}x{p.ny}, {k} of {p.nz - 1} "
-                      f"z-steps, {dt:.1f} s; host CPU: {cpu_model()}"}
+            "sample": f"oracle32 -O3 THERMAL march (per-cell n(), own radix-2 FFT, no FFTW on the host) on "
+                      f"{p.nx}x{p.ny}, first {k} of {p.nz} z-steps, {dt:.1f} s; host CPU: {cpu_model()}"}
+
+
+def bench_thermal(shg, args):
+    import numpy as np
+    p = shg.config2_params(heat_nz=100, jacobi_tol=1e-6)
+    fh0, sh0 = make_inputs(p)
+    ctx = shg.Context(p)
+    ctx.set_fields(fh0, sh0)
+    log("thermal outer loop ...")
+    t0 = time.perf_counter()
+    st = ctx.thermal_run(pump_from_temperature=True)
+    loop_s = time.perf_counter() - t0
+    T = ctx.get_temperature()
+    for _ in range(args.warmup):
+        ctx.forward()
+    import torch
+    torch.cuda.synchronize()
+    log("timed THERMAL marches ...")
+    t0 = time.perf_counter()
+    dev = 0.0
+    for _ in range(args.steps):
+        ctx.forward()
+        dev += ctx.last_forward_ms()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = ctx.cell_steps()
+    _, _, eta = ctx.powers()
+    row_ms, col_ms = ctx.profile_kernels(args.profile_reps)
+    # Jacobi throughput on the heat grid
+    nj = 200
+    t1 = time.perf_counter()
+    ctx.jacobi(nj, residual=False)
+    torch.cuda.synchronize()
+    jac_s = (time.perf_counter() - t1) / nj
+    vox = p.nx * p.ny * p.heat_nz
+    ctx.close()
+    return dict(p=p, T=T, fh0=fh0, sh0=sh0, wall=wall, steps=args.steps, dev_ms=dev / args.steps, cells=cells,
+                eta=eta, row_ms=row_ms, col_ms=col_ms, stats=st, loop_s=loop_s, jac_s=jac_s, vox=vox)
+
+
+def bench_plain(shg, args, p):
+    import torch
+    fh0, sh0 = make_inputs(p)
+    ctx = shg.Context(p)
+    ctx.set_fields(fh0, sh0)
+    for _ in range(args.warmup):
+        ctx.forward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.forward()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    row_ms, col_ms = ctx.profile_kernels(args.profile_reps)
+    cells = ctx.cell_steps()
+    ctx.close()
+    return dict(s_per_solve=wall, cells=cells, row_ms=row_ms, col_ms=col_ms)
+
+
+def bench_batch(shg, args):
+    import numpy as np
+    import torch
+    ps = shg.config3_params(count=args.batch, seed=3)
+    b = shg.Batch(ps)
+    b.init_fields(noise_rel=1e-6, seed=3)
+    log("batch: first run (graph capture) ...")
+    b.run()
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / reps
+    cells = b.cell_steps()
+    _, _, eta = b.powers()
+    # one-at-a-time: every instance solved alone by a single context (graph replayed)
+    log("batch: one-at-a-time reference solves ...")
+    single_s = 0.0
+    sample = list(range(0, len(ps), max(1, len(ps) // args.single_sample)))
+    per = []
+    for i in sample:
+        ctx = shg.Context(ps[i])
+        fh, sh = b.get_input(i)
+        ctx.set_fields(fh, sh)
+        ctx.forward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.forward()
+        torch.cuda.synchronize()
+        per.append((ps[i].nx * ps[i].ny * (ps[i].nz - 1), time.perf_counter() - t0))
+        ctx.close()
+    # extrapolate by cell-steps within each grid size
+    by_n = {}
+    for i, (c, t) in zip(sample, per):
+        by_n.setdefault(ps[i].nx, []).append(t / c)
+    for p in ps:
+        rate = np.mean(by_n.get(p.nx, [np.mean([t / c for c, t in per])]))
+        single_s += rate * p.nx * p.ny * (p.nz - 1)
+    b.close()
+    return dict(count=len(ps), s=batch_s, cells=cells, single_s=single_s, eta_mean=float(np.mean(eta)),
+                sampled=len(sample))
+
+
+def bench_adjoint(shg, args):
+    import torch
+    p = shg.config1_params(nx=256, ny=256, nz=400, lx=8 * 35.0, ly=8 * 35.0, power=10.0, waist=35.0)
+    fh0, sh0 = make_inputs(p)
+    ctx = shg.Context(p)
+    ctx.set_fields(fh0, sh0)
+    ctx.forward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.forward()
+    torch.cuda.synchronize()
+    fwd_s = (time.perf_counter() - t0) / 3
+    res, _ = ctx.adjoint()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        res, _ = ctx.adjoint()
+    torch.cuda.synchronize()
+    adj_s = (time.perf_counter() - t0) / 2
+    res_full, _ = ctx.adjoint(checkpoint_interval=p.nz - 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_full, _ = ctx.adjoint(checkpoint_interval=p.nz - 1)
+    torch.cuda.synchronize()
+    adj_full_s = time.perf_counter() - t0
+    cells = ctx.cell_steps()
+    ctx.close()
+    return dict(fwd_s=fwd_s, adj_s=adj_s, adj_full_s=adj_full_s, cells=cells, res=res)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config2-plain")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--profile-reps", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--single-sample", type=int, default=32)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB 5 times (calibrates rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        import torch
+        x = torch.ones(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_copy": x.numel() * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the solver has no multi-GPU path",
               file=sys.stderr)
@@ -96,48 +241,61 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
 
-    p = workload_params(args.workload)
-    fh0, sh0 = make_inputs(p)
-    ctx = shg.Context(p)
-    ctx.set_fields(fh0, sh0)
-    for _ in range(args.warmup):
-        ctx.forward()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dev_ms = 0.0
-    for _ in range(args.steps):
-        ctx.forward()
-        dev_ms += ctx.last_forward_ms()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    cell_steps = ctx.cell_steps()
-    value = cell_steps / (wall / args.steps)
-    _, _, eta = ctx.powers()
-
-    row_ms, col_ms = ctx.profile_kernels(args.profile_reps)
-    step_s = (row_ms + col_ms) * 1e-3
-    n2 = p.nx * p.ny
-    achieved = BYTES_PER_CELL_STEP * n2 / step_s / 1e9
-    ctx.close()
-
+    th = bench_thermal(shg, args)
+    p = th["p"]
+    ms_per_step = th["wall"] * 1e3 / th["steps"]
+    value = th["cells"] / (th["wall"] / th["steps"])
+    step_s = (th["row_ms"] + th["col_ms"]) * 1e-3
+    achieved = B_THERMAL * p.nx * p.ny / step_s / 1e9
     out = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian pump + 1e-6 relative complex noise, SH = 0)",
-        "config": {"workload": args.workload, "nx": p.nx, "ny": p.ny, "nz": p.nz,
-                   "z_steps": p.nz - 1, "cell_steps_per_solve": cell_steps, "thermal": bool(p.thermal),
-                   "eta": eta, "device_ms_per_solve": dev_ms / args.steps,
-                   "algorithmic_GBps": value * BYTES_PER_CELL_STEP / 1e9},
-        "roofline": {"bound": "hbm", "kernel": "fused z-step = k_row_pass + k_col_pass",
+        "config": {"workload": "config2 THERMAL SSFM march: MgO:sPPLT 30 mm, P=30 W, w0=40 um, 256x256, nz=400, "
+                               "heat grid 256x256x100, Jacobi to mean squared update < 1e-6",
+                   "nx": p.nx, "ny": p.ny, "nz": p.nz, "heat_nz": p.heat_nz,
+                   "cell_steps_per_march": th["cells"], "eta": th["eta"],
+                   "device_ms_per_march": th["dev_ms"],
+                   "algorithmic_GBps": value * B_THERMAL / 1e9,
+                   "thermal_loop": {**th["stats"], "wall_s": th["loop_s"]},
+                   "jacobi_voxel_sweeps_per_s": th["vox"] / th["jac_s"],
+                   "jacobi_GBps": th["vox"] * B_JACOBI / th["jac_s"] / 1e9},
+        "roofline": {"bound": "hbm", "kernel": "fused THERMAL z-step = k_therm_row_pass + k_col_pass (256x256)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "row_pass_us": row_ms * 1e3, "col_pass_us": col_ms * 1e3},
+                     "bytes_per_cell_step": B_THERMAL,
+                     "row_pass_us": th["row_ms"] * 1e3, "col_pass_us": th["col_ms"] * 1e3},
     }
+    if not args.no_extra:
+        log("plain config-2 shape ...")
+        pl = bench_plain(shg, args, shg.config2_params(thermal=0))
+        out["config"]["plain_256x256_nz400"] = {
+            "cell_steps_per_s": pl["cells"] / pl["s_per_solve"], "ms_per_solve": pl["s_per_solve"] * 1e3,
+            "row_pass_us": pl["row_ms"] * 1e3, "col_pass_us": pl["col_ms"] * 1e3,
+            "step_GBps_algorithmic": B_PLAIN * 65536 / ((pl["row_ms"] + pl["col_ms"]) * 1e-3) / 1e9}
+        log("batched config 3 ...")
+        bt = bench_batch(shg, args)
+        out["config"]["batched_config3"] = {
+            "instances": bt["count"], "s_per_batch": bt["s"], "cell_steps_per_s": bt["cells"] / bt["s"],
+            "instances_per_s": bt["count"] / bt["s"], "algorithmic_GBps": bt["cells"] * B_PLAIN / bt["s"] / 1e9,
+            "hbm_frac": bt["cells"] * B_PLAIN / bt["s"] / 1e9 / HBM_PEAK_GBPS,
+            "one_at_a_time_s": bt["single_s"], "speedup_vs_one_at_a_time": bt["single_s"] / bt["s"],
+            "one_at_a_time_sampled_instances": bt["sampled"], "mean_eta": bt["eta_mean"]}
+        log("forward+adjoint config 4 ...")
+        ad = bench_adjoint(shg, args)
+        out["config"]["fwd_adjoint_config4"] = {
+            "forward_ms": ad["fwd_s"] * 1e3, "fwd_adj_ms": ad["adj_s"] * 1e3,
+            "fwd_adj_over_fwd": ad["adj_s"] / ad["fwd_s"],
+            "fwd_adj_full_tape_ms": ad["adj_full_s"] * 1e3, "fwd_adj_full_tape_over_fwd": ad["adj_full_s"] / ad["fwd_s"],
+            "cell_steps_per_s": ad["cells"] / ad["adj_s"],
+            "checkpoint_interval": ad["res"]["checkpoint_interval"], "eta": ad["res"]["eta"],
+            "d_eta_d_temp": ad["res"]["d_eta_d_temp"], "d_eta_d_focus": ad["res"]["d_eta_d_focus"]}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, fh0, sh0)
-        out["cpu_baseline"]["gpu_speedup"] = value / out["cpu_baseline"]["value"]
+        log("cpu baseline ...")
+        cb = cpu_baseline_thermal(p, th["T"], th["fh0"], th["sh0"])
+        cb["gpu_speedup"] = value / cb["value"]
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

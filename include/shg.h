@@ -222,6 +222,14 @@ int shg_batch_powers(shg_batch *b, double *p_fh_in, double *p_sh_out, double *et
 int64_t shg_batch_cell_steps(const shg_batch *b);
 int shg_batch_last_ms(shg_batch *b, double *ms);
 
+/* ----- Launch-shape autotuning (no reference counterpart) ------------------------------
+ * The row and column passes exist in several thread layouts (FFT elements per thread,
+ * rows or columns per workgroup).  shg_forward / shg_batch_run tune them on first use
+ * (timing each variant on the context's own data) unless the environment sets
+ * SHG_NO_TUNE=1; these calls tune explicitly or pin a choice.  Rows 0..3, columns 0..2. */
+int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
+int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
+
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass, each launched `reps` times back to back.  Clobbers the current
  * fields; the next shg_forward starts again from the shg_set_fields state. */

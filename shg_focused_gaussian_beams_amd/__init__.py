@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .params import (ShgParams, default_params, config1_params, config2_params,  # noqa: F401
+from .params import (ShgParams, default_params, config1_params, config2_params, config3_params,  # noqa: F401
                      SHG_CRYSTAL_SPPLT, SHG_CRYSTAL_PPLN, SHG_SCHEME_REFERENCE, SHG_SCHEME_STRANG)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -85,6 +85,8 @@ def load_library(path=LIB_PATH):
         "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_set_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_adjoint": (ctypes.c_int, [vp, ctypes.c_int, fp, ctypes.POINTER(AdjointResult)]),
+        "shg_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
         "shg_batch_destroy": (None, [vp]),
         "shg_batch_init_fields": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint64]),
@@ -185,6 +187,15 @@ class Context:
 
     def cell_steps(self):
         return int(self.lib.shg_cell_steps(self.h))
+
+    def tune(self, reps=10):
+        """Times every row/column-pass layout variant and keeps the fastest."""
+        r, c = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.shg_tune(self.h, reps, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    def set_variants(self, row_variant, col_variant):
+        _check(self.lib.shg_set_variants(self.h, row_variant, col_variant))
 
     def adjoint(self, checkpoint_interval=0, want_field=True):
         """Efficiency and its gradients (discrete adjoint).  Returns (result dict,

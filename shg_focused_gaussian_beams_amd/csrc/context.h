@@ -50,6 +50,7 @@ struct shg_ctx {
     double last_ms = 0.0;
     hipGraphExec_t gexec = nullptr;
     int use_graph = 1;
+    int row_v = -1, col_v = -1;      // plain-pass layout variants (-1: not tuned yet)
     std::vector<float> host_stage;   // 4 floats per cell
     AdjointBuffers adj;
 };

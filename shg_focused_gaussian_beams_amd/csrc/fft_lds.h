@@ -160,32 +160,47 @@ __device__ __forceinline__ void exchange(float2 (&a)[E], float2 (&b)[E], P la, P
     lds_gather<N, E>(a, b, la, lb, t, st);
 }
 
-// Full length-N transform of two fields held as described at the top of this file.
-// `sync` is the barrier of the caller's thread grouping (a workgroup barrier).
-// `la`/`lb` address the LDS of the two fields (element o of this transform at la[pad(o)*st]).
+// Radix plan for length N with E elements per thread: radix E while it fits, then the
+// remainder (N = E^k * r).  Stage s has stride Ns = product of the earlier radices.
+__host__ __device__ constexpr int plan_radix(int N, int E, int s) {
+    int ns = 1;
+    for (int i = 0; i < s; i++) ns *= (N / ns >= E ? E : N / ns);
+    return N / ns >= E ? E : N / ns;
+}
+__host__ __device__ constexpr int plan_stride(int N, int E, int s) {
+    int ns = 1;
+    for (int i = 0; i < s; i++) ns *= plan_radix(N, E, i);
+    return ns;
+}
+
+template <int N, int E, int S, int SIGN, typename P, typename Sync>
+__device__ __forceinline__ void fft_stages(float2 (&a)[E], float2 (&b)[E], P la, P lb, int t, int st,
+                                           const float2 *tw, Sync sync) {
+    constexpr int R = plan_radix(N, E, S), Ns = plan_stride(N, E, S);
+    static_assert(R >= 2 && E % R == 0, "radix must divide the elements per thread");
+    stage_compute<N, E, R, Ns, SIGN>(a, b, t, tw);
+    if constexpr (Ns * R < N) {
+        exchange<N, E, R, Ns>(a, b, la, lb, t, st, sync);
+        fft_stages<N, E, S + 1, SIGN>(a, b, la, lb, t, st, tw, sync);
+    }
+}
+
+// Full length-N transform of two fields held as described at the top of this file, with
+// E elements per thread (E in {4, 8, 16}; T = N/E threads).  `sync` is the barrier of the
+// caller's thread grouping; `la`/`lb` address the LDS of the two fields (element o of this
+// transform at la[pad(o)*st]).
+template <int N, int E, int SIGN, typename P, typename Sync>
+__device__ __forceinline__ void fft2fe(float2 (&a)[E], float2 (&b)[E], P la, P lb, int t, int st, const float2 *tw,
+                                       Sync sync) {
+    static_assert(N >= 64 && N <= 512 && (N & (N - 1)) == 0, "supported lengths: 64..512, powers of two");
+    fft_stages<N, E, 0, SIGN>(a, b, la, lb, t, st, tw, sync);
+}
+
+// The default plan of each length (FftPlan<N>::E elements per thread).
 template <int N, int SIGN, typename P, typename Sync>
 __device__ __forceinline__ void fft2f(float2 (&a)[FftPlan<N>::E], float2 (&b)[FftPlan<N>::E], P la, P lb, int t,
                                       int st, const float2 *tw, Sync sync) {
-    if constexpr (N == 64) {
-        stage_compute<64, 8, 8, 1, SIGN>(a, b, t, tw);
-        exchange<64, 8, 8, 1>(a, b, la, lb, t, st, sync);
-        stage_compute<64, 8, 8, 8, SIGN>(a, b, t, tw);
-    } else if constexpr (N == 128) {
-        stage_compute<128, 16, 16, 1, SIGN>(a, b, t, tw);
-        exchange<128, 16, 16, 1>(a, b, la, lb, t, st, sync);
-        stage_compute<128, 16, 8, 16, SIGN>(a, b, t, tw);
-    } else if constexpr (N == 256) {
-        stage_compute<256, 16, 16, 1, SIGN>(a, b, t, tw);
-        exchange<256, 16, 16, 1>(a, b, la, lb, t, st, sync);
-        stage_compute<256, 16, 16, 16, SIGN>(a, b, t, tw);
-    } else {
-        static_assert(N == 512, "supported lengths: 64, 128, 256, 512");
-        stage_compute<512, 8, 8, 1, SIGN>(a, b, t, tw);
-        exchange<512, 8, 8, 1>(a, b, la, lb, t, st, sync);
-        stage_compute<512, 8, 8, 8, SIGN>(a, b, t, tw);
-        exchange<512, 8, 8, 8>(a, b, la, lb, t, st, sync);
-        stage_compute<512, 8, 8, 64, SIGN>(a, b, t, tw);
-    }
+    fft2fe<N, FftPlan<N>::E, SIGN>(a, b, la, lb, t, st, tw, sync);
 }
 
 }  // namespace shg

@@ -1,26 +1,33 @@
 // pass_cfg.h — thread layouts of the row and column passes.
+//
+// Row pass <N, E, W>: W waves per workgroup, T = N/E threads per row, ROWS rows per
+// workgroup.  Column pass <N, E, CW>: CW adjacent columns per workgroup (a CW*16-byte
+// segment of every row), T threads per column.  E is the FFT's elements per thread.
 #pragma once
 #include "fft_lds.h"
 
 namespace shg {
 
-// Row pass: one wave per workgroup, T threads per row, ROWS rows per workgroup.
-template <int N> struct RowCfg {
-    static constexpr int E = FftPlan<N>::E;
-    static constexpr int T = N / E;               // threads per row
-    static constexpr int THREADS = 64;            // one wave per workgroup
-    static constexpr int ROWS = THREADS / T;      // rows per workgroup
+template <int N, int E_, int W> struct RowCfgT {
+    static constexpr int E = E_;
+    static constexpr int T = N / E;
+    static constexpr int THREADS = 64 * W;
+    static constexpr int ROWS = THREADS / T;
     static constexpr int L = LdsLen<N>::value;
+    static_assert(ROWS >= 1 && THREADS % T == 0, "a row must fit in the workgroup");
 };
 
-// Column pass: CW adjacent columns per workgroup (one 64- or 128-byte segment per row),
-// T threads per column.
-template <int N> struct ColCfg {
-    static constexpr int E = FftPlan<N>::E;
-    static constexpr int T = N / E;               // threads per column
-    static constexpr int CW = N == 512 ? 4 : 8;   // columns per workgroup
+template <int N, int E_, int CW_> struct ColCfgT {
+    static constexpr int E = E_;
+    static constexpr int T = N / E;
+    static constexpr int CW = CW_;
     static constexpr int THREADS = CW * T;
     static constexpr int L = LdsLen<N>::value;
+    static_assert(THREADS <= 1024 && THREADS % 64 == 0, "column workgroup size");
 };
+
+// Defaults (used by the THERMAL, tape and adjoint passes).
+template <int N> using RowCfg = RowCfgT<N, FftPlan<N>::E, 1>;
+template <int N> using ColCfg = ColCfgT<N, FftPlan<N>::E, (N == 512 ? 4 : 8)>;
 
 }  // namespace shg

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -65,6 +66,7 @@ struct shg_batch {
     hipGraphExec_t gexec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
+    int row_v[4] = {-1, -1, -1, -1}, col_v[4] = {-1, -1, -1, -1};   // layout variant per size class
 };
 
 namespace {
@@ -111,14 +113,52 @@ int after_fields_set(shg_batch *b) {
     return SHG_OK;
 }
 
+// Per size class, times `reps` launches of every layout variant over all the class's
+// instances and keeps the fastest (clobbers U; shg_batch_run restores it from U0).
+int tune_batch(shg_batch *b, int reps) {
+    for (int c = 0; c < 4; c++) {
+        if (!b->cls_cnt[c]) continue;
+        const int n = kSizes[c], cnt = b->cls_cnt[c];
+        const shg::InstDesc *d = b->desc + b->cls_off[c];
+        float best = 1e30f;
+        for (int v = 0; v < shg::kRowVariantCount; v++) {
+            BHIP_TRY(shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, v));
+            BHIP_TRY(hipEventRecord(b->e0, b->stream));
+            for (int r = 0; r < reps; r++) BHIP_TRY(shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, v));
+            BHIP_TRY(hipEventRecord(b->e1, b->stream));
+            BHIP_TRY(hipEventSynchronize(b->e1));
+            float ms = 0.f;
+            BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+            if (ms < best) { best = ms; b->row_v[c] = v; }
+        }
+        best = 1e30f;
+        for (int v = 0; v < shg::kColVariantCount; v++) {
+            BHIP_TRY(shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, v));
+            BHIP_TRY(hipEventRecord(b->e0, b->stream));
+            for (int r = 0; r < reps; r++) BHIP_TRY(shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, v));
+            BHIP_TRY(hipEventRecord(b->e1, b->stream));
+            BHIP_TRY(hipEventSynchronize(b->e1));
+            float ms = 0.f;
+            BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+            if (ms < best) { best = ms; b->col_v[c] = v; }
+        }
+    }
+    return SHG_OK;
+}
+
 // The lock-step schedule: START pass for everybody, then per step a column pass and an
 // AUTO row pass (MID while more steps follow, END on an instance's last step) over the
 // instances of each size that are still marching.
 int enqueue_batch(shg_batch *b) {
+    int rv[4], cv[4];
+    for (int c = 0; c < 4; c++) {
+        rv[c] = b->row_v[c] < 0 ? 0 : b->row_v[c];
+        cv[c] = b->col_v[c] < 0 ? 0 : b->col_v[c];
+    }
     for (int c = 0; c < 4; c++)
         if (b->cls_cnt[c])
             BHIP_TRY(shg::launch_row_pass(kSizes[c], b->desc + b->cls_off[c], b->cls_cnt[c], b->tw[c], 0,
-                                          shg::ROW_START, b->stream));
+                                          shg::ROW_START, b->stream, rv[c]));
     for (int s = 0; s < b->max_steps; s++) {
         int active[4];
         for (int c = 0; c < 4; c++) {
@@ -127,11 +167,12 @@ int enqueue_batch(shg_batch *b) {
             while (active[c] < (int)v.size() && v[active[c]] > s) active[c]++;
         }
         for (int c = 0; c < 4; c++)
-            if (active[c]) BHIP_TRY(shg::launch_col_pass(kSizes[c], b->desc + b->cls_off[c], active[c], b->tw[c], b->stream));
+            if (active[c])
+                BHIP_TRY(shg::launch_col_pass(kSizes[c], b->desc + b->cls_off[c], active[c], b->tw[c], b->stream, cv[c]));
         for (int c = 0; c < 4; c++)
             if (active[c])
                 BHIP_TRY(shg::launch_row_pass(kSizes[c], b->desc + b->cls_off[c], active[c], b->tw[c], s, shg::ROW_AUTO,
-                                              b->stream));
+                                              b->stream, rv[c]));
     }
     return SHG_OK;
 }
@@ -299,6 +340,14 @@ int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *
 int shg_batch_run(shg_batch *b) {
     if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_run: null batch");
     if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_run: set or init the fields first");
+    if (b->row_v[0] < 0 && b->row_v[1] < 0 && b->row_v[2] < 0 && b->row_v[3] < 0) {
+        const char *nt = std::getenv("SHG_NO_TUNE");
+        if (nt && nt[0] == '1') {
+            for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = 0;
+        } else if (int rc = tune_batch(b, 5)) {
+            return rc;
+        }
+    }
     BHIP_TRY(hipMemcpyAsync(b->U, b->U0, b->total_cells * sizeof(float4), hipMemcpyDeviceToDevice, b->stream));
     if (!b->gexec) {
         hipGraph_t graph;

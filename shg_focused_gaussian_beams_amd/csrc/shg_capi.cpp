@@ -61,12 +61,51 @@ int upload_propagators(shg_ctx *c, float temp) {
 // is in real space before and after.
 int enqueue_march_plain(shg_ctx *c, int first, int count) {
     if (count <= 0) return SHG_OK;
-    HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, first, shg::ROW_START, c->stream));
+    const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+    HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, first, shg::ROW_START, c->stream, rv));
     const int last = first + count - 1;
     for (int s = first; s <= last; s++) {
-        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
-        HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, s, s < last ? shg::ROW_MID : shg::ROW_END, c->stream));
+        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
+        HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, s, s < last ? shg::ROW_MID : shg::ROW_END, c->stream,
+                                     rv));
     }
+    return SHG_OK;
+}
+
+// Times `reps` back-to-back launches of every plain-pass layout variant on the context's
+// data and keeps the fastest row and column variants.  Clobbers U (the next march
+// restores it from U0).
+int tune_variants(shg_ctx *c, int reps) {
+    const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
+    float best_r = 1e30f, best_c = 1e30f;
+    int br = 0, bc = 0;
+    // THERMAL contexts have their own row pass (and no plain step coefficients)
+    for (int v = 0; v < (c->p.thermal ? 0 : shg::kRowVariantCount); v++) {
+        HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v));   // warm
+        HIP_TRY(hipEventRecord(c->e0, c->stream));
+        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v));
+        HIP_TRY(hipEventRecord(c->e1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+        if (ms < best_r) { best_r = ms; br = v; }
+    }
+    for (int v = 0; v < shg::kColVariantCount; v++) {
+        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v));
+        HIP_TRY(hipEventRecord(c->e0, c->stream));
+        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v));
+        HIP_TRY(hipEventRecord(c->e1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+        if (ms < best_c) { best_c = ms; bc = v; }
+    }
+    if (c->gexec && (br != c->row_v || bc != c->col_v)) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->row_v = br;
+    c->col_v = bc;
     return SHG_OK;
 }
 
@@ -91,7 +130,7 @@ int enqueue_march_thermal(shg_ctx *c) {
     float *T = c->T[c->tcur];
     HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
     for (int s = 0; s < nz; s++) {
-        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
+        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, c->col_v < 0 ? 0 : c->col_v));
         a.prev = therm_step(c, s);
         if (s < nz - 1) {
             a.cur = therm_step(c, s + 1);
@@ -190,6 +229,16 @@ int thermal_init_temperature(shg_ctx *c) {
 }
 
 int timed_march(shg_ctx *c) {
+    if (c->row_v < 0) {
+        const char *nt = std::getenv("SHG_NO_TUNE");
+        if (nt && nt[0] == '1') {
+            c->row_v = 0;
+            c->col_v = 0;
+        } else {
+            RC_TRY(tune_variants(c, 10));
+            HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
     if (!c->p.thermal && c->use_graph && !c->gexec) {
         hipGraph_t graph;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -433,6 +482,31 @@ int shg_last_forward_ms(shg_ctx *c, double *ms) {
 
 int64_t shg_cell_steps(const shg_ctx *c) { return c ? (int64_t)c->n * c->n * c->steps : 0; }
 
+int shg_tune(shg_ctx *c, int reps, int *row_variant, int *col_variant) {
+    if (!c || reps <= 0) return fail(SHG_E_INVALID, "shg_tune: bad argument");
+    RC_TRY(tune_variants(c, reps));
+    if (c->p.thermal) c->row_v = 0;
+    if (c->have_fields)
+        HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (row_variant) *row_variant = c->row_v;
+    if (col_variant) *col_variant = c->col_v;
+    return SHG_OK;
+}
+
+int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
+    if (!c || row_variant < 0 || row_variant >= shg::kRowVariantCount || col_variant < 0 ||
+        col_variant >= shg::kColVariantCount || (c->p.thermal && row_variant != 0))
+        return fail(SHG_E_INVALID, "shg_set_variants: variant out of range");
+    if (c->gexec && (row_variant != c->row_v || col_variant != c->col_v)) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->row_v = row_variant;
+    c->col_v = col_variant;
+    return SHG_OK;
+}
+
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
     float ms = 0.f;
@@ -447,14 +521,17 @@ int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
             HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->T[c->tcur], c->Q, c->DKint, a, c->stream));
     } else {
         const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
-        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream));
+        const int rv = c->row_v < 0 ? 0 : c->row_v;
+        for (int r = 0; r < reps; r++)
+            HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv));
     }
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
     *row_ms = ms / reps;
     HIP_TRY(hipEventRecord(c->e0, c->stream));
-    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream));
+    const int cv = c->col_v < 0 ? 0 : c->col_v;
+    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
     HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));

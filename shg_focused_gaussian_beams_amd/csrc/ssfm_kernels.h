@@ -37,8 +37,11 @@ enum RowMode : int {
     ROW_AUTO = 3     // ROW_MID if s + 1 < steps of the instance, else ROW_END
 };
 
-hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st);
-hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st);
+// `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..3, columns 0..2).
+constexpr int kRowVariantCount = 4, kColVariantCount = 3;
+hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
+                           int variant = 0);
+hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st, int variant = 0);
 hipError_t launch_power(const float4 *U, int64_t ncell, double2 *part, int nparts, double2 *out, hipStream_t st);
 // Per-instance FP64 sums of |FH|^2 and |SH|^2: out[i] for table entry i (one block each).
 hipError_t launch_power_batch(const InstDesc *d, const int *ncell, int ninst, double2 *out, hipStream_t st);

@@ -27,12 +27,13 @@ __device__ __forceinline__ void rk4(float2 &P, float2 &S, float4 c, const NlCons
     rk4_cell(P, S, c, c, c, k.lossP, k.lossS, k.tpa, k.h6);
 }
 
-// Row pass over rows [blockIdx.x*ROWS, +ROWS) of instance blockIdx.y (see RowMode).
-template <int N>
-__global__ __launch_bounds__(64) void k_row_pass(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
-                                                 int step, int mode) {
-    using C = RowCfg<N>;
-    constexpr int E = C::E, T = C::T;
+// Row pass over rows [blockIdx.x*ROWS, +ROWS) of instance blockIdx.y (see RowMode);
+// layout <N, E, W> as in pass_cfg.h.
+template <int N, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
+                                                     int step, int mode) {
+    using C = RowCfgT<N, E, W>;
+    constexpr int T = C::T;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[blockIdx.y];
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(64) void k_row_pass(const InstDesc *__restrict__ de
     }
     auto sync = [] { block_sync(); };
     if (mode != ROW_START) {
-        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
         const float4 c = d.coef[step];
 #pragma unroll
         for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(64) void k_row_pass(const InstDesc *__restrict__ de
         const float4 c = d.coef[mode == ROW_START ? step : step + 1];
 #pragma unroll
         for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
-        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        fft2fe<N, E, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
     }
 #pragma unroll
     for (int j = 0; j < E; j++) Ur[t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
@@ -69,12 +70,12 @@ __global__ __launch_bounds__(64) void k_row_pass(const InstDesc *__restrict__ de
 
 // Column pass over columns [blockIdx.x*CW, +CW) of instance blockIdx.y: C_bwd, x H/(N*N),
 // C_fwd.  G holds the separable propagator: Gx_p[N], Gy_p[N], Gx_s[N], Gy_s[N] (Gy
-// carries the 1/N^2 of FFTscale).
-template <int N>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(const InstDesc *__restrict__ descs,
-                                                                 const float2 *__restrict__ twg) {
-    using C = ColCfg<N>;
-    constexpr int E = C::E, T = C::T, CW = C::CW;
+// carries the 1/N^2 of FFTscale).  Layout <N, E, CW> as in pass_cfg.h.
+template <int N, int E, int CW>
+__global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__restrict__ descs,
+                                                          const float2 *__restrict__ twg) {
+    using C = ColCfgT<N, E, CW>;
+    constexpr int T = C::T;
     __shared__ float2 lds[2][C::L * CW];
     __shared__ float2 tw[N];
     const InstDesc d = descs[blockIdx.y];
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(const InstDesc 
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { block_sync(); };
-    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
     const float2 gxp = G[x], gxs = G[2 * N + x];
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_pass(const InstDesc 
         fa[j] = c_mul(c_mul(gxp, G[N + y]), fa[j]);
         fb[j] = c_mul(c_mul(gxs, G[3 * N + y]), fb[j]);
     }
-    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++)
         U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
@@ -167,38 +168,63 @@ __global__ __launch_bounds__(256) void k_power_batch(const InstDesc *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Launchers.
+// Launchers.  Variant tables (index 0 = default): rows {E, waves}, columns {E, CW}.
 
-template <int N>
+template <int N, int E, int W>
 static hipError_t row_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
-    using C = RowCfg<N>;
-    hipLaunchKernelGGL(k_row_pass<N>, dim3(N / C::ROWS, ninst), dim3(C::THREADS), 0, st, d, tw, step, mode);
+    using C = RowCfgT<N, E, W>;
+    hipLaunchKernelGGL((k_row_pass<N, E, W>), dim3(N / C::ROWS, ninst), dim3(C::THREADS), 0, st, d, tw, step, mode);
     return hipGetLastError();
 }
-template <int N> static hipError_t col_launch(const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
-    using C = ColCfg<N>;
-    hipLaunchKernelGGL(k_col_pass<N>, dim3(N / C::CW, ninst), dim3(C::THREADS), 0, st, d, tw);
+template <int N, int E, int CW>
+static hipError_t col_launch(const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
+    using C = ColCfgT<N, E, CW>;
+    hipLaunchKernelGGL((k_col_pass<N, E, CW>), dim3(N / CW, ninst), dim3(C::THREADS), 0, st, d, tw);
     return hipGetLastError();
 }
 
-hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
+#define ROWV(N, E, W) return row_launch<N, E, W>(d, ninst, tw, step, mode, st)
+#define COLV(N, E, CW) return col_launch<N, E, CW>(d, ninst, tw, st)
+
+hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
+                           int variant) {
     if (ninst <= 0 || ninst > 65535 || step < 0 || mode < ROW_START || mode > ROW_AUTO) return hipErrorInvalidValue;
-    switch (n) {
-        case 64: return row_launch<64>(d, ninst, tw, step, mode, st);
-        case 128: return row_launch<128>(d, ninst, tw, step, mode, st);
-        case 256: return row_launch<256>(d, ninst, tw, step, mode, st);
-        case 512: return row_launch<512>(d, ninst, tw, step, mode, st);
+    switch (n * 16 + variant) {
+        case 64 * 16 + 0: ROWV(64, 8, 1);
+        case 64 * 16 + 1: ROWV(64, 4, 1);
+        case 64 * 16 + 2: ROWV(64, 8, 4);
+        case 64 * 16 + 3: ROWV(64, 16, 1);
+        case 128 * 16 + 0: ROWV(128, 16, 1);
+        case 128 * 16 + 1: ROWV(128, 8, 1);
+        case 128 * 16 + 2: ROWV(128, 4, 1);
+        case 128 * 16 + 3: ROWV(128, 16, 2);
+        case 256 * 16 + 0: ROWV(256, 16, 1);
+        case 256 * 16 + 1: ROWV(256, 8, 1);
+        case 256 * 16 + 2: ROWV(256, 4, 1);
+        case 256 * 16 + 3: ROWV(256, 16, 2);
+        case 512 * 16 + 0: ROWV(512, 8, 1);
+        case 512 * 16 + 1: ROWV(512, 16, 1);
+        case 512 * 16 + 2: ROWV(512, 4, 2);
+        case 512 * 16 + 3: ROWV(512, 8, 4);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
+hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st, int variant) {
     if (ninst <= 0 || ninst > 65535) return hipErrorInvalidValue;
-    switch (n) {
-        case 64: return col_launch<64>(d, ninst, tw, st);
-        case 128: return col_launch<128>(d, ninst, tw, st);
-        case 256: return col_launch<256>(d, ninst, tw, st);
-        case 512: return col_launch<512>(d, ninst, tw, st);
+    switch (n * 16 + variant) {
+        case 64 * 16 + 0: COLV(64, 8, 8);
+        case 64 * 16 + 1: COLV(64, 4, 8);
+        case 64 * 16 + 2: COLV(64, 8, 16);
+        case 128 * 16 + 0: COLV(128, 16, 8);
+        case 128 * 16 + 1: COLV(128, 8, 8);
+        case 128 * 16 + 2: COLV(128, 4, 8);
+        case 256 * 16 + 0: COLV(256, 16, 8);
+        case 256 * 16 + 1: COLV(256, 8, 8);
+        case 256 * 16 + 2: COLV(256, 4, 8);
+        case 512 * 16 + 0: COLV(512, 8, 4);
+        case 512 * 16 + 1: COLV(512, 16, 4);
+        case 512 * 16 + 2: COLV(512, 4, 4);
         default: return hipErrorInvalidValue;
     }
 }

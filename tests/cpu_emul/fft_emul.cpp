@@ -3,7 +3,7 @@
 // becomes a std::barrier.  LDS is a bounds-checked array.  Reads complex input from
 // stdin-free arguments: generates a seeded field, runs rows or columns mode, prints
 // the transformed values for comparison with numpy.
-//   usage: fft_emul <N> <sign +1|-1> <mode row|col> <seed>
+//   usage: fft_emul <N> <sign +1|-1> <mode row|col> <seed> [E elements per thread, 0 = default plan]
 #include <barrier>
 #include <cmath>
 #include <cstdio>
@@ -47,28 +47,34 @@ int main(int argc, char **argv) {
     const int L = N + N / 16;
     std::vector<float2> lds_a((size_t)L * CW), lds_b((size_t)L * CW);
     std::vector<float2> outA(A.size()), outB(B.size());
-    int E = 0;
-    if (N == 64 || N == 512) E = 8; else E = 16;
+    int E = argc > 5 ? std::atoi(argv[5]) : 0;
+    if (E == 0) E = (N == 64 || N == 512) ? 8 : 16;
+    if (E != 4 && E != 8 && E != 16) return 2;
     const int T = N / E;
     std::barrier bar(T * CW);
     auto worker = [&](int tid) {
         const int c = tid % CW, t = tid / CW;
         Lds la{lds_a.data() + c, -c, (long)L * CW - c}, lb{lds_b.data() + c, -c, (long)L * CW - c};
         auto sync = [&] { bar.arrive_and_wait(); };
-        auto run = [&](auto NN) {
+        auto run = [&](auto NN, auto EE) {
             constexpr int n = decltype(NN)::value;
-            constexpr int e = shg::FftPlan<n>::E;
+            constexpr int e = decltype(EE)::value;
             float2 a[e], b[e];
             for (int j = 0; j < e; j++) { a[j] = A[(size_t)(t + j * T) * CW + c]; b[j] = B[(size_t)(t + j * T) * CW + c]; }
-            if (sign > 0) shg::fft2f<n, +1>(a, b, la, lb, t, CW, tw.data(), sync);
-            else shg::fft2f<n, -1>(a, b, la, lb, t, CW, tw.data(), sync);
+            if (sign > 0) shg::fft2fe<n, e, +1>(a, b, la, lb, t, CW, tw.data(), sync);
+            else shg::fft2fe<n, e, -1>(a, b, la, lb, t, CW, tw.data(), sync);
             for (int j = 0; j < e; j++) { outA[(size_t)(t + j * T) * CW + c] = a[j]; outB[(size_t)(t + j * T) * CW + c] = b[j]; }
         };
+        auto runE = [&](auto NN) {
+            if (E == 4) run(NN, std::integral_constant<int, 4>{});
+            else if (E == 8) run(NN, std::integral_constant<int, 8>{});
+            else run(NN, std::integral_constant<int, 16>{});
+        };
         switch (N) {
-            case 64: run(std::integral_constant<int, 64>{}); break;
-            case 128: run(std::integral_constant<int, 128>{}); break;
-            case 256: run(std::integral_constant<int, 256>{}); break;
-            case 512: run(std::integral_constant<int, 512>{}); break;
+            case 64: runE(std::integral_constant<int, 64>{}); break;
+            case 128: runE(std::integral_constant<int, 128>{}); break;
+            case 256: runE(std::integral_constant<int, 256>{}); break;
+            case 512: runE(std::integral_constant<int, 512>{}); break;
         }
     };
     std::vector<std::thread> th;

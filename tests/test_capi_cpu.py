@@ -67,10 +67,12 @@ def test_device_fft_index_logic_in_host_build(tmp_path):
     exe = tmp_path / "fft_emul"
     subprocess.run(["g++", "-std=c++20", "-O1", "-pthread", "-o", str(exe),
                     os.path.join(ROOT, "tests", "cpu_emul", "fft_emul.cpp")], check=True)
-    for n in (64, 128, 256, 512):
+    for n, e in [(n, e) for n in (64, 128, 256, 512) for e in (0, 4, 8, 16) if n // max(e, 1) <= 128]:
         for sign in (1, -1):
             for mode in ("row", "col"):
-                out = subprocess.run([str(exe), str(n), str(sign), mode, "11"], check=True,
+                if mode == "col" and e == 4 and n == 512:
+                    continue
+                out = subprocess.run([str(exe), str(n), str(sign), mode, "11", str(e)], check=True,
                                      capture_output=True, text=True).stdout
                 d = np.array([list(map(float, l.split())) for l in out.strip().split("\n")])
                 cw = 1 if mode == "row" else (4 if n == 512 else 8)

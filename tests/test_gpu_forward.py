@@ -88,3 +88,23 @@ def test_input_power_matches_closed_form(gpu):
         ctx.set_fields(fh0, np.zeros_like(fh0))
         p_in, _, _ = ctx.powers()
     assert abs(p_in - p.power) / p.power < 2e-2, p_in
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_every_layout_variant_matches_oracle32(gpu, n):
+    # the autotuner may pick any row/column layout; each must meet the parity bar
+    p = config1_params(nx=n, ny=n, nz=10, lx=4.0 * n, ly=4.0 * n)
+    fh0, sh0 = _inputs(p)
+    rfh, rsh = orc.cwes(p, fh0, sh0, prec=32)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(fh0, sh0)
+        for rv in range(4):
+            for cv in range(3):
+                ctx.set_variants(rv, cv)
+                ctx.forward()
+                fh, sh = ctx.get_fields()
+                e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
+                assert max(e) <= TOL_FIELD, (rv, cv, e)
+        tuned = ctx.tune(reps=3)
+        print(f"\n  n={n} tuned variants {tuned}")
