@@ -27,7 +27,9 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st);
 hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st);
-int adj_row_blocks(int n);
-int adj_col_blocks(int n);
+// Partial records per step that the adjoint row pass (5 doubles each, one per wavefront)
+// and column pass (1 double per workgroup) write.
+int adj_row_parts(int n);
+int adj_col_parts(int n);
 
 }  // namespace shg

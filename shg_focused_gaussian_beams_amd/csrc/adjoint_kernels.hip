@@ -98,15 +98,25 @@ __device__ __forceinline__ void wave_store(AdjAcc a, double *out) {
 }
 
 // ---------------------------------------------------------------------------
+// Layouts of the tape and adjoint passes: four elements per thread, which gives the most
+// wavefronts per pass (one per 256-point row) and leaves room for the adjoint's
+// recomputed RK4 stages and FP64 accumulators.
+template <int N> struct AdjLayout;
+template <> struct AdjLayout<64>  { using Row = RowCfgT<64, 4, 1>;  using Col = ColCfgT<64, 4, 8>; };
+template <> struct AdjLayout<128> { using Row = RowCfgT<128, 4, 1>; using Col = ColCfgT<128, 4, 8>; };
+template <> struct AdjLayout<256> { using Row = RowCfgT<256, 4, 1>; using Col = ColCfgT<256, 4, 8>; };
+template <> struct AdjLayout<512> { using Row = RowCfgT<512, 4, 2>; using Col = ColCfgT<512, 4, 4>; };
+
 // Forward row pass with tape (single instance, modes as RowMode without AUTO).
 // START at s0: tape_a[0] = input, then RK4_s0, R_bwd.
 // MID at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s, tape_a[s+1-s0] = A_{s+1}, RK4_{s+1}, R_bwd.
 // END at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s  (real-space state of the segment end).
 // A null tape writes nothing (checkpoint-only forward: START copies its input to ckpt).
 template <int N>
-__global__ __launch_bounds__(64) void k_row_tape(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
-                                                 int step, int mode, TapeArgs tp) {
-    using C = RowCfg<N>;
+__global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_row_tape(const InstDesc *__restrict__ descs,
+                                                                          const float2 *__restrict__ twg, int step,
+                                                                          int mode, TapeArgs tp) {
+    using C = typename AdjLayout<N>::Row;
     constexpr int E = C::E, T = C::T;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
@@ -132,7 +142,7 @@ __global__ __launch_bounds__(64) void k_row_tape(const InstDesc *__restrict__ de
 #pragma unroll
             for (int j = 0; j < E; j++) dst[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
     } else {
-        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
         const float4 c = d.coef[step];
 #pragma unroll
         for (int j = 0; j < E; j++) {
@@ -146,7 +156,7 @@ __global__ __launch_bounds__(64) void k_row_tape(const InstDesc *__restrict__ de
         const float4 c = d.coef[mode == ROW_START ? step : step + 1];
 #pragma unroll
         for (int j = 0; j < E; j++) rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
-        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        fft2fe<N, E, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
     }
 #pragma unroll
     for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
@@ -154,9 +164,10 @@ __global__ __launch_bounds__(64) void k_row_tape(const InstDesc *__restrict__ de
 
 // Forward column pass with tape: tape_spec[s-s0] = the 2D spectrum before x H.
 template <int N>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_tape(const InstDesc *__restrict__ descs,
-                                                                 const float2 *__restrict__ twg, int step, TapeArgs tp) {
-    using C = ColCfg<N>;
+__global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_col_tape(const InstDesc *__restrict__ descs,
+                                                                          const float2 *__restrict__ twg, int step,
+                                                                          TapeArgs tp) {
+    using C = typename AdjLayout<N>::Col;
     constexpr int E = C::E, T = C::T, CW = C::CW;
     __shared__ float2 lds[2][C::L * CW];
     __shared__ float2 tw[N];
@@ -174,7 +185,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_tape(const InstDesc 
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { __syncthreads(); };
-    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
     const size_t soff = (size_t)(step - tp.s0) * N * N;
     const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
 #pragma unroll
@@ -184,7 +195,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_tape(const InstDesc 
         fa[j] = c_mul(c_mul(gxp, d.G[N + y]), fa[j]);
         fb[j] = c_mul(c_mul(gxs, d.G[3 * N + y]), fb[j]);
     }
-    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++)
         d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
@@ -195,21 +206,25 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_col_tape(const InstDesc 
 // ADJ_START at s: RK4_s^T at A''_s, R_bwd.
 // ADJ_MID   at s: R_fwd, RK4_s^T at A_s, RK4_{s-1}^T at A''_{s-1}, R_bwd.
 // ADJ_END   at s: R_fwd, RK4_s^T at A_s (real-space lambda of the segment start).
-// g1[(s*nblk + blk)*5] gets the sensitivities of the first RK4 of step s (at A_s),
-// g2[...] those of the second (at A''_s).
+// g1[(s*parts + wave)*5] gets the sensitivities of the first RK4 of step s (at A_s),
+// g2[...] those of the second (at A''_s); parts = wavefronts per pass.  Between the
+// transforms the thread's cells are parked in the row's LDS slots and visited by a rolled
+// loop (RK4^T is too large to unroll per element).
 template <int N>
-__global__ __launch_bounds__(64) void k_adj_row(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
-                                                int step, int mode, TapeArgs tp, double *__restrict__ g1,
-                                                double *__restrict__ g2) {
-    using C = RowCfg<N>;
-    constexpr int E = C::E, T = C::T;
+__global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_adj_row(const InstDesc *__restrict__ descs,
+                                                                         const float2 *__restrict__ twg, int step,
+                                                                         int mode, TapeArgs tp,
+                                                                         double *__restrict__ g1,
+                                                                         double *__restrict__ g2) {
+    using C = typename AdjLayout<N>::Row;
+    constexpr int E = C::E, T = C::T, W = C::THREADS / 64;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
     const int lane = threadIdx.x;
     const int t = lane % T, rr = lane / T;
     const int row = blockIdx.x * C::ROWS + rr;
-    const int nblk = gridDim.x;
+    const size_t part = (size_t)blockIdx.x * W + (lane >> 6), parts = (size_t)gridDim.x * W;
     for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
     __syncthreads();
     const size_t n2 = (size_t)N * N, base = (size_t)row * N;
@@ -221,32 +236,55 @@ __global__ __launch_bounds__(64) void k_adj_row(const InstDesc *__restrict__ des
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { __syncthreads(); };
+    float2 *la = lds[0][rr], *lb = lds[1][rr];
+    if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, la, lb, t, 1, tw, sync);
+    __syncthreads();                          // the transform's last LDS reads are done
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        la[lds_pad(t + j * T)] = fa[j];
+        lb[lds_pad(t + j * T)] = fb[j];
+    }
+    // each thread reads back only the slots it wrote: no barrier
     const int ls = step - tp.s0;
     if (mode != ADJ_START) {
-        fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
         const float4 c = d.coef[step];
+        const float4 *tape = tp.a + (size_t)ls * n2 + base + t;
         AdjAcc acc;
         acc_zero(acc);
-#pragma unroll 2
+#pragma unroll 1
         for (int j = 0; j < E; j++) {
-            const float4 a = tp.a[(size_t)ls * n2 + base + t + j * T];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), fa[j], fb[j], c, d.nl, acc);
+            const int o = lds_pad(t + j * T);
+            float2 P = la[o], S = lb[o];
+            const float4 a = tape[j * T];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c, d.nl, acc);
+            la[o] = P;
+            lb[o] = S;
         }
-        wave_store(acc, g1 + ((size_t)step * nblk + blockIdx.x) * 5);
+        wave_store(acc, g1 + ((size_t)step * parts + part) * 5);
     }
     if (mode != ADJ_END) {
         const int sp = mode == ADJ_START ? step : step - 1;   // step of the second RK4
         const float4 c = d.coef[sp];
+        const float4 *tape = tp.a2 + (size_t)(sp - tp.s0) * n2 + base + t;
         AdjAcc acc;
         acc_zero(acc);
-#pragma unroll 2
+#pragma unroll 1
         for (int j = 0; j < E; j++) {
-            const float4 a = tp.a2[(size_t)(sp - tp.s0) * n2 + base + t + j * T];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), fa[j], fb[j], c, d.nl, acc);
+            const int o = lds_pad(t + j * T);
+            float2 P = la[o], S = lb[o];
+            const float4 a = tape[j * T];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c, d.nl, acc);
+            la[o] = P;
+            lb[o] = S;
         }
-        wave_store(acc, g2 + ((size_t)sp * nblk + blockIdx.x) * 5);
-        fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+        wave_store(acc, g2 + ((size_t)sp * parts + part) * 5);
     }
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        fa[j] = la[lds_pad(t + j * T)];
+        fb[j] = lb[lds_pad(t + j * T)];
+    }
+    if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, la, lb, t, 1, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
@@ -255,15 +293,16 @@ __global__ __launch_bounds__(64) void k_adj_row(const InstDesc *__restrict__ des
 // sum Re(conj(lambda_hat) (Gx Gy) x_hat * i (a_p, a_s) (qx + qy)) over the block's cells,
 // the sensitivity of J to a temperature change of the propagator (d log H/dT = i a q).
 template <int N>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_adj_col(const InstDesc *__restrict__ descs,
-                                                                const float2 *__restrict__ twg, int step, TapeArgs tp,
-                                                                const float2 *__restrict__ qxy, float a_p, float a_s,
-                                                                double *__restrict__ gH) {
-    using C = ColCfg<N>;
+__global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_adj_col(const InstDesc *__restrict__ descs,
+                                                                         const float2 *__restrict__ twg, int step,
+                                                                         TapeArgs tp, const float2 *__restrict__ qxy,
+                                                                         float a_p, float a_s,
+                                                                         double *__restrict__ gH) {
+    using C = typename AdjLayout<N>::Col;
     constexpr int E = C::E, T = C::T, CW = C::CW;
     __shared__ float2 lds[2][C::L * CW];
     __shared__ float2 tw[N];
-    __shared__ double red[ColCfg<N>::THREADS / 64];
+    __shared__ double red[C::THREADS / 64];
     const InstDesc d = descs[0];
     const int tid = threadIdx.x;
     const int c = tid % CW, t = tid / CW;
@@ -278,7 +317,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_adj_col(const InstDesc *
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { __syncthreads(); };
-    fft2f<N, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
     const size_t soff = (size_t)(step - tp.s0) * N * N;
     const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
     const float qx = qxy[x].x;
@@ -296,7 +335,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_adj_col(const InstDesc *
         fa[j] = c_mul(cj(hp), fa[j]);
         fb[j] = c_mul(cj(hs), fb[j]);
     }
-    fft2f<N, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++)
         d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
@@ -321,27 +360,8 @@ __global__ __launch_bounds__(256) void k_adj_seed(const float4 *__restrict__ U, 
 }
 
 // ---------------------------------------------------------------------------
-template <int N> static int row_blocks() { return N / RowCfg<N>::ROWS; }
-template <int N> static int col_blocks() { return N / ColCfg<N>::CW; }
-
-int adj_row_blocks(int n) {
-    switch (n) {
-        case 64: return row_blocks<64>();
-        case 128: return row_blocks<128>();
-        case 256: return row_blocks<256>();
-        case 512: return row_blocks<512>();
-        default: return 0;
-    }
-}
-int adj_col_blocks(int n) {
-    switch (n) {
-        case 64: return col_blocks<64>();
-        case 128: return col_blocks<128>();
-        case 256: return col_blocks<256>();
-        case 512: return col_blocks<512>();
-        default: return 0;
-    }
-}
+template <int N> static int row_blocks() { return N / AdjLayout<N>::Row::ROWS; }
+template <int N> static int col_blocks() { return N / AdjLayout<N>::Col::CW; }
 
 #define SHG_DISPATCH(n, CALL)                \
     switch (n) {                             \
@@ -352,31 +372,53 @@ int adj_col_blocks(int n) {
         default: return hipErrorInvalidValue;            \
     }
 
+template <int N> static int row_parts() { return row_blocks<N>() * (AdjLayout<N>::Row::THREADS / 64); }
+
+int adj_row_parts(int n) {
+    switch (n) {
+        case 64: return row_parts<64>();
+        case 128: return row_parts<128>();
+        case 256: return row_parts<256>();
+        case 512: return row_parts<512>();
+        default: return 0;
+    }
+}
+int adj_col_parts(int n) {
+    switch (n) {
+        case 64: return col_blocks<64>();
+        case 128: return col_blocks<128>();
+        case 256: return col_blocks<256>();
+        case 512: return col_blocks<512>();
+        default: return 0;
+    }
+}
+
 hipError_t launch_row_tape(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
                            hipStream_t st) {
     if (mode != ROW_START && mode != ROW_MID && mode != ROW_END) return hipErrorInvalidValue;
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_row_tape<NN>, dim3(row_blocks<NN>()), dim3(64), 0, st, d, tw, step, mode, tp));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_row_tape<NN>, dim3(row_blocks<NN>()), dim3(AdjLayout<NN>::Row::THREADS), 0,
+                                       st, d, tw, step, mode, tp));
     return hipGetLastError();
 }
 
 hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(ColCfg<NN>::THREADS), 0, st, d, tw,
-                                       step, tp));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(AdjLayout<NN>::Col::THREADS), 0,
+                                       st, d, tw, step, tp));
     return hipGetLastError();
 }
 
 hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
                           double *g1, double *g2, hipStream_t st) {
     if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(row_blocks<NN>()), dim3(64), 0, st, d, tw, step, mode, tp, g1,
-                                       g2));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(row_blocks<NN>()), dim3(AdjLayout<NN>::Row::THREADS), 0,
+                                       st, d, tw, step, mode, tp, g1, g2));
     return hipGetLastError();
 }
 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(ColCfg<NN>::THREADS), 0, st, d, tw,
-                                       step, tp, qxy, a_p, a_s, gH));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(AdjLayout<NN>::Col::THREADS), 0,
+                                       st, d, tw, step, tp, qxy, a_p, a_s, gH));
     return hipGetLastError();
 }
 

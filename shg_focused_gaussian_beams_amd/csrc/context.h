@@ -39,6 +39,7 @@ struct shg_ctx {
     float *T[2] = {nullptr, nullptr};
     int tcur = 0;                    // index of the buffer holding the latest temperature
     float *Q = nullptr, *DKint = nullptr;
+    float2 *ntab = nullptr;          // {n_p(T), n_s(T)} per heat voxel, refreshed per march
     double *jpart = nullptr, *jres = nullptr;
     int njpart = 0;
     shg::JacobiConst jc{};

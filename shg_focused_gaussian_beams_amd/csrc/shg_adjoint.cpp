@@ -46,9 +46,9 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
     const int M = c->steps;
     if (!b.lam) {
         if (hipMalloc(&b.lam, n2 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&b.g1, (size_t)M * shg::adj_row_blocks(c->n) * 5 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.g2, (size_t)M * shg::adj_row_blocks(c->n) * 5 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.gH, (size_t)M * shg::adj_col_blocks(c->n) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.g1, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.g2, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.gH, (size_t)M * shg::adj_col_parts(c->n) * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.qxy, (size_t)c->n * sizeof(float2)) != hipSuccess ||
             hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess)
             return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
@@ -106,7 +106,7 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
     ARC_TRY(ensure_buffers(c, K, nseg));
     AdjointBuffers &b = c->adj;
     hipStream_t st = c->stream;
-    const int rb = shg::adj_row_blocks(n), cb = shg::adj_col_blocks(n);
+    const int rb = shg::adj_row_parts(n), cb = shg::adj_col_parts(n);
 
     // 1. checkpointed forward
     AHIP_TRY(hipEventRecord(c->e0, st));

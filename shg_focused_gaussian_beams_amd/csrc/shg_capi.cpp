@@ -75,29 +75,61 @@ int enqueue_march_plain(shg_ctx *c, int first, int count) {
 // Times `reps` back-to-back launches of every plain-pass layout variant on the context's
 // data and keeps the fastest row and column variants.  Clobbers U (the next march
 // restores it from U0).
+shg::ThermStep therm_step(const shg_ctx *c, int s);
+
+// Average time of `reps` launches of `launch` (after one warm-up launch), in ms.
+template <typename F> static int time_launches(shg_ctx *c, int reps, F launch, float *ms) {
+    HIP_TRY(launch());
+    HIP_TRY(hipEventRecord(c->e0, c->stream));
+    for (int r = 0; r < reps; r++) HIP_TRY(launch());
+    HIP_TRY(hipEventRecord(c->e1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->e1));
+    HIP_TRY(hipEventElapsedTime(ms, c->e0, c->e1));
+    return SHG_OK;
+}
+
+// Row-pass arguments of a THERMAL middle step (every stage of the pass switched on).
+static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
+    shg::ThermRowArgs a{};
+    a.k = c->tc;
+    a.prev = therm_step(c, 0);
+    a.cur = therm_step(c, 1);
+    a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
+    return a;
+}
+
+// Times every row and column layout on this context's size and keeps the fastest.  The
+// passes run on the context's buffers: the caller restores the fields afterwards (a
+// THERMAL march rewrites DKint and every heat-source slice it touches from scratch).
 int tune_variants(shg_ctx *c, int reps) {
     const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
     float best_r = 1e30f, best_c = 1e30f;
     int br = 0, bc = 0;
-    // THERMAL contexts have their own row pass (and no plain step coefficients)
-    for (int v = 0; v < (c->p.thermal ? 0 : shg::kRowVariantCount); v++) {
-        HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v));   // warm
-        HIP_TRY(hipEventRecord(c->e0, c->stream));
-        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v));
-        HIP_TRY(hipEventRecord(c->e1, c->stream));
-        HIP_TRY(hipEventSynchronize(c->e1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
-        if (ms < best_r) { best_r = ms; br = v; }
+    if (c->p.thermal) {
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
+                                        c->tc.lp, c->tc.ls, c->stream));
+        const shg::ThermRowArgs a = therm_mid_args(c);
+        for (int v = 0; v < shg::kThermRowVariantCount; v++) {
+            float ms = 0.f;
+            RC_TRY(time_launches(c, reps, [&] {
+                return shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a,
+                                                    c->stream, v);
+            }, &ms));
+            if (ms < best_r) { best_r = ms; br = v; }
+        }
+    } else {
+        for (int v = 0; v < shg::kRowVariantCount; v++) {
+            float ms = 0.f;
+            RC_TRY(time_launches(c, reps, [&] {
+                return shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v);
+            }, &ms));
+            if (ms < best_r) { best_r = ms; br = v; }
+        }
     }
     for (int v = 0; v < shg::kColVariantCount; v++) {
-        HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v));
-        HIP_TRY(hipEventRecord(c->e0, c->stream));
-        for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v));
-        HIP_TRY(hipEventRecord(c->e1, c->stream));
-        HIP_TRY(hipEventSynchronize(c->e1));
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+        RC_TRY(time_launches(c, reps, [&] { return shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v); },
+                             &ms));
         if (ms < best_c) { best_c = ms; bc = v; }
     }
     if (c->gexec && (br != c->row_v || bc != c->col_v)) {
@@ -127,8 +159,11 @@ int enqueue_march_thermal(shg_ctx *c) {
     a.k = c->tc;
     a.cur = therm_step(c, 0);
     a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
-    float *T = c->T[c->tcur];
-    HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
+    HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal, c->tc.lp,
+                                    c->tc.ls, c->stream));
+    const float2 *NT = c->ntab;
+    const int tv = c->row_v < 0 ? 0 : c->row_v;
+    HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
     for (int s = 0; s < nz; s++) {
         HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, c->col_v < 0 ? 0 : c->col_v));
         a.prev = therm_step(c, s);
@@ -139,7 +174,7 @@ int enqueue_march_thermal(shg_ctx *c) {
         } else {
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV;
         }
-        HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, T, c->Q, c->DKint, a, c->stream));
+        HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
     }
     return SHG_OK;
 }
@@ -168,7 +203,7 @@ void release(shg_ctx *c) {
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
                       (void *)c->d_coef, (void *)c->d_desc,
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
-                      (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt, (void *)c->adj.a,
+                      (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt, (void *)c->adj.a,
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
                       (void *)c->adj.qxy, (void *)c->adj.dlam})
         if (ptr) (void)hipFree(ptr);
@@ -367,6 +402,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         if (hipMalloc(&c->Uout, ncell * sizeof(float4)) != hipSuccess ||
             hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
             hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->ntab, n3 * sizeof(float2)) != hipSuccess ||
             hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
             hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
@@ -485,7 +521,6 @@ int64_t shg_cell_steps(const shg_ctx *c) { return c ? (int64_t)c->n * c->n * c->
 int shg_tune(shg_ctx *c, int reps, int *row_variant, int *col_variant) {
     if (!c || reps <= 0) return fail(SHG_E_INVALID, "shg_tune: bad argument");
     RC_TRY(tune_variants(c, reps));
-    if (c->p.thermal) c->row_v = 0;
     if (c->have_fields)
         HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -495,8 +530,8 @@ int shg_tune(shg_ctx *c, int reps, int *row_variant, int *col_variant) {
 }
 
 int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
-    if (!c || row_variant < 0 || row_variant >= shg::kRowVariantCount || col_variant < 0 ||
-        col_variant >= shg::kColVariantCount || (c->p.thermal && row_variant != 0))
+    const int nrow = c && c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
+    if (!c || row_variant < 0 || row_variant >= nrow || col_variant < 0 || col_variant >= shg::kColVariantCount)
         return fail(SHG_E_INVALID, "shg_set_variants: variant out of range");
     if (c->gexec && (row_variant != c->row_v || col_variant != c->col_v)) {
         (void)hipGraphExecDestroy(c->gexec);
@@ -510,15 +545,16 @@ int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
     float ms = 0.f;
+    if (c->p.thermal)
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
+                                        c->tc.lp, c->tc.ls, c->stream));
     HIP_TRY(hipEventRecord(c->e0, c->stream));
     if (c->p.thermal) {
-        shg::ThermRowArgs a{};
-        a.k = c->tc;
-        a.prev = therm_step(c, 0);
-        a.cur = therm_step(c, 1);
-        a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
+        const shg::ThermRowArgs a = therm_mid_args(c);
+        const int tv = c->row_v < 0 ? 0 : c->row_v;
         for (int r = 0; r < reps; r++)
-            HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->T[c->tcur], c->Q, c->DKint, a, c->stream));
+            HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a, c->stream,
+                                                 tv));
     } else {
         const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
         const int rv = c->row_v < 0 ? 0 : c->row_v;

@@ -51,8 +51,13 @@ struct ThermRowArgs {
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                          double *part, int nparts, hipStream_t st);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
-hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q,
-                                   float *DKint, const ThermRowArgs &a, hipStream_t st);
+// {n_p(T), n_s(T)} of every heat voxel (T is frozen during one march, so the row pass
+// reads the two indices instead of evaluating the FP64 Sellmeier fits per cell and step).
+hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, int crystal, float lp, float ls, hipStream_t st);
+// Row-pass layouts <E, W> per size (the tuner times all of them): variant v of size n.
+constexpr int kThermRowVariantCount = 4;
+hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0);
 // Grid size the Jacobi launcher uses (number of residual partials it writes).
 int jacobi_blocks(int nx, int ny, int hz);
 

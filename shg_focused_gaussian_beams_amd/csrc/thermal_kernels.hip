@@ -118,13 +118,24 @@ struct CellPhys {
     float np, ns, kp, ks, dk, tpa;
 };
 
-// n_p, n_s, the coupling magnitudes, the phase mismatch (src/headers/PhaseMatching.h:47)
-// and the TPA coefficient at temperature T.
-__device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float T) {
+// Refractive indices of every heat voxel, {n_p(T), n_s(T)}: the temperature is frozen
+// during a march, so the FP64 Sellmeier evaluation runs once per voxel instead of twice
+// per cell per row pass (the values are the same floats).
+__global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T, float2 *__restrict__ tab, int64_t n,
+                                                     int crystal, float lp, float ls) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float t = T[i];
+    tab[i] = make_float2(index_dev(crystal, lp, t), index_dev(crystal, ls, t));
+}
+
+// The coupling magnitudes, the phase mismatch (src/headers/PhaseMatching.h:47) and the
+// TPA coefficient from the cell's indices, with the reference's float operations.
+__device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float2 nn) {
 #pragma clang fp contract(off)
     CellPhys c;
-    c.np = index_dev(k.crystal, k.lp, T);
-    c.ns = index_dev(k.crystal, k.ls, T);
+    c.np = nn.x;
+    c.ns = nn.y;
     c.kp = ((1.0f * k.pi) * k.dq) / (c.np * k.lp);
     c.ks = ((0.5f * k.pi) * k.dq) / (c.ns * k.ls);
     c.dk = (2.0f * k.pi) * (((2.0f * c.np) / k.lp - c.ns / k.ls) + k.inv_period);
@@ -170,12 +181,18 @@ __device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys
 // DKint holds the running float prefix sum of dk over the executed steps: the reference
 // recomputes the same float sum from z' = 0 twice per step (src/headers/PhaseMatching.h:
 // 25-38, Solver.h:198-199); the sequential float additions are identical.
-template <int N>
-__global__ __launch_bounds__(64) void k_therm_row_pass(float4 *__restrict__ U, float4 *__restrict__ Uout,
-                                                       const float2 *__restrict__ twg, const float *__restrict__ T,
-                                                       float *__restrict__ Q, float *__restrict__ DKint, ThermRowArgs a) {
-    using C = RowCfg<N>;
-    constexpr int E = C::E, TT = C::T;
+//
+// Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
+// are parked in the row's LDS slots and stepped by a rolled loop: the per-cell work (two
+// RK4 with three sincos each, the heat source) is too large to unroll E times, and a
+// rolled loop over register arrays would move them to scratch memory.
+template <int N, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ U, float4 *__restrict__ Uout,
+                                                           const float2 *__restrict__ twg,
+                                                           const float2 *__restrict__ NT, float *__restrict__ Q,
+                                                           float *__restrict__ DKint, ThermRowArgs a) {
+    using C = RowCfgT<N, E, W>;
+    constexpr int TT = C::T;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
     const int lane = threadIdx.x;
@@ -194,25 +211,43 @@ __global__ __launch_bounds__(64) void k_therm_row_pass(float4 *__restrict__ U, f
     }
     auto sync = [] { __syncthreads(); };
     const ThermConst &k = a.k;
-    if (a.flags & TROW_FWD) fft2f<N, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
-#pragma unroll 2
+    float2 *la = lds[0][rr], *lb = lds[1][rr];
+    if (a.flags & TROW_FWD) fft2fe<N, E, -1>(fa, fb, la, lb, t, 1, tw, sync);
+    __syncthreads();                          // the transform's last LDS reads are done
+#pragma unroll
     for (int j = 0; j < E; j++) {
+        la[lds_pad(t + j * TT)] = fa[j];
+        lb[lds_pad(t + j * TT)] = fb[j];
+    }
+    // each thread reads back only the slots it wrote: no barrier
+    const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
+#pragma unroll 1
+    for (int j = 0; j < E; j++) {
+        const int o = lds_pad(t + j * TT);
         const size_t idx = base + t + j * TT;
+        float2 P = la[o], S = lb[o];
         const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         if (a.flags & (TROW_NPREV | TROW_QPREV)) {
-            const CellPhys cp = cell_phys(k, T[(size_t)a.prev.slice * n2 + idx]);
-            if (a.flags & TROW_NPREV) rk4_thermal(fa[j], fb[j], cp, dkprev, a.prev, k);
-            if (a.flags & TROW_QPREV) Q[(size_t)a.prev.slice * n2 + idx] = heat_source(k, cp, fa[j], fb[j]);
+            const CellPhys cp = cell_phys(k, NT[sprev + idx]);
+            if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
+            if (a.flags & TROW_QPREV) Q[sprev + idx] = heat_source(k, cp, P, S);
         }
-        if (a.flags & TROW_SNAP) Uout[idx] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+        if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
         if (a.flags & TROW_NCUR) {
-            const CellPhys cc = cell_phys(k, T[(size_t)a.cur.slice * n2 + idx]);
+            const CellPhys cc = cell_phys(k, NT[scur + idx]);
             const float dk = dkprev + cc.dk;
             DKint[idx] = dk;
-            rk4_thermal(fa[j], fb[j], cc, dk, a.cur, k);
+            rk4_thermal(P, S, cc, dk, a.cur, k);
         }
+        la[o] = P;
+        lb[o] = S;
     }
-    if (a.flags & TROW_BWD) fft2f<N, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        fa[j] = la[lds_pad(t + j * TT)];
+        fb[j] = lb[lds_pad(t + j * TT)];
+    }
+    if (a.flags & TROW_BWD) fft2fe<N, E, +1>(fa, fb, la, lb, t, 1, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++) U[base + t + j * TT] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
@@ -238,23 +273,46 @@ hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipS
     return hipGetLastError();
 }
 
-template <int N>
-static hipError_t therm_row_launch(float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q, float *DKint,
-                                   const ThermRowArgs &a, hipStream_t st) {
-    using C = RowCfg<N>;
-    hipLaunchKernelGGL(k_therm_row_pass<N>, dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, Uout, tw, T, Q, DKint, a);
+hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, int crystal, float lp, float ls, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_index_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, tab, n, crystal, lp, ls);
     return hipGetLastError();
 }
 
-hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float *T, float *Q,
+template <int N, int E, int W>
+static hipError_t therm_row_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
                                    float *DKint, const ThermRowArgs &a, hipStream_t st) {
-    switch (n) {
-        case 64: return therm_row_launch<64>(U, Uout, tw, T, Q, DKint, a, st);
-        case 128: return therm_row_launch<128>(U, Uout, tw, T, Q, DKint, a, st);
-        case 256: return therm_row_launch<256>(U, Uout, tw, T, Q, DKint, a, st);
-        case 512: return therm_row_launch<512>(U, Uout, tw, T, Q, DKint, a, st);
+    using C = RowCfgT<N, E, W>;
+    hipLaunchKernelGGL((k_therm_row_pass<N, E, W>), dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, Uout, tw, NT, Q,
+                       DKint, a);
+    return hipGetLastError();
+}
+
+#define THRV(N, E, W) return therm_row_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
+
+hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant) {
+    if (variant < 0 || variant >= kThermRowVariantCount) return hipErrorInvalidValue;
+    switch (n * 16 + variant) {
+        case 64 * 16 + 0: THRV(64, 8, 1);
+        case 64 * 16 + 1: THRV(64, 4, 1);
+        case 64 * 16 + 2: THRV(64, 16, 1);
+        case 64 * 16 + 3: THRV(64, 8, 2);
+        case 128 * 16 + 0: THRV(128, 16, 1);
+        case 128 * 16 + 1: THRV(128, 8, 1);
+        case 128 * 16 + 2: THRV(128, 4, 1);
+        case 128 * 16 + 3: THRV(128, 4, 2);
+        case 256 * 16 + 0: THRV(256, 16, 1);
+        case 256 * 16 + 1: THRV(256, 8, 1);
+        case 256 * 16 + 2: THRV(256, 4, 1);
+        case 256 * 16 + 3: THRV(256, 4, 4);
+        case 512 * 16 + 0: THRV(512, 8, 1);
+        case 512 * 16 + 1: THRV(512, 4, 2);
+        case 512 * 16 + 2: THRV(512, 16, 1);
+        case 512 * 16 + 3: THRV(512, 8, 2);
         default: return hipErrorInvalidValue;
     }
 }
+#undef THRV
 
 }  // namespace shg

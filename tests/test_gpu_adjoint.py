@@ -8,10 +8,11 @@ central finite differences.
 - field and focus derivatives against central FD of the FP32 restatement (oracle32, the
   same arithmetic as the GPU forward, so the same operating point), step 1e-2 relative.
 - field and temperature derivatives against central FD of the FP64 restatement
-  (oracle64).  The FP32 reference arithmetic carries a float phase-mismatch offset
-  (0.008 rad over 30 mm at 27 C) that the FP64 model does not, so these agree at the
-  percent level; the temperature derivative cannot use oracle32 FD (the float dk is a
-  staircase in T).
+  (oracle64).  The FP32 reference arithmetic carries a float phase-mismatch offset that
+  the FP64 model does not (0.007 rad over 30 mm at 27 C, which moves the field gradient
+  by 2%), so this check runs at 26.913 C, where the float and double dk agree to 7e-7
+  rad over the crystal; the temperature derivative cannot use oracle32 FD (the float dk
+  is a staircase in T).
 - the checkpoint interval changes nothing; the torch.autograd wrapper returns the
   adjoint field.
 """
@@ -114,7 +115,8 @@ def test_field_and_focus_gradient_vs_oracle32_fd(gpu):
 
 @pytest.mark.gpu
 def test_field_and_temperature_gradient_vs_oracle64_fd(gpu):
-    p = _params(nz=30)
+    p = _params(nz=30, temp=26.913)
+    assert abs(orc.dk(p, 32, p.temp) - orc.dk(p, 64, p.temp)) * p.lz < 1e-5
     pump, noise, sh = _inputs(gpu, p)
     a0 = (pump + noise).astype(np.complex64)
     with gpu.Context(p) as ctx:

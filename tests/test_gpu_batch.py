@@ -13,24 +13,37 @@ def _mixed(count, seed, nz_range=(8, 30)):
 
 
 @pytest.mark.gpu
-def test_batch_equals_single_instance_solves(gpu):
+def test_batch_equals_single_instance_solves(gpu, monkeypatch):
+    # With the same pass layouts (tuning off) a batched instance is bit-identical to its
+    # single-instance solve; with tuning on, each side may pick other FFT layouts, which
+    # changes only the rounding.
     ps = _mixed(12, seed=11)
     print("\nbatch sizes", [p.nx for p in ps], "nz", [p.nz for p in ps], flush=True)
-    with gpu.Batch(ps) as b:
-        b.init_fields(noise_rel=1e-6, seed=5)
-        b.run()
-        outs = [b.get_fields(i) for i in range(len(ps))]
-        ins = [b.get_input(i) for i in range(len(ps))]
-        p_in, p_out, eta = b.powers()
-    for i, p in enumerate(ps):
-        with gpu.Context(p) as ctx:
-            ctx.set_fields(*ins[i])
-            ctx.forward()
-            fh, sh = ctx.get_fields()
-            pin1, pout1, eta1 = ctx.powers()
-        assert np.array_equal(fh, outs[i][0]) and np.array_equal(sh, outs[i][1]), i
-        # same fields; the FP64 power sums differ only in summation order
-        assert abs(eta[i] - eta1) <= 1e-10 * max(eta1, 1e-30), (i, eta[i], eta1)
+    for tuned in (False, True):
+        if tuned:
+            monkeypatch.delenv("SHG_NO_TUNE", raising=False)
+        else:
+            monkeypatch.setenv("SHG_NO_TUNE", "1")
+        with gpu.Batch(ps) as b:
+            b.init_fields(noise_rel=1e-6, seed=5)
+            b.run()
+            outs = [b.get_fields(i) for i in range(len(ps))]
+            ins = [b.get_input(i) for i in range(len(ps))]
+            p_in, p_out, eta = b.powers()
+        for i, p in enumerate(ps):
+            with gpu.Context(p) as ctx:
+                ctx.set_fields(*ins[i])
+                ctx.forward()
+                fh, sh = ctx.get_fields()
+                pin1, pout1, eta1 = ctx.powers()
+            if tuned:
+                e = max(orc.rel_l2(outs[i][0], fh), orc.rel_l2(outs[i][1], sh))
+                assert e <= 1e-5, (i, e)
+                assert abs(eta[i] - eta1) <= 1e-5 * max(eta1, 1e-30), (i, eta[i], eta1)
+            else:
+                assert np.array_equal(fh, outs[i][0]) and np.array_equal(sh, outs[i][1]), i
+                # same fields; the FP64 power sums differ only in summation order
+                assert abs(eta[i] - eta1) <= 1e-10 * max(eta1, 1e-30), (i, eta[i], eta1)
 
 
 @pytest.mark.gpu

@@ -120,3 +120,27 @@ def test_thermal_reference_convergence_rule(gpu):
     assert st["outer_iterations"] == rst[0]
     assert st["sweeps_last"] == 1
     assert np.abs(T - rT).max() < 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_thermal_every_layout_variant_matches_oracle(gpu, n):
+    # the tuner may pick any THERMAL row layout; each must meet the parity bar
+    p = _thermal_params(nx=n, ny=n, nz=6, heat_nz=0, power=40.0, waist=30.0, lx=2.0 * n, ly=2.0 * n)
+    print(f"\nthermal layouts n={n}", flush=True)
+    T = _rand_T(p, 7)
+    fh0 = gpu.make_pump(p, float(T[0, n // 2, n // 2]))
+    sh0 = np.zeros_like(fh0)
+    rfh, rsh, rQ = orc.thermal_cwes(p, T, fh0, sh0)
+    with gpu.Context(p) as ctx:
+        ctx.set_temperature(T)
+        for rv in range(4):
+            ctx.set_variants(rv, 0)
+            ctx.set_fields(fh0, sh0)
+            ctx.forward()
+            fh, sh = ctx.get_fields()
+            Q = ctx.get_heat_source()
+            e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh), float(np.linalg.norm(Q - rQ) / np.linalg.norm(rQ)))
+            print(f"  variant {rv}: relL2 FH {e[0]:.2e} SH {e[1]:.2e} Q {e[2]:.2e}")
+            assert max(e) <= TOL, (rv, e)
+        print(f"  tuned variants {ctx.tune(reps=3)}")
