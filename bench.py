@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=200)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
+    ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint"], default="all",
+                    help="run one extra workload alone and print its figures (for profiler runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -240,6 +242,18 @@ def main():
     import shg_focused_gaussian_beams_amd as shg
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+
+    if args.section != "all":
+        if args.section == "plain":
+            r = bench_plain(shg, args, shg.config2_params(thermal=0))
+        elif args.section == "batch":
+            r = bench_batch(shg, args)
+        else:
+            r = bench_adjoint(shg, args)
+            r["res"] = {k: v for k, v in r["res"].items()}
+        print(json.dumps({"section": args.section,
+                          **{k: v for k, v in r.items() if isinstance(v, (int, float, dict))}}), flush=True)
+        return
 
     th = bench_thermal(shg, args)
     p = th["p"]

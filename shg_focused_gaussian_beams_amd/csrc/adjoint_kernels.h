@@ -8,9 +8,10 @@
 namespace shg {
 
 // Tape of one checkpoint segment starting at step s0 (arrays of n*n float4 per step):
-// a[k] = state before the first RK4 of step s0+k, a2[k] = before its second RK4,
-// spec[k] = 2D spectrum the column pass of step s0+k multiplies by H.  `ckpt` receives
-// the segment-start state when the tape is off (checkpoint-only forward).
+// a = the segment-start state (before the first RK4 of s0, i.e. its checkpoint),
+// a2[k] = the state before the second RK4 of step s0+k, spec[k] = the 2D spectrum the
+// column pass of step s0+k multiplies by H.  A set `ckpt` receives the segment-start
+// state from the START row pass.
 struct TapeArgs {
     float4 *a, *a2, *spec, *ckpt;
     int s0;
@@ -27,8 +28,11 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st);
 hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st);
-// Partial records per step that the adjoint row pass (5 doubles each, one per wavefront)
-// and column pass (1 double per workgroup) write.
+// Per-step sums of the gradient partials: out[s*6 + 0..4] from g1 + g2, out[s*6 + 5] from gH.
+hipError_t launch_adj_reduce(const double *g1, const double *g2, int rparts, const double *gH, int cparts, int steps,
+                             double *out, hipStream_t st);
+// Partial records per step that the adjoint row pass (5 doubles each, one per row) and
+// column pass (1 double per workgroup) write.
 int adj_row_parts(int n);
 int adj_col_parts(int n);
 

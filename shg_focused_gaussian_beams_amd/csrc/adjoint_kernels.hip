@@ -1,8 +1,10 @@
 // adjoint_kernels.hip — discrete adjoint of the plain z-march on gfx950.
 //
 // Forward (one checkpoint segment, steps [s0, s1)) with a tape:
-//   row pass writes the real-space state before each step's first RK4 (A_s) and before
-//   its second RK4 (A''_s); column pass writes the 2D spectrum it multiplies by H.
+//   row pass writes the real-space state before each step's second RK4 (A''_s); column
+//   pass writes the 2D spectrum it multiplies by H.  The state before a step's first RK4,
+//   A_s = RK4_{s-1}(A''_{s-1}), is recomputed by the adjoint (the segment's first one is
+//   its checkpoint).
 // Adjoint (reverse step order), with lambda the gradient dJ = Re sum conj(lambda) dA:
 //   row pass  : R_fwd -> RK4_s^T at A_s -> RK4_{s-1}^T at A''_{s-1} -> R_bwd
 //   col pass  : C_bwd -> x conj(H)/N^2 -> C_fwd   (L^H = (1/N^2) F_fwd conj(H) F_bwd)
@@ -83,8 +85,11 @@ __device__ __forceinline__ void rk4_adj(float2 P, float2 S, float2 &lP, float2 &
     lS = add2(ybs, rs);
 }
 
-// Wave-64 sum of the five accumulators; lane 0 writes them to `out`.
-__device__ __forceinline__ void wave_store(AdjAcc a, double *out) {
+// Sum of the five accumulators over the workgroup (wave shuffles, then LDS across the
+// waves); thread 0 writes them to `out`.  All threads of the group must call it.
+template <int WAVES>
+__device__ __forceinline__ void block_store(AdjAcc a, double *out) {
+    __shared__ double red[WAVES][5];
     for (int off = 32; off > 0; off >>= 1) {
         a.c1r += __shfl_down(a.c1r, off, 64);
         a.c1i += __shfl_down(a.c1i, off, 64);
@@ -92,8 +97,15 @@ __device__ __forceinline__ void wave_store(AdjAcc a, double *out) {
         a.c2i += __shfl_down(a.c2i, off, 64);
         a.tpa += __shfl_down(a.tpa, off, 64);
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        out[0] = a.c1r; out[1] = a.c1i; out[2] = a.c2r; out[3] = a.c2i; out[4] = a.tpa;
+        red[w][0] = a.c1r; red[w][1] = a.c1i; red[w][2] = a.c2r; red[w][3] = a.c2i; red[w][4] = a.tpa;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double s = 0.0;
+        for (int k = 0; k < WAVES; k++) s += red[k][threadIdx.x];
+        out[threadIdx.x] = s;
     }
 }
 
@@ -107,11 +119,20 @@ template <> struct AdjLayout<128> { using Row = RowCfgT<128, 4, 1>; using Col = 
 template <> struct AdjLayout<256> { using Row = RowCfgT<256, 4, 1>; using Col = ColCfgT<256, 4, 8>; };
 template <> struct AdjLayout<512> { using Row = RowCfgT<512, 4, 2>; using Col = ColCfgT<512, 4, 4>; };
 
+// The adjoint row pass is wide: one row per workgroup of W waves; the first N/E lanes
+// (at most one wave) run the transforms and every thread steps N/(64 W) cells of the row
+// between them, so the recomputed RK4 stages and their transposes spread over W times as
+// many wavefronts as the transforms need.
+template <int N> struct AdjWide;
+template <> struct AdjWide<64>  { static constexpr int E = 4, W = 1; };
+template <> struct AdjWide<128> { static constexpr int E = 4, W = 2; };
+template <> struct AdjWide<256> { static constexpr int E = 4, W = 4; };
+template <> struct AdjWide<512> { static constexpr int E = 8, W = 8; };
+
 // Forward row pass with tape (single instance, modes as RowMode without AUTO).
-// START at s0: tape_a[0] = input, then RK4_s0, R_bwd.
-// MID at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s, tape_a[s+1-s0] = A_{s+1}, RK4_{s+1}, R_bwd.
+// START at s0: ckpt = input (if ckpt is set), then RK4_s0, R_bwd.
+// MID at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s, RK4_{s+1}, R_bwd.
 // END at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s  (real-space state of the segment end).
-// A null tape writes nothing (checkpoint-only forward: START copies its input to ckpt).
 template <int N>
 __global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_row_tape(const InstDesc *__restrict__ descs,
                                                                           const float2 *__restrict__ twg, int step,
@@ -137,10 +158,9 @@ __global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_row_tape(const I
     auto sync = [] { __syncthreads(); };
     const int ls = step - tp.s0;
     if (mode == ROW_START) {
-        float4 *dst = tp.a ? tp.a : tp.ckpt;
-        if (dst)
+        if (tp.ckpt)
 #pragma unroll
-            for (int j = 0; j < E; j++) dst[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+            for (int j = 0; j < E; j++) tp.ckpt[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
     } else {
         fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
         const float4 c = d.coef[step];
@@ -148,8 +168,6 @@ __global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_row_tape(const I
         for (int j = 0; j < E; j++) {
             if (tp.a2) tp.a2[(size_t)ls * n2 + base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
             rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
-            if (mode == ROW_MID && tp.a)
-                tp.a[(size_t)(ls + 1) * n2 + base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
         }
     }
     if (mode != ROW_END) {
@@ -202,91 +220,91 @@ __global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_col_tape(const I
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's).
+// Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's),
+// wide layout (AdjWide).
 // ADJ_START at s: RK4_s^T at A''_s, R_bwd.
-// ADJ_MID   at s: R_fwd, RK4_s^T at A_s, RK4_{s-1}^T at A''_{s-1}, R_bwd.
-// ADJ_END   at s: R_fwd, RK4_s^T at A_s (real-space lambda of the segment start).
-// g1[(s*parts + wave)*5] gets the sensitivities of the first RK4 of step s (at A_s),
-// g2[...] those of the second (at A''_s); parts = wavefronts per pass.  Between the
-// transforms the thread's cells are parked in the row's LDS slots and visited by a rolled
-// loop (RK4^T is too large to unroll per element).
+// ADJ_MID   at s: R_fwd, RK4_s^T at A_s = RK4_{s-1}(A''_{s-1}), RK4_{s-1}^T at A''_{s-1}, R_bwd.
+// ADJ_END   at s: R_fwd, RK4_s^T at A_s = tp.a (the segment's checkpoint); leaves the
+//                 real-space lambda of the segment start.
+// g1[(s*N + row)*5] gets the row's sensitivities of the first RK4 of step s (at A_s),
+// g2[...] those of the second (at A''_s).
 template <int N>
-__global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_adj_row(const InstDesc *__restrict__ descs,
-                                                                         const float2 *__restrict__ twg, int step,
-                                                                         int mode, TapeArgs tp,
-                                                                         double *__restrict__ g1,
-                                                                         double *__restrict__ g2) {
-    using C = typename AdjLayout<N>::Row;
-    constexpr int E = C::E, T = C::T, W = C::THREADS / 64;
-    __shared__ float2 lds[2][C::ROWS][C::L];
+__global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *__restrict__ descs,
+                                                               const float2 *__restrict__ twg, int step, int mode,
+                                                               TapeArgs tp, double *__restrict__ g1,
+                                                               double *__restrict__ g2) {
+    constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    static_assert(T <= 64 && N % THREADS == 0, "the transforms run in one wavefront");
+    __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
-    const int lane = threadIdx.x;
-    const int t = lane % T, rr = lane / T;
-    const int row = blockIdx.x * C::ROWS + rr;
-    const size_t part = (size_t)blockIdx.x * W + (lane >> 6), parts = (size_t)gridDim.x * W;
-    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    const int tid = threadIdx.x;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
-    const size_t n2 = (size_t)N * N, base = (size_t)row * N;
-    float2 fa[E], fb[E];
+    auto sync = [] { wave_sync(); };
+    if (tid < T) {
+        float2 fa[E], fb[E];
 #pragma unroll
-    for (int j = 0; j < E; j++) {
-        const float4 v = d.U[base + t + j * T];
-        fa[j] = make_float2(v.x, v.y);
-        fb[j] = make_float2(v.z, v.w);
-    }
-    auto sync = [] { __syncthreads(); };
-    float2 *la = lds[0][rr], *lb = lds[1][rr];
-    if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, la, lb, t, 1, tw, sync);
-    __syncthreads();                          // the transform's last LDS reads are done
+        for (int j = 0; j < E; j++) {
+            const float4 v = d.U[base + tid + j * T];
+            fa[j] = make_float2(v.x, v.y);
+            fb[j] = make_float2(v.z, v.w);
+        }
+        if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
+        wave_sync();
 #pragma unroll
-    for (int j = 0; j < E; j++) {
-        la[lds_pad(t + j * T)] = fa[j];
-        lb[lds_pad(t + j * T)] = fb[j];
+        for (int j = 0; j < E; j++) {
+            lds[0][lds_pad(tid + j * T)] = fa[j];
+            lds[1][lds_pad(tid + j * T)] = fb[j];
+        }
     }
-    // each thread reads back only the slots it wrote: no barrier
+    __syncthreads();
     const int ls = step - tp.s0;
-    if (mode != ADJ_START) {
-        const float4 c = d.coef[step];
-        const float4 *tape = tp.a + (size_t)ls * n2 + base + t;
-        AdjAcc acc;
-        acc_zero(acc);
+    AdjAcc acc1, acc2;
+    acc_zero(acc1);
+    acc_zero(acc2);
+    const float4 c_s = d.coef[step];
+    const float4 c_p = d.coef[step > 0 ? step - 1 : 0];
 #pragma unroll 1
-        for (int j = 0; j < E; j++) {
-            const int o = lds_pad(t + j * T);
-            float2 P = la[o], S = lb[o];
-            const float4 a = tape[j * T];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c, d.nl, acc);
-            la[o] = P;
-            lb[o] = S;
+    for (int cc = tid; cc < N; cc += THREADS) {
+        const int o = lds_pad(cc);
+        const size_t idx = base + cc;
+        float2 P = lds[0][o], S = lds[1][o];
+        if (mode == ADJ_START) {
+            const float4 a = tp.a2[(size_t)ls * n2 + idx];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_s, d.nl, acc2);
+        } else if (mode == ADJ_END) {
+            const float4 a = tp.a[idx];
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_s, d.nl, acc1);
+        } else {
+            const float4 a = tp.a2[(size_t)(ls - 1) * n2 + idx];
+            float2 Ap = f2(a.x, a.y), As = f2(a.z, a.w);
+            rk4_cell(Ap, As, c_p, c_p, c_p, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);   // A_s
+            rk4_adj(Ap, As, P, S, c_s, d.nl, acc1);
+            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_p, d.nl, acc2);
         }
-        wave_store(acc, g1 + ((size_t)step * parts + part) * 5);
+        lds[0][o] = P;
+        lds[1][o] = S;
     }
-    if (mode != ADJ_END) {
-        const int sp = mode == ADJ_START ? step : step - 1;   // step of the second RK4
-        const float4 c = d.coef[sp];
-        const float4 *tape = tp.a2 + (size_t)(sp - tp.s0) * n2 + base + t;
-        AdjAcc acc;
-        acc_zero(acc);
-#pragma unroll 1
+    if (mode != ADJ_START) block_store<W>(acc1, g1 + ((size_t)step * N + blockIdx.x) * 5);
+    if (mode == ADJ_START) block_store<W>(acc2, g2 + ((size_t)step * N + blockIdx.x) * 5);
+    else if (mode == ADJ_MID) {
+        __syncthreads();                      // block_store's LDS slots are reused
+        block_store<W>(acc2, g2 + ((size_t)(step - 1) * N + blockIdx.x) * 5);
+    }
+    __syncthreads();
+    if (tid < T) {
+        float2 fa[E], fb[E];
+#pragma unroll
         for (int j = 0; j < E; j++) {
-            const int o = lds_pad(t + j * T);
-            float2 P = la[o], S = lb[o];
-            const float4 a = tape[j * T];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c, d.nl, acc);
-            la[o] = P;
-            lb[o] = S;
+            fa[j] = lds[0][lds_pad(tid + j * T)];
+            fb[j] = lds[1][lds_pad(tid + j * T)];
         }
-        wave_store(acc, g2 + ((size_t)sp * parts + part) * 5);
-    }
+        if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
 #pragma unroll
-    for (int j = 0; j < E; j++) {
-        fa[j] = la[lds_pad(t + j * T)];
-        fb[j] = lb[lds_pad(t + j * T)];
+        for (int j = 0; j < E; j++) d.U[base + tid + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
     }
-    if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, la, lb, t, 1, tw, sync);
-#pragma unroll
-    for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
 
 // Adjoint column pass: C_bwd, x conj(Gx Gy), C_fwd.  gH[step*nblk + blk] receives
@@ -359,6 +377,28 @@ __global__ __launch_bounds__(256) void k_adj_seed(const float4 *__restrict__ U, 
     L[i] = make_float4(0.0f, 0.0f, scale * v.z, scale * v.w);
 }
 
+// Per step s (one workgroup each): out[s*6 + q] = sum over the row-pass partials of
+// g1 + g2 (q < 5: c1.re, c1.im, c2.re, c2.im, tpa) and out[s*6 + 5] = sum of the
+// column-pass partials gH, in a fixed order.
+__global__ __launch_bounds__(64) void k_adj_reduce(const double *__restrict__ g1, const double *__restrict__ g2,
+                                                   int rparts, const double *__restrict__ gH, int cparts,
+                                                   double *__restrict__ out) {
+    const int s = blockIdx.x, lane = threadIdx.x;
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = lane; b < rparts; b += 64) {
+        const size_t o = ((size_t)s * rparts + b) * 5;
+#pragma unroll
+        for (int q = 0; q < 5; q++) v[q] += g1[o + q] + g2[o + q];
+    }
+    for (int b = lane; b < cparts; b += 64) v[5] += gH[(size_t)s * cparts + b];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        double x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) out[(size_t)s * 6 + q] = x;
+    }
+}
+
 // ---------------------------------------------------------------------------
 template <int N> static int row_blocks() { return N / AdjLayout<N>::Row::ROWS; }
 template <int N> static int col_blocks() { return N / AdjLayout<N>::Col::CW; }
@@ -372,17 +412,7 @@ template <int N> static int col_blocks() { return N / AdjLayout<N>::Col::CW; }
         default: return hipErrorInvalidValue;            \
     }
 
-template <int N> static int row_parts() { return row_blocks<N>() * (AdjLayout<N>::Row::THREADS / 64); }
-
-int adj_row_parts(int n) {
-    switch (n) {
-        case 64: return row_parts<64>();
-        case 128: return row_parts<128>();
-        case 256: return row_parts<256>();
-        case 512: return row_parts<512>();
-        default: return 0;
-    }
-}
+int adj_row_parts(int n) { return n == 64 || n == 128 || n == 256 || n == 512 ? n : 0; }
 int adj_col_parts(int n) {
     switch (n) {
         case 64: return col_blocks<64>();
@@ -410,8 +440,8 @@ hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step,
 hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
                           double *g1, double *g2, hipStream_t st) {
     if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(row_blocks<NN>()), dim3(AdjLayout<NN>::Row::THREADS), 0,
-                                       st, d, tw, step, mode, tp, g1, g2));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(NN), dim3(64 * AdjWide<NN>::W), 0, st, d, tw, step, mode,
+                                       tp, g1, g2));
     return hipGetLastError();
 }
 
@@ -419,6 +449,13 @@ hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, 
                           float a_p, float a_s, double *gH, hipStream_t st) {
     SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(AdjLayout<NN>::Col::THREADS), 0,
                                        st, d, tw, step, tp, qxy, a_p, a_s, gH));
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_reduce(const double *g1, const double *g2, int rparts, const double *gH, int cparts, int steps,
+                             double *out, hipStream_t st) {
+    if (steps <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_reduce, dim3(steps), dim3(64), 0, st, g1, g2, rparts, gH, cparts, out);
     return hipGetLastError();
 }
 

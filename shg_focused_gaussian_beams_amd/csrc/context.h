@@ -14,11 +14,16 @@
 struct AdjointBuffers {
     int k = 0, nseg = 0;                       // checkpoint interval and segment count they were sized for
     float4 *lam = nullptr;                     // adjoint state
-    float4 *ckpt = nullptr;                    // nseg checkpoints
-    float4 *a = nullptr, *a2 = nullptr, *spec = nullptr;   // segment tape
+    float4 *ckpt = nullptr;                    // nseg checkpoints (segment-start states)
+    float4 *a2 = nullptr, *spec = nullptr;     // two segment tapes of K steps each (double buffer)
     double *g1 = nullptr, *g2 = nullptr, *gH = nullptr;    // FP64 parameter-sensitivity partials
+    double *gs = nullptr;                      // their per-step sums (6 per step)
     float2 *qxy = nullptr;                     // 2 pi^2 fx^2, 2 pi^2 fy^2 per index
     shg::InstDesc *dlam = nullptr;             // instance table whose U is `lam`
+    hipStream_t ts = nullptr;                  // stream of the tape recomputation
+    hipEvent_t ev_u = nullptr;                 // the forward state buffer is free for `ts`
+    hipEvent_t ev_tape[2] = {nullptr, nullptr};  // tape slot k written
+    hipEvent_t ev_free[2] = {nullptr, nullptr};  // tape slot k read by the adjoint
 };
 
 struct shg_ctx {

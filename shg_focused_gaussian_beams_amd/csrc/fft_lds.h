@@ -196,6 +196,17 @@ __device__ __forceinline__ void fft2fe(float2 (&a)[E], float2 (&b)[E], P la, P l
     fft_stages<N, E, 0, SIGN>(a, b, la, lb, t, st, tw, sync);
 }
 
+// Ordering point for LDS exchanges inside one wavefront (a transform run by at most 64
+// lanes of one wave): a wave's LDS operations complete in issue order, so only the
+// compiler's reordering has to be fenced.
+#ifndef SHG_HOST_EMULATION
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+#endif
+
 // The default plan of each length (FftPlan<N>::E elements per thread).
 template <int N, int SIGN, typename P, typename Sync>
 __device__ __forceinline__ void fft2f(float2 (&a)[FftPlan<N>::E], float2 (&b)[FftPlan<N>::E], P la, P lb, int t,

@@ -1,10 +1,12 @@
 // shg_adjoint.cpp — gradients of the SH efficiency of a plain context (C ABI in
 // include/shg.h).  New: the reference has no derivatives.
 //
-// J = eta = P_SH(out) / P_FH(z = 0).  The z-march is differentiated discretely: a
-// checkpointed forward (state copied every K steps), then per segment, last first, a
-// forward recomputation that tapes the RK4 inputs and the column-pass spectra, and the
-// adjoint march through the segment (adjoint_kernels.hip).  Parameter sensitivities:
+// J = eta = P_SH(out) / P_FH(z = 0).  The z-march is differentiated discretely
+// (adjoint_kernels.hip): a forward that keeps the state every K steps (checkpoints) and
+// tapes the last segment, then per segment, last first, the adjoint march through it.
+// The taped recomputation of segment j-1 runs on a second stream while the adjoint of
+// segment j runs (two tape buffers): both are latency-bound passes over one field, so
+// overlapping them fills the GPU that either leaves mostly idle.  Parameter sensitivities:
 //   - the z = 0 pump field: lambda_0 plus the direct P_in term;
 //   - the focal position: through the reference pump setPump (src/headers/Efields.h:74-104)
 //     differentiated analytically;
@@ -49,9 +51,13 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
             hipMalloc(&b.g1, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.g2, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.gH, (size_t)M * shg::adj_col_parts(c->n) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.gs, (size_t)M * 6 * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.qxy, (size_t)c->n * sizeof(float2)) != hipSuccess ||
             hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess)
             return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+        AHIP_TRY(hipStreamCreateWithFlags(&b.ts, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&b.ev_u, &b.ev_tape[0], &b.ev_tape[1], &b.ev_free[0], &b.ev_free[1]})
+            AHIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
         // 2 pi^2 f^2 on the fftshifted index grid (DC at 0), x and y spacings
         const shg::phys::Consts k(c->p);
         std::vector<float> q(2 * (size_t)c->n);
@@ -69,17 +75,28 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
         AHIP_TRY(hipMemcpy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice));
     }
     if (b.k != K || b.nseg != nseg) {
-        for (float4 **ptr : {&b.ckpt, &b.a, &b.a2, &b.spec})
+        for (float4 **ptr : {&b.ckpt, &b.a2, &b.spec})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
+        const int slots = nseg > 1 ? 2 : 1;
         if (hipMalloc(&b.ckpt, (size_t)nseg * n2 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&b.a, (size_t)K * n2 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&b.a2, (size_t)K * n2 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&b.spec, (size_t)K * n2 * sizeof(float4)) != hipSuccess) {
+            hipMalloc(&b.a2, (size_t)slots * K * n2 * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b.spec, (size_t)slots * K * n2 * sizeof(float4)) != hipSuccess) {
             b.k = 0;
             return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: tape allocation failed");
         }
         b.k = K;
         b.nseg = nseg;
+    }
+    return SHG_OK;
+}
+
+// Taped forward of the segment [s0, s1) from the state in the context's field buffer.
+int enqueue_tape(shg_ctx *c, const shg::TapeArgs &tp, int s1, hipStream_t st) {
+    const int n = c->n;
+    AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, tp.s0, shg::ROW_START, tp, st));
+    for (int s = tp.s0; s < s1; s++) {
+        AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
+        AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < s1 - 1 ? shg::ROW_MID : shg::ROW_END, tp, st));
     }
     return SHG_OK;
 }
@@ -108,14 +125,21 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
     hipStream_t st = c->stream;
     const int rb = shg::adj_row_parts(n), cb = shg::adj_col_parts(n);
 
-    // 1. checkpointed forward
+    // tape of segment j: slot j % 2, its start state is checkpoint j
+    auto tape_of = [&](int j, bool write_ckpt) {
+        const size_t off = (size_t)(nseg > 1 ? (j & 1) : 0) * K * n2;
+        return shg::TapeArgs{b.ckpt + (size_t)j * n2, b.a2 + off, b.spec + off,
+                             write_ckpt ? b.ckpt + (size_t)j * n2 : nullptr, j * K, 0};
+    };
+
+    // 1. forward: checkpoints of segments 0 .. nseg-2, the last segment taped
     AHIP_TRY(hipEventRecord(c->e0, st));
     AHIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
-    for (int j = 0; j < nseg; j++) {
-        const int s0 = j * K, s1 = std::min(M, s0 + K);
+    for (int j = 0; j < nseg - 1; j++) {
         AHIP_TRY(hipMemcpyAsync(b.ckpt + (size_t)j * n2, c->U, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        ARC_TRY(shg_internal_enqueue_march_plain(c, s0, s1 - s0));
+        ARC_TRY(shg_internal_enqueue_march_plain(c, j * K, K));
     }
+    ARC_TRY(enqueue_tape(c, tape_of(nseg - 1, true), M, st));
     AHIP_TRY(hipEventRecord(c->e1, st));
     AHIP_TRY(hipEventSynchronize(c->e1));
     float fwd_ms = 0.f;
@@ -134,27 +158,34 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
 
     AHIP_TRY(hipEventRecord(c->e0, st));
     AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
+    AHIP_TRY(hipEventRecord(b.ev_u, st));
+    AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
 
-    // 2. per segment, last first: taped recomputation, then the adjoint
+    // 2. per segment, last first: the adjoint on `st`, the next tape on `ts`
     const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
     const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
     const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
     const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
     for (int j = nseg - 1; j >= 0; j--) {
-        const int s0 = j * K, s1 = std::min(M, s0 + K);
-        shg::TapeArgs tp{b.a, b.a2, b.spec, nullptr, s0, 0};
-        AHIP_TRY(hipMemcpyAsync(c->U, b.ckpt + (size_t)j * n2, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s0, shg::ROW_START, tp, st));
-        for (int s = s0; s < s1; s++) {
-            AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
-            AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < s1 - 1 ? shg::ROW_MID : shg::ROW_END, tp, st));
+        if (j >= 1) {
+            // slot (j-1) % 2 was last read by the adjoint of segment j+1
+            if (j + 1 <= nseg - 1) AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_free[(j - 1) & 1], 0));
+            AHIP_TRY(hipMemcpyAsync(c->U, b.ckpt + (size_t)(j - 1) * n2, n2 * sizeof(float4), hipMemcpyDeviceToDevice,
+                                    b.ts));
+            ARC_TRY(enqueue_tape(c, tape_of(j - 1, false), j * K, b.ts));
+            AHIP_TRY(hipEventRecord(b.ev_tape[(j - 1) & 1], b.ts));
         }
+        if (j < nseg - 1) AHIP_TRY(hipStreamWaitEvent(st, b.ev_tape[j & 1], 0));
+        const int s0 = j * K, s1 = std::min(M, s0 + K);
+        const shg::TapeArgs tp = tape_of(j, false);
         AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st));
         for (int s = s1 - 1; s >= s0; s--) {
             AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
             AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1, b.g2, st));
         }
+        AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
     }
+    AHIP_TRY(shg::launch_adj_reduce(b.g1, b.g2, rb, b.gH, cb, M, b.gs, st));
     AHIP_TRY(hipEventRecord(c->e1, st));
     AHIP_TRY(hipEventSynchronize(c->e1));
     float adj_ms = 0.f;
@@ -162,12 +193,10 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
 
     // 3. gradients on the host (FP64)
     std::vector<float> lam(4 * n2), u0(4 * n2);
-    std::vector<double> g1((size_t)M * rb * 5), g2((size_t)M * rb * 5), gH((size_t)M * cb);
+    std::vector<double> gs((size_t)M * 6);
     AHIP_TRY(hipMemcpy(lam.data(), b.lam, n2 * sizeof(float4), hipMemcpyDeviceToHost));
     AHIP_TRY(hipMemcpy(u0.data(), c->U0, n2 * sizeof(float4), hipMemcpyDeviceToHost));
-    AHIP_TRY(hipMemcpy(g1.data(), b.g1, g1.size() * sizeof(double), hipMemcpyDeviceToHost));
-    AHIP_TRY(hipMemcpy(g2.data(), b.g2, g2.size() * sizeof(double), hipMemcpyDeviceToHost));
-    AHIP_TRY(hipMemcpy(gH.data(), b.gH, gH.size() * sizeof(double), hipMemcpyDeviceToHost));
+    AHIP_TRY(hipMemcpy(gs.data(), b.gs, gs.size() * sizeof(double), hipMemcpyDeviceToHost));
 
     // dJ/dA0 = lambda_0 (FH part) + d(1/P_in)/dA0 term
     std::vector<cd> grad(n2);
@@ -188,20 +217,15 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
     double dT_coef = 0.0, dT_H = 0.0;
     float z = 0.0f;
     for (int s = 0; s < M; s++) {
-        cd c1b(0, 0), c2b(0, 0);
-        double tb = 0.0;
-        for (int blk = 0; blk < rb; blk++) {
-            const double *x1 = &g1[((size_t)s * rb + blk) * 5], *x2 = &g2[((size_t)s * rb + blk) * 5];
-            c1b += cd(x1[0] + x2[0], x1[1] + x2[1]);
-            c2b += cd(x1[2] + x2[2], x1[3] + x2[3]);
-            tb += x1[4] + x2[4];
-        }
+        const double *x = &gs[(size_t)s * 6];
+        const cd c1b(x[0], x[1]), c2b(x[2], x[3]);
+        const double tb = x[4];
         const double zs = (double)z, th = (double)(float)(z * (float)dkf);
         const cd I(0, 1);
         const cd dc1 = I * std::exp(-I * th) * (dkap - I * kap * zs * ddk);
         const cd dc2 = I * std::exp(I * th) * (dkas + I * kas * zs * ddk);
         dT_coef += (std::conj(c1b) * dc1).real() + (std::conj(c2b) * dc2).real() + tb * dtpa;
-        for (int blk = 0; blk < cb; blk++) dT_H += gH[(size_t)s * cb + blk];
+        dT_H += x[5];
         z += k.dz;
     }
     const double dT_powers = J * (dns / ns - dnp / np);

@@ -115,32 +115,29 @@ int after_fields_set(shg_batch *b) {
 
 // Per size class, times `reps` launches of every layout variant over all the class's
 // instances and keeps the fastest (clobbers U; shg_batch_run restores it from U0).
+// Per size class, times every (row layout, column layout) pair as the schedule runs them
+// (a row pass then a column pass over the class) and keeps the fastest pair.
 int tune_batch(shg_batch *b, int reps) {
     for (int c = 0; c < 4; c++) {
         if (!b->cls_cnt[c]) continue;
         const int n = kSizes[c], cnt = b->cls_cnt[c];
         const shg::InstDesc *d = b->desc + b->cls_off[c];
+        auto step = [&](int rv, int cv) {
+            const hipError_t e = shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, rv);
+            return e != hipSuccess ? e : shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, cv);
+        };
         float best = 1e30f;
-        for (int v = 0; v < shg::kRowVariantCount; v++) {
-            BHIP_TRY(shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, v));
-            BHIP_TRY(hipEventRecord(b->e0, b->stream));
-            for (int r = 0; r < reps; r++) BHIP_TRY(shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, v));
-            BHIP_TRY(hipEventRecord(b->e1, b->stream));
-            BHIP_TRY(hipEventSynchronize(b->e1));
-            float ms = 0.f;
-            BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
-            if (ms < best) { best = ms; b->row_v[c] = v; }
-        }
-        best = 1e30f;
-        for (int v = 0; v < shg::kColVariantCount; v++) {
-            BHIP_TRY(shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, v));
-            BHIP_TRY(hipEventRecord(b->e0, b->stream));
-            for (int r = 0; r < reps; r++) BHIP_TRY(shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, v));
-            BHIP_TRY(hipEventRecord(b->e1, b->stream));
-            BHIP_TRY(hipEventSynchronize(b->e1));
-            float ms = 0.f;
-            BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
-            if (ms < best) { best = ms; b->col_v[c] = v; }
+        for (int rv = 0; rv < shg::kRowVariantCount; rv++) {
+            for (int cv = 0; cv < shg::kColVariantCount; cv++) {
+                BHIP_TRY(step(rv, cv));
+                BHIP_TRY(hipEventRecord(b->e0, b->stream));
+                for (int r = 0; r < reps; r++) BHIP_TRY(step(rv, cv));
+                BHIP_TRY(hipEventRecord(b->e1, b->stream));
+                BHIP_TRY(hipEventSynchronize(b->e1));
+                float ms = 0.f;
+                BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+                if (ms < best) { best = ms; b->row_v[c] = rv; b->col_v[c] = cv; }
+            }
         }
     }
     return SHG_OK;

@@ -98,39 +98,35 @@ static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
     return a;
 }
 
-// Times every row and column layout on this context's size and keeps the fastest.  The
-// passes run on the context's buffers: the caller restores the fields afterwards (a
-// THERMAL march rewrites DKint and every heat-source slice it touches from scratch).
+// Times every (row layout, column layout) pair on this context's size as the march runs
+// them, a row pass then a column pass, and keeps the fastest pair: timed alone, a pass
+// finds its input in the L2 of the XCD that wrote it, but in the march every row pass
+// reads what the column pass scattered over all XCDs and vice versa.  The passes run on
+// the context's buffers: the caller restores the fields afterwards (a THERMAL march
+// rewrites DKint and every heat-source slice it touches from scratch).
 int tune_variants(shg_ctx *c, int reps) {
     const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
-    float best_r = 1e30f, best_c = 1e30f;
-    int br = 0, bc = 0;
+    const int nrow = c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
+    shg::ThermRowArgs a{};
     if (c->p.thermal) {
         HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
                                         c->tc.lp, c->tc.ls, c->stream));
-        const shg::ThermRowArgs a = therm_mid_args(c);
-        for (int v = 0; v < shg::kThermRowVariantCount; v++) {
-            float ms = 0.f;
-            RC_TRY(time_launches(c, reps, [&] {
-                return shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a,
-                                                    c->stream, v);
-            }, &ms));
-            if (ms < best_r) { best_r = ms; br = v; }
-        }
-    } else {
-        for (int v = 0; v < shg::kRowVariantCount; v++) {
-            float ms = 0.f;
-            RC_TRY(time_launches(c, reps, [&] {
-                return shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, v);
-            }, &ms));
-            if (ms < best_r) { best_r = ms; br = v; }
-        }
+        a = therm_mid_args(c);
     }
-    for (int v = 0; v < shg::kColVariantCount; v++) {
-        float ms = 0.f;
-        RC_TRY(time_launches(c, reps, [&] { return shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, v); },
-                             &ms));
-        if (ms < best_c) { best_c = ms; bc = v; }
+    float best = 1e30f;
+    int br = 0, bc = 0;
+    for (int rv = 0; rv < nrow; rv++) {
+        for (int cv = 0; cv < shg::kColVariantCount; cv++) {
+            float ms = 0.f;
+            RC_TRY(time_launches(c, reps, [&] {
+                const hipError_t e = c->p.thermal
+                    ? shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a,
+                                                   c->stream, rv)
+                    : shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv);
+                return e != hipSuccess ? e : shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv);
+            }, &ms));
+            if (ms < best) { best = ms; br = rv; bc = cv; }
+        }
     }
     if (c->gexec && (br != c->row_v || bc != c->col_v)) {
         (void)hipGraphExecDestroy(c->gexec);
@@ -203,10 +199,13 @@ void release(shg_ctx *c) {
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
                       (void *)c->d_coef, (void *)c->d_desc,
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
-                      (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt, (void *)c->adj.a,
+                      (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
-                      (void *)c->adj.qxy, (void *)c->adj.dlam})
+                      (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam})
         if (ptr) (void)hipFree(ptr);
+    for (hipEvent_t e : {c->adj.ev_u, c->adj.ev_tape[0], c->adj.ev_tape[1], c->adj.ev_free[0], c->adj.ev_free[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (c->adj.ts) (void)hipStreamDestroy(c->adj.ts);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -54,8 +54,9 @@ hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipS
 // {n_p(T), n_s(T)} of every heat voxel (T is frozen during one march, so the row pass
 // reads the two indices instead of evaluating the FP64 Sellmeier fits per cell and step).
 hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, int crystal, float lp, float ls, hipStream_t st);
-// Row-pass layouts <E, W> per size (the tuner times all of them): variant v of size n.
-constexpr int kThermRowVariantCount = 4;
+// Row-pass layouts per size (the tuner times all of them): variants 0-3 are <E, W> row
+// layouts, 4-5 wide layouts (one row per workgroup, one cell per thread or two).
+constexpr int kThermRowVariantCount = 6;
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
                                    float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0);
 // Grid size the Jacobi launcher uses (number of residual partials it writes).

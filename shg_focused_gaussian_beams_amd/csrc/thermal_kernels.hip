@@ -182,6 +182,26 @@ __device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys
 // recomputes the same float sum from z' = 0 twice per step (src/headers/PhaseMatching.h:
 // 25-38, Solver.h:198-199); the sequential float additions are identical.
 //
+// One cell of the pass between the transforms (idx: cell index in the plane).
+__device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, size_t sprev, size_t scur,
+                                           float4 *__restrict__ Uout, const float2 *__restrict__ NT,
+                                           float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
+    const ThermConst &k = a.k;
+    const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
+    if (a.flags & (TROW_NPREV | TROW_QPREV)) {
+        const CellPhys cp = cell_phys(k, NT[sprev + idx]);
+        if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
+        if (a.flags & TROW_QPREV) Q[sprev + idx] = heat_source(k, cp, P, S);
+    }
+    if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
+    if (a.flags & TROW_NCUR) {
+        const CellPhys cc = cell_phys(k, NT[scur + idx]);
+        const float dk = dkprev + cc.dk;
+        DKint[idx] = dk;
+        rk4_thermal(P, S, cc, dk, a.cur, k);
+    }
+}
+
 // Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
 // are parked in the row's LDS slots and stepped by a rolled loop: the per-cell work (two
 // RK4 with three sincos each, the heat source) is too large to unroll E times, and a
@@ -210,7 +230,6 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
         fb[j] = make_float2(v.z, v.w);
     }
     auto sync = [] { __syncthreads(); };
-    const ThermConst &k = a.k;
     float2 *la = lds[0][rr], *lb = lds[1][rr];
     if (a.flags & TROW_FWD) fft2fe<N, E, -1>(fa, fb, la, lb, t, 1, tw, sync);
     __syncthreads();                          // the transform's last LDS reads are done
@@ -224,21 +243,8 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
 #pragma unroll 1
     for (int j = 0; j < E; j++) {
         const int o = lds_pad(t + j * TT);
-        const size_t idx = base + t + j * TT;
         float2 P = la[o], S = lb[o];
-        const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
-        if (a.flags & (TROW_NPREV | TROW_QPREV)) {
-            const CellPhys cp = cell_phys(k, NT[sprev + idx]);
-            if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
-            if (a.flags & TROW_QPREV) Q[sprev + idx] = heat_source(k, cp, P, S);
-        }
-        if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
-        if (a.flags & TROW_NCUR) {
-            const CellPhys cc = cell_phys(k, NT[scur + idx]);
-            const float dk = dkprev + cc.dk;
-            DKint[idx] = dk;
-            rk4_thermal(P, S, cc, dk, a.cur, k);
-        }
+        therm_cell(P, S, base + t + j * TT, sprev, scur, Uout, NT, Q, DKint, a);
         la[o] = P;
         lb[o] = S;
     }
@@ -250,6 +256,64 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
     if (a.flags & TROW_BWD) fft2fe<N, E, +1>(fa, fb, la, lb, t, 1, tw, sync);
 #pragma unroll
     for (int j = 0; j < E; j++) U[base + t + j * TT] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// Wide layout <N, E, W>: one row per workgroup of W waves.  The first T = N/E lanes (one
+// wavefront or less) run the two transforms; between them every thread of the group
+// steps N/(64 W) cells of the row, so the per-cell work spreads over W times as many
+// wavefronts as the transforms need.
+template <int N, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ U, float4 *__restrict__ Uout,
+                                                           const float2 *__restrict__ twg,
+                                                           const float2 *__restrict__ NT, float *__restrict__ Q,
+                                                           float *__restrict__ DKint, ThermRowArgs a) {
+    constexpr int TT = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    static_assert(TT <= 64 && N % THREADS == 0, "the transforms run in one wavefront");
+    __shared__ float2 lds[2][L];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    __syncthreads();
+    auto sync = [] { wave_sync(); };
+    if (tid < TT) {
+        float2 fa[E], fb[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const float4 v = U[base + tid + j * TT];
+            fa[j] = make_float2(v.x, v.y);
+            fb[j] = make_float2(v.z, v.w);
+        }
+        if (a.flags & TROW_FWD) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            lds[0][lds_pad(tid + j * TT)] = fa[j];
+            lds[1][lds_pad(tid + j * TT)] = fb[j];
+        }
+    }
+    __syncthreads();
+    const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
+#pragma unroll 1
+    for (int c = tid; c < N; c += THREADS) {
+        const int o = lds_pad(c);
+        float2 P = lds[0][o], S = lds[1][o];
+        therm_cell(P, S, base + c, sprev, scur, Uout, NT, Q, DKint, a);
+        lds[0][o] = P;
+        lds[1][o] = S;
+    }
+    __syncthreads();
+    if (tid < TT) {
+        float2 fa[E], fb[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            fa[j] = lds[0][lds_pad(tid + j * TT)];
+            fb[j] = lds[1][lds_pad(tid + j * TT)];
+        }
+        if (a.flags & TROW_BWD) fft2fe<N, E, +1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
+#pragma unroll
+        for (int j = 0; j < E; j++) U[base + tid + j * TT] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -288,7 +352,15 @@ static hipError_t therm_row_launch(float4 *U, float4 *Uout, const float2 *tw, co
     return hipGetLastError();
 }
 
+template <int N, int E, int W>
+static hipError_t therm_wide_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
+                                    float *DKint, const ThermRowArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_therm_row_wide<N, E, W>), dim3(N), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
+    return hipGetLastError();
+}
+
 #define THRV(N, E, W) return therm_row_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
+#define THRW(N, E, W) return therm_wide_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
 
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
                                    float *DKint, const ThermRowArgs &a, hipStream_t st, int variant) {
@@ -298,21 +370,30 @@ hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 
         case 64 * 16 + 1: THRV(64, 4, 1);
         case 64 * 16 + 2: THRV(64, 16, 1);
         case 64 * 16 + 3: THRV(64, 8, 2);
+        case 64 * 16 + 4: THRW(64, 4, 1);
+        case 64 * 16 + 5: THRW(64, 8, 1);
         case 128 * 16 + 0: THRV(128, 16, 1);
         case 128 * 16 + 1: THRV(128, 8, 1);
         case 128 * 16 + 2: THRV(128, 4, 1);
         case 128 * 16 + 3: THRV(128, 4, 2);
+        case 128 * 16 + 4: THRW(128, 4, 2);
+        case 128 * 16 + 5: THRW(128, 4, 1);
         case 256 * 16 + 0: THRV(256, 16, 1);
         case 256 * 16 + 1: THRV(256, 8, 1);
         case 256 * 16 + 2: THRV(256, 4, 1);
         case 256 * 16 + 3: THRV(256, 4, 4);
+        case 256 * 16 + 4: THRW(256, 4, 4);
+        case 256 * 16 + 5: THRW(256, 4, 2);
         case 512 * 16 + 0: THRV(512, 8, 1);
         case 512 * 16 + 1: THRV(512, 4, 2);
         case 512 * 16 + 2: THRV(512, 16, 1);
         case 512 * 16 + 3: THRV(512, 8, 2);
+        case 512 * 16 + 4: THRW(512, 8, 8);
+        case 512 * 16 + 5: THRW(512, 8, 4);
         default: return hipErrorInvalidValue;
     }
 }
 #undef THRV
+#undef THRW
 
 }  // namespace shg

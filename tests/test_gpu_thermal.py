@@ -134,7 +134,7 @@ def test_thermal_every_layout_variant_matches_oracle(gpu, n):
     rfh, rsh, rQ = orc.thermal_cwes(p, T, fh0, sh0)
     with gpu.Context(p) as ctx:
         ctx.set_temperature(T)
-        for rv in range(4):
+        for rv in range(6):
             ctx.set_variants(rv, 0)
             ctx.set_fields(fh0, sh0)
             ctx.forward()
