@@ -28,6 +28,7 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st);
 hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st);
+hipError_t launch_copy_cells(float4 *dst, const float4 *src, int64_t n, hipStream_t st);
 // Per-step sums of the gradient partials: out[s*6 + 0..4] from g1 + g2, out[s*6 + 5] from gH.
 hipError_t launch_adj_reduce(const double *g1, const double *g2, int rparts, const double *gH, int cparts, int steps,
                              double *out, hipStream_t st);

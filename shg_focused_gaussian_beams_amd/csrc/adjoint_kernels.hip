@@ -15,6 +15,8 @@
 // the temperature gradient).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "adjoint_kernels.h"
 #include "fft_lds.h"
 #include "nl_step.h"
@@ -377,6 +379,11 @@ __global__ __launch_bounds__(256) void k_adj_seed(const float4 *__restrict__ U, 
     L[i] = make_float4(0.0f, 0.0f, scale * v.z, scale * v.w);
 }
 
+// dst[i] = src[i] (field copies inside captured graphs: kernels, not copy nodes).
+__global__ __launch_bounds__(256) void k_copy_cells(float4 *__restrict__ dst, const float4 *__restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 // Per step s (one workgroup each): out[s*6 + q] = sum over the row-pass partials of
 // g1 + g2 (q < 5: c1.re, c1.im, c2.re, c2.im, tpa) and out[s*6 + 5] = sum of the
 // column-pass partials gH, in a fixed order.
@@ -456,6 +463,13 @@ hipError_t launch_adj_reduce(const double *g1, const double *g2, int rparts, con
                              double *out, hipStream_t st) {
     if (steps <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_adj_reduce, dim3(steps), dim3(64), 0, st, g1, g2, rparts, gH, cparts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_cells(float4 *dst, const float4 *src, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_copy_cells, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
     return hipGetLastError();
 }
 

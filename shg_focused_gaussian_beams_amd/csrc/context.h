@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <complex>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -24,6 +26,9 @@ struct AdjointBuffers {
     hipEvent_t ev_u = nullptr;                 // the forward state buffer is free for `ts`
     hipEvent_t ev_tape[2] = {nullptr, nullptr};  // tape slot k written
     hipEvent_t ev_free[2] = {nullptr, nullptr};  // tape slot k read by the adjoint
+    hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;  // captured forward / adjoint for interval k
+    int g_row_v = -1, g_col_v = -1;            // plain-pass layouts the forward graph was captured with
+    std::vector<std::complex<double>> pump_df, pump_dn;   // d(pump)/d(focus), d(pump)/d(n_p) per cell
 };
 
 struct shg_ctx {
@@ -66,3 +71,10 @@ int shg_internal_fail(int code, const std::string &msg);
 int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count);
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
 double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);
+// Captures the work `enqueue` issues on `st` (and on streams forked from it through
+// events) into an executable graph.
+int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out);
+// Average time in ms of `one` (a sequence of launches on `st`), replayed from a graph of
+// `reps` repetitions after one warm-up replay.
+int shg_internal_time_graph(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, const std::function<int()> &one,
+                            float *ms);

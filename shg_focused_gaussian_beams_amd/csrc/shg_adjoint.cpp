@@ -75,6 +75,8 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
         AHIP_TRY(hipMemcpy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice));
     }
     if (b.k != K || b.nseg != nseg) {
+        for (hipGraphExec_t *g : {&b.g_fwd, &b.g_adj})
+            if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
         for (float4 **ptr : {&b.ckpt, &b.a2, &b.spec})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
         const int slots = nseg > 1 ? 2 : 1;
@@ -132,60 +134,89 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
                              write_ckpt ? b.ckpt + (size_t)j * n2 : nullptr, j * K, 0};
     };
 
-    // 1. forward: checkpoints of segments 0 .. nseg-2, the last segment taped
-    AHIP_TRY(hipEventRecord(c->e0, st));
-    AHIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
-    for (int j = 0; j < nseg - 1; j++) {
-        AHIP_TRY(hipMemcpyAsync(b.ckpt + (size_t)j * n2, c->U, n2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
-        ARC_TRY(shg_internal_enqueue_march_plain(c, j * K, K));
-    }
-    ARC_TRY(enqueue_tape(c, tape_of(nseg - 1, true), M, st));
-    AHIP_TRY(hipEventRecord(c->e1, st));
-    AHIP_TRY(hipEventSynchronize(c->e1));
-    float fwd_ms = 0.f;
-    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+    // 1. forward: checkpoints of segments 0 .. nseg-2, the last segment taped, the powers
+    auto enqueue_forward = [&]() -> int {
+        AHIP_TRY(shg::launch_copy_cells(c->U, c->U0, (int64_t)n2, st));
+        for (int j = 0; j < nseg - 1; j++) {
+            AHIP_TRY(shg::launch_copy_cells(b.ckpt + (size_t)j * n2, c->U, (int64_t)n2, st));
+            ARC_TRY(shg_internal_enqueue_march_plain(c, j * K, K));
+        }
+        ARC_TRY(enqueue_tape(c, tape_of(nseg - 1, true), M, st));
+        AHIP_TRY(shg::launch_power(c->U, (int64_t)n2, c->part, c->nparts, c->red, st));
+        return SHG_OK;
+    };
 
-    // efficiency and the adjoint seed
+    // 2. per segment, last first: the adjoint on `st`, the next segment's tape on `ts`
     const shg::phys::Consts k(c->p);
     const double T = c->p.temp;
     const double eps0 = 8.8541878128e-6, cl = 299.792458;
     const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
     const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
     const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
-    double sf = 0.0, ss = 0.0;
-    ARC_TRY(shg_internal_reduce_powers(c, c->U, &sf, &ss));
-    const double pin = c->p_fh_in, J = cs * ss / pin;
-
-    AHIP_TRY(hipEventRecord(c->e0, st));
-    AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
-    AHIP_TRY(hipEventRecord(b.ev_u, st));
-    AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
-
-    // 2. per segment, last first: the adjoint on `st`, the next tape on `ts`
     const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
     const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
     const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
     const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
-    for (int j = nseg - 1; j >= 0; j--) {
-        if (j >= 1) {
-            // slot (j-1) % 2 was last read by the adjoint of segment j+1
-            if (j + 1 <= nseg - 1) AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_free[(j - 1) & 1], 0));
-            AHIP_TRY(hipMemcpyAsync(c->U, b.ckpt + (size_t)(j - 1) * n2, n2 * sizeof(float4), hipMemcpyDeviceToDevice,
-                                    b.ts));
-            ARC_TRY(enqueue_tape(c, tape_of(j - 1, false), j * K, b.ts));
-            AHIP_TRY(hipEventRecord(b.ev_tape[(j - 1) & 1], b.ts));
+    auto enqueue_adjoint = [&]() -> int {
+        if (nseg > 1) {                                     // forks `ts` off `st`
+            AHIP_TRY(hipEventRecord(b.ev_u, st));
+            AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
         }
-        if (j < nseg - 1) AHIP_TRY(hipStreamWaitEvent(st, b.ev_tape[j & 1], 0));
-        const int s0 = j * K, s1 = std::min(M, s0 + K);
-        const shg::TapeArgs tp = tape_of(j, false);
-        AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st));
-        for (int s = s1 - 1; s >= s0; s--) {
-            AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
-            AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1, b.g2, st));
+        for (int j = nseg - 1; j >= 0; j--) {
+            if (j >= 1) {
+                // slot (j-1) % 2 was last read by the adjoint of segment j+1
+                if (j + 1 <= nseg - 1) AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_free[(j - 1) & 1], 0));
+                AHIP_TRY(shg::launch_copy_cells(c->U, b.ckpt + (size_t)(j - 1) * n2, (int64_t)n2, b.ts));
+                ARC_TRY(enqueue_tape(c, tape_of(j - 1, false), j * K, b.ts));
+                AHIP_TRY(hipEventRecord(b.ev_tape[(j - 1) & 1], b.ts));
+            }
+            if (j < nseg - 1) AHIP_TRY(hipStreamWaitEvent(st, b.ev_tape[j & 1], 0));   // joins `ts` at j = 0
+            const int s0 = j * K, s1 = std::min(M, s0 + K);
+            const shg::TapeArgs tp = tape_of(j, false);
+            AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st));
+            for (int s = s1 - 1; s >= s0; s--) {
+                AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
+                AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1,
+                                             b.g2, st));
+            }
+            if (j >= 2) AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
         }
-        AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
+        AHIP_TRY(shg::launch_adj_reduce(b.g1, b.g2, rb, b.gH, cb, M, b.gs, st));
+        return SHG_OK;
+    };
+
+    const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+    if (c->use_graph && (b.g_row_v != rv || b.g_col_v != cv))
+        for (hipGraphExec_t *g : {&b.g_fwd, &b.g_adj})
+            if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
+    if (c->use_graph && !b.g_fwd) {
+        ARC_TRY(shg_internal_capture(st, enqueue_forward, &b.g_fwd));
+        ARC_TRY(shg_internal_capture(st, enqueue_adjoint, &b.g_adj));
+        b.g_row_v = rv;
+        b.g_col_v = cv;
     }
-    AHIP_TRY(shg::launch_adj_reduce(b.g1, b.g2, rb, b.gH, cb, M, b.gs, st));
+
+    AHIP_TRY(hipEventRecord(c->e0, st));
+    if (c->use_graph) {
+        AHIP_TRY(hipGraphLaunch(b.g_fwd, st));
+    } else {
+        ARC_TRY(enqueue_forward());
+    }
+    AHIP_TRY(hipEventRecord(c->e1, st));
+    double2 sums;
+    AHIP_TRY(hipMemcpyAsync(&sums, c->red, sizeof(double2), hipMemcpyDeviceToHost, st));
+    AHIP_TRY(hipStreamSynchronize(st));
+    float fwd_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+    const double pin = c->p_fh_in, J = cs * sums.y / pin;
+
+    AHIP_TRY(hipEventRecord(c->e0, st));
+    AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
+    if (c->use_graph) {
+        AHIP_TRY(hipGraphLaunch(b.g_adj, st));
+    } else {
+        ARC_TRY(enqueue_adjoint());
+    }
     AHIP_TRY(hipEventRecord(c->e1, st));
     AHIP_TRY(hipEventSynchronize(c->e1));
     float adj_ms = 0.f;
@@ -235,19 +266,26 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
     const double zr = M_PI * np * w0 * w0 / lam0;
     const double ap0 = std::sqrt(4.0 * c->p.power / (eps0 * cl * M_PI * np * w0 * w0));
     const cd Mq(1.0, -f / zr);
+    // the pump's derivative fields depend on the context's parameters only: built once
+    std::vector<cd> &pdf = b.pump_df, &pdn = b.pump_dn;
+    if (pdf.size() != n2) {
+        pdf.resize(n2);
+        pdn.resize(n2);
+        for (int iy = 0; iy < n; iy++)
+            for (int ix = 0; ix < n; ix++) {
+                const double xx = (ix - 0.5 * n) * (double)k.dx, yy = (iy - 0.5 * n) * (double)k.dy;
+                const double r2 = xx * xx + yy * yy;
+                const cd A = ap0 / Mq * std::exp(-r2 / (w0 * w0 * Mq));
+                const cd dA_dM = A * (-1.0 / Mq + r2 / (w0 * w0 * Mq * Mq));
+                pdf[(size_t)iy * n + ix] = dA_dM * cd(0.0, -1.0 / zr);
+                pdn[(size_t)iy * n + ix] = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
+            }
+    }
     double d_focus = 0.0, dT_pump = 0.0;
-    for (int iy = 0; iy < n; iy++)
-        for (int ix = 0; ix < n; ix++) {
-            const double xx = (ix - 0.5 * n) * (double)k.dx, yy = (iy - 0.5 * n) * (double)k.dy;
-            const double r2 = xx * xx + yy * yy;
-            const cd A = ap0 / Mq * std::exp(-r2 / (w0 * w0 * Mq));
-            const cd dA_dM = A * (-1.0 / Mq + r2 / (w0 * w0 * Mq * Mq));
-            const cd dA_df = dA_dM * cd(0.0, -1.0 / zr);
-            const cd dA_dn = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
-            const cd g = grad[(size_t)iy * n + ix];
-            d_focus += (std::conj(g) * dA_df).real();
-            dT_pump += (std::conj(g) * dA_dn).real() * dnp;
-        }
+    for (size_t i = 0; i < n2; i++) {
+        d_focus += grad[i].real() * pdf[i].real() + grad[i].imag() * pdf[i].imag();
+        dT_pump += (grad[i].real() * pdn[i].real() + grad[i].imag() * pdn[i].imag()) * dnp;
+    }
 
     res->eta = J;
     res->d_eta_d_focus = d_focus;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -23,6 +24,9 @@
 
 // Error reporting shared with shg_capi.cpp.
 int shg_internal_fail(int code, const std::string &msg);
+int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out);
+int shg_internal_time_graph(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, const std::function<int()> &one,
+                            float *ms);
 
 namespace {
 
@@ -116,26 +120,25 @@ int after_fields_set(shg_batch *b) {
 // Per size class, times `reps` launches of every layout variant over all the class's
 // instances and keeps the fastest (clobbers U; shg_batch_run restores it from U0).
 // Per size class, times every (row layout, column layout) pair as the schedule runs them
-// (a row pass then a column pass over the class) and keeps the fastest pair.
+// (a row pass then a column pass over the class, replayed from a graph) and keeps the
+// fastest pair.
 int tune_batch(shg_batch *b, int reps) {
     for (int c = 0; c < 4; c++) {
         if (!b->cls_cnt[c]) continue;
         const int n = kSizes[c], cnt = b->cls_cnt[c];
         const shg::InstDesc *d = b->desc + b->cls_off[c];
-        auto step = [&](int rv, int cv) {
-            const hipError_t e = shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, rv);
-            return e != hipSuccess ? e : shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, cv);
-        };
         float best = 1e30f;
         for (int rv = 0; rv < shg::kRowVariantCount; rv++) {
             for (int cv = 0; cv < shg::kColVariantCount; cv++) {
-                BHIP_TRY(step(rv, cv));
-                BHIP_TRY(hipEventRecord(b->e0, b->stream));
-                for (int r = 0; r < reps; r++) BHIP_TRY(step(rv, cv));
-                BHIP_TRY(hipEventRecord(b->e1, b->stream));
-                BHIP_TRY(hipEventSynchronize(b->e1));
                 float ms = 0.f;
-                BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+                const int rc = shg_internal_time_graph(b->stream, b->e0, b->e1, reps, [&] {
+                    hipError_t e = shg::launch_row_pass(n, d, cnt, b->tw[c], 0, shg::ROW_AUTO, b->stream, rv);
+                    if (e == hipSuccess) e = shg::launch_col_pass(n, d, cnt, b->tw[c], b->stream, cv);
+                    return e == hipSuccess ? (int)SHG_OK
+                                           : shg_internal_fail(SHG_E_DEVICE, std::string("tuning pass: ") +
+                                                                                 hipGetErrorString(e));
+                }, &ms);
+                if (rc) return rc;
                 if (ms < best) { best = ms; b->row_v[c] = rv; b->col_v[c] = cv; }
             }
         }
@@ -346,20 +349,8 @@ int shg_batch_run(shg_batch *b) {
         }
     }
     BHIP_TRY(hipMemcpyAsync(b->U, b->U0, b->total_cells * sizeof(float4), hipMemcpyDeviceToDevice, b->stream));
-    if (!b->gexec) {
-        hipGraph_t graph;
-        BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_batch(b);
-        const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
-        if (rc) return rc;
-        if (ee != hipSuccess) return shg_internal_fail(SHG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-        const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) {
-            b->gexec = nullptr;
-            return shg_internal_fail(SHG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-        }
-    }
+    if (!b->gexec)
+        if (int rc = shg_internal_capture(b->stream, [b] { return enqueue_batch(b); }, &b->gexec)) return rc;
     BHIP_TRY(hipEventRecord(b->e0, b->stream));
     BHIP_TRY(hipGraphLaunch(b->gexec, b->stream));
     BHIP_TRY(hipEventRecord(b->e1, b->stream));

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -45,6 +46,48 @@ bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512;
 
 int shg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
 
+int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out) {
+    *out = nullptr;
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    const hipError_t ee = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ee != hipSuccess) return fail(SHG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        *out = nullptr;
+        return fail(SHG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return SHG_OK;
+}
+
+int shg_internal_time_graph(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, const std::function<int()> &one,
+                            float *ms) {
+    hipGraphExec_t g = nullptr;
+    const int rc = shg_internal_capture(st, [&] {
+        for (int r = 0; r < reps; r++)
+            if (int e = one()) return e;
+        return (int)SHG_OK;
+    }, &g);
+    if (rc) return rc;
+    hipError_t e = hipGraphLaunch(g, st);                      // warm-up replay
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(g, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipGraphExecDestroy(g);
+    if (e != hipSuccess) return fail(SHG_E_DEVICE, std::string("timing replay: ") + hipGetErrorString(e));
+    *ms = t / reps;
+    return SHG_OK;
+}
+
 
 namespace {
 
@@ -77,17 +120,6 @@ int enqueue_march_plain(shg_ctx *c, int first, int count) {
 // restores it from U0).
 shg::ThermStep therm_step(const shg_ctx *c, int s);
 
-// Average time of `reps` launches of `launch` (after one warm-up launch), in ms.
-template <typename F> static int time_launches(shg_ctx *c, int reps, F launch, float *ms) {
-    HIP_TRY(launch());
-    HIP_TRY(hipEventRecord(c->e0, c->stream));
-    for (int r = 0; r < reps; r++) HIP_TRY(launch());
-    HIP_TRY(hipEventRecord(c->e1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->e1));
-    HIP_TRY(hipEventElapsedTime(ms, c->e0, c->e1));
-    return SHG_OK;
-}
-
 // Row-pass arguments of a THERMAL middle step (every stage of the pass switched on).
 static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
     shg::ThermRowArgs a{};
@@ -99,11 +131,12 @@ static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
 }
 
 // Times every (row layout, column layout) pair on this context's size as the march runs
-// them, a row pass then a column pass, and keeps the fastest pair: timed alone, a pass
-// finds its input in the L2 of the XCD that wrote it, but in the march every row pass
-// reads what the column pass scattered over all XCDs and vice versa.  The passes run on
-// the context's buffers: the caller restores the fields afterwards (a THERMAL march
-// rewrites DKint and every heat-source slice it touches from scratch).
+// them, a row pass then a column pass replayed from a graph, and keeps the fastest pair:
+// timed alone, a pass finds its input in the L2 of the XCD that wrote it, but in the
+// march every row pass reads what the column pass scattered over all XCDs and vice
+// versa; launched one by one from the host, the launch cost hides the passes' own.  The
+// passes run on the context's buffers: the caller restores the fields afterwards (a
+// THERMAL march rewrites DKint and every heat-source slice it touches from scratch).
 int tune_variants(shg_ctx *c, int reps) {
     const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
     const int nrow = c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
@@ -118,12 +151,14 @@ int tune_variants(shg_ctx *c, int reps) {
     for (int rv = 0; rv < nrow; rv++) {
         for (int cv = 0; cv < shg::kColVariantCount; cv++) {
             float ms = 0.f;
-            RC_TRY(time_launches(c, reps, [&] {
-                const hipError_t e = c->p.thermal
+            RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
+                hipError_t e = c->p.thermal
                     ? shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a,
                                                    c->stream, rv)
                     : shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv);
-                return e != hipSuccess ? e : shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv);
+                if (e == hipSuccess) e = shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv);
+                return e == hipSuccess ? (int)SHG_OK : fail(SHG_E_DEVICE, std::string("tuning pass: ") +
+                                                                              hipGetErrorString(e));
             }, &ms));
             if (ms < best) { best = ms; br = rv; bc = cv; }
         }
@@ -149,14 +184,13 @@ shg::ThermStep therm_step(const shg_ctx *c, int s) {
 
 // THERMAL build: all nz steps of Solver::CWES (src/headers/Solver.h:245-276) with the
 // heat-source update after every step; Uout receives the fields before the last step.
+// Reads the index table (launch_index_table of the current temperature first).
 int enqueue_march_thermal(shg_ctx *c) {
     const int nz = c->p.nz;
     shg::ThermRowArgs a{};
     a.k = c->tc;
     a.cur = therm_step(c, 0);
     a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
-    HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal, c->tc.lp,
-                                    c->tc.ls, c->stream));
     const float2 *NT = c->ntab;
     const int tv = c->row_v < 0 ? 0 : c->row_v;
     HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
@@ -203,6 +237,8 @@ void release(shg_ctx *c) {
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
                       (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam})
         if (ptr) (void)hipFree(ptr);
+    for (hipGraphExec_t g : {c->adj.g_fwd, c->adj.g_adj})
+        if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : {c->adj.ev_u, c->adj.ev_tape[0], c->adj.ev_tape[1], c->adj.ev_free[0], c->adj.ev_free[1]})
         if (e) (void)hipEventDestroy(e);
     if (c->adj.ts) (void)hipStreamDestroy(c->adj.ts);
@@ -273,27 +309,16 @@ int timed_march(shg_ctx *c) {
             HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
         }
     }
-    if (!c->p.thermal && c->use_graph && !c->gexec) {
-        hipGraph_t graph;
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_march_plain(c, 0, c->steps);
-        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-        if (rc) return rc;
-        if (ee != hipSuccess) return fail(SHG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-        const hipError_t ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) {
-            c->gexec = nullptr;
-            return fail(SHG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-        }
-    }
+    auto enqueue = [c] { return c->p.thermal ? enqueue_march_thermal(c) : enqueue_march_plain(c, 0, c->steps); };
+    if (c->use_graph && !c->gexec) RC_TRY(shg_internal_capture(c->stream, enqueue, &c->gexec));
     HIP_TRY(hipEventRecord(c->e0, c->stream));
-    if (c->p.thermal) {
-        RC_TRY(enqueue_march_thermal(c));
-    } else if (c->use_graph) {
+    if (c->p.thermal)
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
+                                        c->tc.lp, c->tc.ls, c->stream));
+    if (c->use_graph) {
         HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
     } else {
-        RC_TRY(enqueue_march_plain(c, 0, c->steps));
+        RC_TRY(enqueue());
     }
     HIP_TRY(hipEventRecord(c->e1, c->stream));
     HIP_TRY(hipEventSynchronize(c->e1));
