@@ -211,6 +211,12 @@ int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *
  * Signal_out state (after its own nz-1 steps). */
 int shg_batch_run(shg_batch *b);
 
+/* Schedule of shg_batch_run (no counterpart in the reference): every grid size is cut
+ * into groups of at most group_mib MiB of fields, each group marches all its steps
+ * before the next one of its stream (its fields stay in the last-level cache), and the
+ * groups are spread over `streams` (1..4) concurrent streams.  Defaults: 64 MiB, 2. */
+int shg_batch_set_schedule(shg_batch *b, int group_mib, int streams);
+
 /* Current fields (after shg_batch_run: the outputs) and the z = 0 fields of one instance. */
 int shg_batch_get_fields(shg_batch *b, int index, float *fh, float *sh);
 int shg_batch_get_input(shg_batch *b, int index, float *fh, float *sh);

@@ -92,6 +92,7 @@ def load_library(path=LIB_PATH):
         "shg_batch_init_fields": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint64]),
         "shg_batch_set_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "shg_batch_run": (ctypes.c_int, [vp]),
+        "shg_batch_set_schedule": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_batch_get_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "shg_batch_get_input": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "shg_batch_powers": (ctypes.c_int, [vp, dp, dp, dp]),
@@ -290,6 +291,10 @@ class Batch:
 
     def run(self):
         _check(self.lib.shg_batch_run(self.h))
+
+    def set_schedule(self, group_mib=64, streams=2):
+        """Group size (MiB of fields marched together) and number of concurrent streams."""
+        _check(self.lib.shg_batch_set_schedule(self.h, group_mib, streams))
 
     def _get(self, fn, i):
         shape = (self.params[i].ny, self.params[i].nx)

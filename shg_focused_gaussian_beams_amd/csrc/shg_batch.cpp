@@ -37,6 +37,7 @@ namespace {
     } while (0)
 
 const int kSizes[4] = {64, 128, 256, 512};
+constexpr int kMaxStreams = 4, kDefaultGroupMiB = 64, kDefaultStreams = 2;
 
 int size_class(int n) {
     for (int c = 0; c < 4; c++)
@@ -71,6 +72,20 @@ struct shg_batch {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0.0;
     int row_v[4] = {-1, -1, -1, -1}, col_v[4] = {-1, -1, -1, -1};   // layout variant per size class
+    // schedule: each size class is cut into groups of at most group_mb MiB of fields; a
+    // group marches all its steps before the next group of its stream starts, so its
+    // fields stay in the last-level cache between the passes; groups are spread over
+    // nstreams streams (longest first onto the least-loaded one), which overlaps the
+    // latency-bound small grids with the large ones.
+    struct Group {
+        int c, first, count, steps;
+        double work;
+        int stream;
+    };
+    int group_mb = kDefaultGroupMiB, nstreams = kDefaultStreams;
+    std::vector<Group> groups;
+    hipStream_t xs[kMaxStreams - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxStreams - 1] = {};
 };
 
 namespace {
@@ -78,6 +93,11 @@ namespace {
 void batch_release(shg_batch *b) {
     if (!b) return;
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipStream_t x : b->xs)
+        if (x) (void)hipStreamDestroy(x);
+    for (hipEvent_t e : b->ev_join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     for (void *ptr : {(void *)b->U, (void *)b->U0, (void *)b->G, (void *)b->coef, (void *)b->desc, (void *)b->ncell,
                       (void *)b->pow, (void *)b->tw[0], (void *)b->tw[1], (void *)b->tw[2], (void *)b->tw[3]})
         if (ptr) (void)hipFree(ptr);
@@ -120,12 +140,15 @@ int after_fields_set(shg_batch *b) {
 // Per size class, times `reps` launches of every layout variant over all the class's
 // instances and keeps the fastest (clobbers U; shg_batch_run restores it from U0).
 // Per size class, times every (row layout, column layout) pair as the schedule runs them
-// (a row pass then a column pass over the class, replayed from a graph) and keeps the
-// fastest pair.
+// (a row pass then a column pass over a group of the class, replayed from a graph) and
+// keeps the fastest pair.
 int tune_batch(shg_batch *b, int reps) {
     for (int c = 0; c < 4; c++) {
         if (!b->cls_cnt[c]) continue;
-        const int n = kSizes[c], cnt = b->cls_cnt[c];
+        int cnt = 1;
+        for (const auto &gr : b->groups)
+            if (gr.c == c) cnt = std::max(cnt, gr.count);
+        const int n = kSizes[c];
         const shg::InstDesc *d = b->desc + b->cls_off[c];
         float best = 1e30f;
         for (int rv = 0; rv < shg::kRowVariantCount; rv++) {
@@ -146,33 +169,59 @@ int tune_batch(shg_batch *b, int reps) {
     return SHG_OK;
 }
 
-// The lock-step schedule: START pass for everybody, then per step a column pass and an
-// AUTO row pass (MID while more steps follow, END on an instance's last step) over the
-// instances of each size that are still marching.
-int enqueue_batch(shg_batch *b) {
-    int rv[4], cv[4];
+// Cuts every size class into groups (slots of a class are sorted by steps, descending)
+// and assigns them to streams.
+void build_groups(shg_batch *b) {
+    b->groups.clear();
+    const double cap = (double)b->group_mb * (1 << 20);
     for (int c = 0; c < 4; c++) {
-        rv[c] = b->row_v[c] < 0 ? 0 : b->row_v[c];
-        cv[c] = b->col_v[c] < 0 ? 0 : b->col_v[c];
-    }
-    for (int c = 0; c < 4; c++)
-        if (b->cls_cnt[c])
-            BHIP_TRY(shg::launch_row_pass(kSizes[c], b->desc + b->cls_off[c], b->cls_cnt[c], b->tw[c], 0,
-                                          shg::ROW_START, b->stream, rv[c]));
-    for (int s = 0; s < b->max_steps; s++) {
-        int active[4];
-        for (int c = 0; c < 4; c++) {
-            const auto &v = b->cls_steps[c];   // descending: the entries > s come first
-            active[c] = 0;
-            while (active[c] < (int)v.size() && v[active[c]] > s) active[c]++;
+        if (!b->cls_cnt[c]) continue;
+        const double inst = (double)kSizes[c] * kSizes[c] * sizeof(float4);
+        const int g = std::max(1, (int)(cap / inst));
+        for (int first = 0; first < b->cls_cnt[c]; first += g) {
+            shg_batch::Group gr{c, first, std::min(g, b->cls_cnt[c] - first), b->cls_steps[c][first], 0.0, 0};
+            for (int i = first; i < first + gr.count; i++) gr.work += (double)kSizes[c] * kSizes[c] * b->cls_steps[c][i];
+            b->groups.push_back(gr);
         }
-        for (int c = 0; c < 4; c++)
-            if (active[c])
-                BHIP_TRY(shg::launch_col_pass(kSizes[c], b->desc + b->cls_off[c], active[c], b->tw[c], b->stream, cv[c]));
-        for (int c = 0; c < 4; c++)
-            if (active[c])
-                BHIP_TRY(shg::launch_row_pass(kSizes[c], b->desc + b->cls_off[c], active[c], b->tw[c], s, shg::ROW_AUTO,
-                                              b->stream, rv[c]));
+    }
+    std::stable_sort(b->groups.begin(), b->groups.end(),
+                     [](const shg_batch::Group &x, const shg_batch::Group &y) { return x.work > y.work; });
+    std::vector<double> load(b->nstreams, 0.0);
+    for (auto &gr : b->groups) {
+        gr.stream = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[gr.stream] += gr.work;
+    }
+}
+
+// One group's march: START row pass, then per step a column pass and an AUTO row pass
+// (MID while more steps follow, END on an instance's last step) over the group's
+// instances still marching.
+int enqueue_group(shg_batch *b, const shg_batch::Group &gr, hipStream_t st) {
+    const int n = kSizes[gr.c], rv = b->row_v[gr.c] < 0 ? 0 : b->row_v[gr.c], cv = b->col_v[gr.c] < 0 ? 0 : b->col_v[gr.c];
+    const shg::InstDesc *d = b->desc + b->cls_off[gr.c] + gr.first;
+    const int *steps = b->cls_steps[gr.c].data() + gr.first;   // descending
+    BHIP_TRY(shg::launch_row_pass(n, d, gr.count, b->tw[gr.c], 0, shg::ROW_START, st, rv));
+    int active = gr.count;
+    for (int s = 0; s < gr.steps; s++) {
+        while (active > 0 && steps[active - 1] <= s) active--;
+        BHIP_TRY(shg::launch_col_pass(n, d, active, b->tw[gr.c], st, cv));
+        BHIP_TRY(shg::launch_row_pass(n, d, active, b->tw[gr.c], s, shg::ROW_AUTO, st, rv));
+    }
+    return SHG_OK;
+}
+
+// The whole schedule on b->stream, forking the other streams off it and joining them.
+int enqueue_batch(shg_batch *b) {
+    const int S = b->nstreams;
+    if (S > 1) {
+        BHIP_TRY(hipEventRecord(b->ev_fork, b->stream));
+        for (int k = 1; k < S; k++) BHIP_TRY(hipStreamWaitEvent(b->xs[k - 1], b->ev_fork, 0));
+    }
+    for (const auto &gr : b->groups)
+        if (int rc = enqueue_group(b, gr, gr.stream == 0 ? b->stream : b->xs[gr.stream - 1])) return rc;
+    for (int k = 1; k < S; k++) {
+        BHIP_TRY(hipEventRecord(b->ev_join[k - 1], b->xs[k - 1]));
+        BHIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[k - 1], 0));
     }
     return SHG_OK;
 }
@@ -232,9 +281,14 @@ int shg_batch_create(const shg_params *params, int count, shg_batch **out) {
         b->cls_steps[c].push_back(b->steps[s]);
         b->t_slot[s] = (float)p.temp;
     }
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
-        hipEventCreate(&b->e1) != hipSuccess)
-        return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: stream/event creation failed"));
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess &&
+              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < kMaxStreams - 1; k++)
+        ok = hipStreamCreateWithFlags(&b->xs[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: stream/event creation failed"));
+    build_groups(b);
     if (hipMalloc(&b->U, b->total_cells * sizeof(float4)) != hipSuccess ||
         hipMalloc(&b->U0, b->total_cells * sizeof(float4)) != hipSuccess ||
         hipMalloc(&b->G, g_total * sizeof(float2)) != hipSuccess ||
@@ -337,6 +391,21 @@ int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *
     return after_fields_set(b);
 }
 
+int shg_batch_set_schedule(shg_batch *b, int group_mib, int streams) {
+    if (!b || group_mib < 1 || streams < 1 || streams > kMaxStreams)
+        return shg_internal_fail(SHG_E_INVALID, "shg_batch_set_schedule: group_mib >= 1, 1 <= streams <= " +
+                                                    std::to_string(kMaxStreams));
+    b->group_mb = group_mib;
+    b->nstreams = streams;
+    build_groups(b);
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = -1;   // group sizes changed: tune again
+    return SHG_OK;
+}
+
 int shg_batch_run(shg_batch *b) {
     if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_run: null batch");
     if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_run: set or init the fields first");
@@ -349,10 +418,16 @@ int shg_batch_run(shg_batch *b) {
         }
     }
     BHIP_TRY(hipMemcpyAsync(b->U, b->U0, b->total_cells * sizeof(float4), hipMemcpyDeviceToDevice, b->stream));
-    if (!b->gexec)
+    const char *ng = std::getenv("SHG_NO_GRAPH");
+    const bool graph = !(ng && ng[0] == '1');
+    if (graph && !b->gexec)
         if (int rc = shg_internal_capture(b->stream, [b] { return enqueue_batch(b); }, &b->gexec)) return rc;
     BHIP_TRY(hipEventRecord(b->e0, b->stream));
-    BHIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    if (graph) {
+        BHIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    } else if (int rc = enqueue_batch(b)) {
+        return rc;
+    }
     BHIP_TRY(hipEventRecord(b->e1, b->stream));
     BHIP_TRY(hipEventSynchronize(b->e1));
     float ms = 0.f;
