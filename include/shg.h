@@ -231,8 +231,9 @@ int shg_batch_last_ms(shg_batch *b, double *ms);
 /* ----- Launch-shape autotuning (no reference counterpart) ------------------------------
  * The row and column passes exist in several thread layouts (FFT elements per thread,
  * rows or columns per workgroup).  shg_forward / shg_batch_run tune them on first use
- * (timing each variant on the context's own data) unless the environment sets
- * SHG_NO_TUNE=1; these calls tune explicitly or pin a choice.  Rows 0..3, columns 0..2. */
+ * (timing each (row, column) pair, replayed as the march runs them, on the context's own
+ * data) unless the environment sets SHG_NO_TUNE=1; these calls tune explicitly or pin a
+ * choice.  Rows 0..3 (THERMAL contexts: 0..5), columns 0..4. */
 int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
 int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 

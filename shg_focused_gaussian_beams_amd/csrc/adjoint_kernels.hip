@@ -229,19 +229,23 @@ __global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_col_tape(const I
 // ADJ_END   at s: R_fwd, RK4_s^T at A_s = tp.a (the segment's checkpoint); leaves the
 //                 real-space lambda of the segment start.
 // g1[(s*N + row)*5] gets the row's sensitivities of the first RK4 of step s (at A_s),
-// g2[...] those of the second (at A''_s).
+// g2[...] those of the second (at A''_s).  One cell per thread between the transforms.
 template <int N>
 __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *__restrict__ descs,
                                                                const float2 *__restrict__ twg, int step, int mode,
                                                                TapeArgs tp, double *__restrict__ g1,
                                                                double *__restrict__ g2) {
     constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
-    static_assert(T <= 64 && N % THREADS == 0, "the transforms run in one wavefront");
+    static_assert(T <= 64 && N == THREADS, "the transforms run in one wavefront; one cell per thread");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
     const int tid = threadIdx.x;
-    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + tid;
+    const int ls = step - tp.s0;
+    // the cell's taped forward state, loaded before the transform hides its latency:
+    // START: A''_s;  MID: A''_{s-1};  END: A_s (the checkpoint)
+    const float4 tape = mode == ADJ_END ? tp.a[idx] : tp.a2[(size_t)(mode == ADJ_START ? ls : ls - 1) * n2 + idx];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
     auto sync = [] { wave_sync(); };
@@ -262,29 +266,23 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
         }
     }
     __syncthreads();
-    const int ls = step - tp.s0;
     AdjAcc acc1, acc2;
     acc_zero(acc1);
     acc_zero(acc2);
     const float4 c_s = d.coef[step];
     const float4 c_p = d.coef[step > 0 ? step - 1 : 0];
-#pragma unroll 1
-    for (int cc = tid; cc < N; cc += THREADS) {
-        const int o = lds_pad(cc);
-        const size_t idx = base + cc;
+    {
+        const int o = lds_pad(tid);
         float2 P = lds[0][o], S = lds[1][o];
         if (mode == ADJ_START) {
-            const float4 a = tp.a2[(size_t)ls * n2 + idx];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_s, d.nl, acc2);
+            rk4_adj(f2(tape.x, tape.y), f2(tape.z, tape.w), P, S, c_s, d.nl, acc2);
         } else if (mode == ADJ_END) {
-            const float4 a = tp.a[idx];
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_s, d.nl, acc1);
+            rk4_adj(f2(tape.x, tape.y), f2(tape.z, tape.w), P, S, c_s, d.nl, acc1);
         } else {
-            const float4 a = tp.a2[(size_t)(ls - 1) * n2 + idx];
-            float2 Ap = f2(a.x, a.y), As = f2(a.z, a.w);
+            float2 Ap = f2(tape.x, tape.y), As = f2(tape.z, tape.w);
             rk4_cell(Ap, As, c_p, c_p, c_p, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);   // A_s
             rk4_adj(Ap, As, P, S, c_s, d.nl, acc1);
-            rk4_adj(f2(a.x, a.y), f2(a.z, a.w), P, S, c_p, d.nl, acc2);
+            rk4_adj(f2(tape.x, tape.y), f2(tape.z, tape.w), P, S, c_p, d.nl, acc2);
         }
         lds[0][o] = P;
         lds[1][o] = S;

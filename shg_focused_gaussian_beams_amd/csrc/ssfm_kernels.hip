@@ -216,15 +216,23 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 64 * 16 + 0: COLV(64, 8, 8);
         case 64 * 16 + 1: COLV(64, 4, 8);
         case 64 * 16 + 2: COLV(64, 8, 16);
+        case 64 * 16 + 3: COLV(64, 4, 4);
+        case 64 * 16 + 4: COLV(64, 4, 16);
         case 128 * 16 + 0: COLV(128, 16, 8);
         case 128 * 16 + 1: COLV(128, 8, 8);
         case 128 * 16 + 2: COLV(128, 4, 8);
+        case 128 * 16 + 3: COLV(128, 4, 2);
+        case 128 * 16 + 4: COLV(128, 4, 4);
         case 256 * 16 + 0: COLV(256, 16, 8);
         case 256 * 16 + 1: COLV(256, 8, 8);
         case 256 * 16 + 2: COLV(256, 4, 8);
+        case 256 * 16 + 3: COLV(256, 4, 1);
+        case 256 * 16 + 4: COLV(256, 4, 2);
         case 512 * 16 + 0: COLV(512, 8, 4);
         case 512 * 16 + 1: COLV(512, 16, 4);
         case 512 * 16 + 2: COLV(512, 4, 4);
+        case 512 * 16 + 3: COLV(512, 8, 1);
+        case 512 * 16 + 4: COLV(512, 8, 2);
         default: return hipErrorInvalidValue;
     }
 }

@@ -182,24 +182,34 @@ __device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys
 // recomputes the same float sum from z' = 0 twice per step (src/headers/PhaseMatching.h:
 // 25-38, Solver.h:198-199); the sequential float additions are identical.
 //
-// One cell of the pass between the transforms (idx: cell index in the plane).
-__device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, size_t sprev, size_t scur,
-                                           float4 *__restrict__ Uout, const float2 *__restrict__ NT,
-                                           float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
+// One cell of the pass between the transforms (idx: cell index in the plane), from its
+// already loaded phase accumulator and index-table entries.
+__device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx, size_t sprev, float dkprev, float2 nnp,
+                                               float2 nnc, float4 *__restrict__ Uout, float *__restrict__ Q,
+                                               float *__restrict__ DKint, const ThermRowArgs &a) {
     const ThermConst &k = a.k;
-    const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
     if (a.flags & (TROW_NPREV | TROW_QPREV)) {
-        const CellPhys cp = cell_phys(k, NT[sprev + idx]);
+        const CellPhys cp = cell_phys(k, nnp);
         if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
         if (a.flags & TROW_QPREV) Q[sprev + idx] = heat_source(k, cp, P, S);
     }
     if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
     if (a.flags & TROW_NCUR) {
-        const CellPhys cc = cell_phys(k, NT[scur + idx]);
+        const CellPhys cc = cell_phys(k, nnc);
         const float dk = dkprev + cc.dk;
         DKint[idx] = dk;
         rk4_thermal(P, S, cc, dk, a.cur, k);
     }
+}
+
+// The same, loading the cell's inputs itself.
+__device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, size_t sprev, size_t scur,
+                                           float4 *__restrict__ Uout, const float2 *__restrict__ NT,
+                                           float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
+    const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
+    const float2 nnp = (a.flags & (TROW_NPREV | TROW_QPREV)) ? NT[sprev + idx] : make_float2(0.f, 0.f);
+    const float2 nnc = (a.flags & TROW_NCUR) ? NT[scur + idx] : make_float2(0.f, 0.f);
+    therm_cell_pre(P, S, idx, sprev, dkprev, nnp, nnc, Uout, Q, DKint, a);
 }
 
 // Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
@@ -261,18 +271,29 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
 // Wide layout <N, E, W>: one row per workgroup of W waves.  The first T = N/E lanes (one
 // wavefront or less) run the two transforms; between them every thread of the group
 // steps N/(64 W) cells of the row, so the per-cell work spreads over W times as many
-// wavefronts as the transforms need.
+// wavefronts as the transforms need.  Each thread loads its cells' phase accumulator and
+// index-table entries before the first transform, which hides their latency behind it.
 template <int N, int E, int W>
 __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ U, float4 *__restrict__ Uout,
                                                            const float2 *__restrict__ twg,
                                                            const float2 *__restrict__ NT, float *__restrict__ Q,
                                                            float *__restrict__ DKint, ThermRowArgs a) {
-    constexpr int TT = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    constexpr int TT = N / E, THREADS = 64 * W, L = LdsLen<N>::value, CPT = N / THREADS;
     static_assert(TT <= 64 && N % THREADS == 0, "the transforms run in one wavefront");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     const int tid = threadIdx.x;
     const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
+    const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
+    float dkp[CPT];
+    float2 nnp[CPT], nnc[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const size_t idx = base + tid + q * THREADS;
+        dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
+        nnp[q] = (a.flags & (TROW_NPREV | TROW_QPREV)) ? NT[sprev + idx] : make_float2(0.f, 0.f);
+        nnc[q] = (a.flags & TROW_NCUR) ? NT[scur + idx] : make_float2(0.f, 0.f);
+    }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
     auto sync = [] { wave_sync(); };
@@ -293,12 +314,11 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
         }
     }
     __syncthreads();
-    const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
-#pragma unroll 1
-    for (int c = tid; c < N; c += THREADS) {
-        const int o = lds_pad(c);
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int cc = tid + q * THREADS, o = lds_pad(cc);
         float2 P = lds[0][o], S = lds[1][o];
-        therm_cell(P, S, base + c, sprev, scur, Uout, NT, Q, DKint, a);
+        therm_cell_pre(P, S, base + cc, sprev, dkp[q], nnp[q], nnc[q], Uout, Q, DKint, a);
         lds[0][o] = P;
         lds[1][o] = S;
     }

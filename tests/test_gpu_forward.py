@@ -100,7 +100,7 @@ def test_every_layout_variant_matches_oracle32(gpu, n):
     with gpu.Context(p) as ctx:
         ctx.set_fields(fh0, sh0)
         for rv in range(4):
-            for cv in range(3):
+            for cv in range(5):
                 ctx.set_variants(rv, cv)
                 ctx.forward()
                 fh, sh = ctx.get_fields()

@@ -134,13 +134,13 @@ def test_thermal_every_layout_variant_matches_oracle(gpu, n):
     rfh, rsh, rQ = orc.thermal_cwes(p, T, fh0, sh0)
     with gpu.Context(p) as ctx:
         ctx.set_temperature(T)
-        for rv in range(6):
-            ctx.set_variants(rv, 0)
+        for rv, cv in [(r, 0) for r in range(6)] + [(4, c) for c in range(1, 5)]:
+            ctx.set_variants(rv, cv)
             ctx.set_fields(fh0, sh0)
             ctx.forward()
             fh, sh = ctx.get_fields()
             Q = ctx.get_heat_source()
             e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh), float(np.linalg.norm(Q - rQ) / np.linalg.norm(rQ)))
-            print(f"  variant {rv}: relL2 FH {e[0]:.2e} SH {e[1]:.2e} Q {e[2]:.2e}")
-            assert max(e) <= TOL, (rv, e)
+            print(f"  variant {rv},{cv}: relL2 FH {e[0]:.2e} SH {e[1]:.2e} Q {e[2]:.2e}")
+            assert max(e) <= TOL, (rv, cv, e)
         print(f"  tuned variants {ctx.tune(reps=3)}")
