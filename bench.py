@@ -6,9 +6,11 @@ squared update < 1e-6, dk(T) fed back.  A "step" of this benchmark is one THERMA
 z-march of the field pair through the crystal at the converged temperature (nz steps:
 the last one feeds the heat source), inputs resident in HBM.
   value  = SSFM cell-steps/s = nx*ny*nz / time per march
-  roofline = the fused THERMAL z-step (row pass + column pass): 48 algorithmic bytes
-           per cell-step (FH+SH read+write 32, T slice 4, Q slice 4, dk prefix r/w 8),
-           average launch durations from HIP events on the library stream; 8 TB/s peak.
+  roofline = the fused THERMAL z-step (row pass + column pass): 56 algorithmic bytes
+           per cell-step as the metric defines them (FH+SH read+write 32, T slice 4, Q
+           slice 4, phase accumulator read+write 16), over the device time per z-step of
+           the timed marches (HIP events on the library stream); 8 TB/s peak, and the
+           copy bandwidth this GPU reaches (a device copy timed in the same run).
 Also reported (same run): the THERMAL outer loop, Jacobi voxel-sweeps/s, the batched
 sweep of config 3 (256 uneven instances in one hipGraph) against one-at-a-time solves,
 forward+adjoint of config 4 against one forward, and the plain config-2-shape march.
@@ -27,7 +29,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
-B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 48, 80, 12
+# HBM bytes per THERMAL z-step (row + column pass, 256x256) from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE, each scaled by what the counter reads for a copy of known size
+# (profiles/thermal_step_256x256_pmc.md); None until measured.
+TRAFFIC_THERMAL_STEP = None
+B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 
 
 def log(*a):
@@ -74,6 +80,24 @@ def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=15.0):
     return {"value": cells / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle32 -O3 THERMAL march (per-cell n(), own radix-2 FFT, no FFTW on the host) on "
                       f"{p.nx}x{p.ny}, first {k} of {p.nz} z-steps, {dt:.1f} s; host CPU: {cpu_model()}"}
+
+
+def copy_peak_gbps(mib=1024, reps=5):
+    """Read+write bandwidth of a device-to-device copy (the practical HBM ceiling)."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    e1.synchronize()
+    gbps = 2 * x.numel() * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return gbps
 
 
 def bench_thermal(shg, args):
@@ -259,8 +283,9 @@ def main():
     p = th["p"]
     ms_per_step = th["wall"] * 1e3 / th["steps"]
     value = th["cells"] / (th["wall"] / th["steps"])
-    step_s = (th["row_ms"] + th["col_ms"]) * 1e-3
+    step_s = th["dev_ms"] * 1e-3 / p.nz                 # device time per z-step of the timed marches
     achieved = B_THERMAL * p.nx * p.ny / step_s / 1e9
+    copy_gbps = copy_peak_gbps()
     out = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -275,11 +300,12 @@ def main():
                    "thermal_loop": {**th["stats"], "wall_s": th["loop_s"]},
                    "jacobi_voxel_sweeps_per_s": th["vox"] / th["jac_s"],
                    "jacobi_GBps": th["vox"] * B_JACOBI / th["jac_s"] / 1e9},
-        "roofline": {"bound": "hbm", "kernel": "fused THERMAL z-step = k_therm_row_pass + k_col_pass (256x256)",
+        "roofline": {"bound": "hbm", "kernel": "fused THERMAL z-step = k_therm_row_wide + k_col_pass (256x256)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "bytes_per_cell_step": B_THERMAL,
-                     "row_pass_us": th["row_ms"] * 1e3, "col_pass_us": th["col_ms"] * 1e3},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": TRAFFIC_THERMAL_STEP,
+                     "bytes_per_cell_step": B_THERMAL, "step_us": step_s * 1e6,
+                     "row_pass_us": th["row_ms"] * 1e3, "col_pass_us": th["col_ms"] * 1e3,
+                     "measured_copy_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps},
     }
     if not args.no_extra:
         log("plain config-2 shape ...")

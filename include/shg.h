@@ -214,7 +214,7 @@ int shg_batch_run(shg_batch *b);
 /* Schedule of shg_batch_run (no counterpart in the reference): every grid size is cut
  * into groups of at most group_mib MiB of fields, each group marches all its steps
  * before the next one of its stream (its fields stay in the last-level cache), and the
- * groups are spread over `streams` (1..4) concurrent streams.  Defaults: 64 MiB, 2. */
+ * groups are spread over `streams` (1..4) concurrent streams.  Defaults: 128 MiB, 3. */
 int shg_batch_set_schedule(shg_batch *b, int group_mib, int streams);
 
 /* Current fields (after shg_batch_run: the outputs) and the z = 0 fields of one instance. */
@@ -238,8 +238,9 @@ int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
 int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 
 /* Measurement hook (no reference counterpart): average device time of one row pass and
- * one column pass, each launched `reps` times back to back.  Clobbers the current
- * fields; the next shg_forward starts again from the shg_set_fields state. */
+ * one column pass (the tuned layouts), each replayed `reps` times back to back from a
+ * graph.  Clobbers the current fields; the next shg_forward starts again from the
+ * shg_set_fields state. */
 int shg_profile_kernels(shg_ctx *ctx, int reps, double *row_ms, double *col_ms);
 
 #ifdef __cplusplus

@@ -292,7 +292,7 @@ class Batch:
     def run(self):
         _check(self.lib.shg_batch_run(self.h))
 
-    def set_schedule(self, group_mib=64, streams=2):
+    def set_schedule(self, group_mib=128, streams=3):
         """Group size (MiB of fields marched together) and number of concurrent streams."""
         _check(self.lib.shg_batch_set_schedule(self.h, group_mib, streams))
 

@@ -37,7 +37,7 @@ namespace {
     } while (0)
 
 const int kSizes[4] = {64, 128, 256, 512};
-constexpr int kMaxStreams = 4, kDefaultGroupMiB = 64, kDefaultStreams = 2;
+constexpr int kMaxStreams = 4, kDefaultGroupMiB = 128, kDefaultStreams = 3;
 
 int size_class(int n) {
     for (int c = 0; c < 4; c++)

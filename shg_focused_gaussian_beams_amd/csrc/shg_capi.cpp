@@ -568,34 +568,26 @@ int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
 
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
-    float ms = 0.f;
     if (c->p.thermal)
         HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
                                         c->tc.lp, c->tc.ls, c->stream));
-    HIP_TRY(hipEventRecord(c->e0, c->stream));
-    if (c->p.thermal) {
-        const shg::ThermRowArgs a = therm_mid_args(c);
-        const int tv = c->row_v < 0 ? 0 : c->row_v;
-        for (int r = 0; r < reps; r++)
-            HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint, a, c->stream,
-                                                 tv));
-    } else {
-        const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
-        const int rv = c->row_v < 0 ? 0 : c->row_v;
-        for (int r = 0; r < reps; r++)
-            HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv));
-    }
-    HIP_TRY(hipEventRecord(c->e1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->e1));
-    HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
-    *row_ms = ms / reps;
-    HIP_TRY(hipEventRecord(c->e0, c->stream));
-    const int cv = c->col_v < 0 ? 0 : c->col_v;
-    for (int r = 0; r < reps; r++) HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
-    HIP_TRY(hipEventRecord(c->e1, c->stream));
-    HIP_TRY(hipEventSynchronize(c->e1));
-    HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
-    *col_ms = ms / reps;
+    const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+    const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
+    const shg::ThermRowArgs a = c->p.thermal ? therm_mid_args(c) : shg::ThermRowArgs{};
+    auto check = [](hipError_t e) {
+        return e == hipSuccess ? (int)SHG_OK : fail(SHG_E_DEVICE, std::string("profiled pass: ") + hipGetErrorString(e));
+    };
+    float ms = 0.f;
+    RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
+        return check(c->p.thermal ? shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint,
+                                                                 a, c->stream, rv)
+                                  : shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv));
+    }, &ms));
+    *row_ms = ms;
+    RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
+        return check(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
+    }, &ms));
+    *col_ms = ms;
     return SHG_OK;
 }
 

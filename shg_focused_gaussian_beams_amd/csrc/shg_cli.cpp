@@ -178,14 +178,35 @@ int main(int argc, char *argv[]) {
     gaussian_noise(sh, noise, seed);
 
     shg_ctx *ctx = nullptr;
-    if (p.thermal) {
-        std::cerr << "SHG: --thermal=1 is not available in this build\n";
+    if (p.thermal && !save_only_last) {
+        std::cerr << "SHG: THERMAL runs write the output plane only (save_only_last=1)\n";
         return 1;
     }
     if ((rc = shg_create(&p, &ctx))) return die("shg_create", rc);
     if ((rc = shg_set_fields(ctx, fh.data(), sh.data()))) return die("shg_set_fields", rc);
     std::cout << "\n\nEvolution in the crystal...\n" << std::endl;
-    if (save_only_last) {
+    if (p.thermal) {
+        // THERMAL branch of Solver::run (src/headers/Solver.h:289-340): Jacobi, dk(T), the
+        // march and the heat-source feedback, up to max_outer global iterations.
+        shg_thermal_stats st{};
+        if ((rc = shg_thermal_run(ctx, 1, &st))) return die("shg_thermal_run", rc);
+        std::cout << "Global iterations: " << st.outer_iterations << ", Jacobi sweeps: " << st.sweeps_total
+                  << ", last mean squared update: " << st.residual_last << std::endl;
+        if ((rc = shg_get_fields(ctx, fh.data(), sh.data()))) return die("shg_get_fields", rc);
+        if (!save_complex(outdir, "Pump_out", fh, p.nx, p.ny) || !save_complex(outdir, "Signal_out", sh, p.nx, p.ny)) {
+            std::cerr << "SHG: cannot write output files\n";
+            return 1;
+        }
+        if (save_temperature) {
+            const int hz = p.heat_nz > 0 ? p.heat_nz : p.nz;
+            std::vector<float> T(n2 * (size_t)hz);
+            if ((rc = shg_get_temperature(ctx, T.data()))) return die("shg_get_temperature", rc);
+            if (!save_tensor(outdir, "Tfinal", T, p.nx, p.ny, hz)) {
+                std::cerr << "SHG: cannot write temperature files\n";
+                return 1;
+            }
+        }
+    } else if (save_only_last) {
         if ((rc = shg_forward(ctx))) return die("shg_forward", rc);
         if ((rc = shg_get_fields(ctx, fh.data(), sh.data()))) return die("shg_get_fields", rc);
         if (!save_complex(outdir, "Pump_out", fh, p.nx, p.ny) || !save_complex(outdir, "Signal_out", sh, p.nx, p.ny)) {
@@ -207,8 +228,6 @@ int main(int argc, char *argv[]) {
     if ((rc = shg_powers(ctx, &pin, &pout, &eta))) return die("shg_powers", rc);
     std::cout << std::setprecision(10) << "P_FH(z=0) = " << pin << " W, P_SH(out) = " << pout
               << " W, efficiency = " << eta << std::endl;
-    (void)save_temperature;
-    (void)save_tensor;
     shg_destroy(ctx);
     std::cout << "Deallocating memory" << std::endl;
 
