@@ -239,7 +239,7 @@ int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass (the tuned layouts), each replayed `reps` times back to back from a
- * graph.  Clobbers the current fields; the next shg_forward starts again from the
+ * graph (the fastest of three replays).  Clobbers the current fields; the next shg_forward starts again from the
  * shg_set_fields state. */
 int shg_profile_kernels(shg_ctx *ctx, int reps, double *row_ms, double *col_ms);
 

@@ -75,6 +75,6 @@ double shg_internal_power_of(const shg_ctx *c, int which, double sum, double tem
 // events) into an executable graph.
 int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out);
 // Average time in ms of `one` (a sequence of launches on `st`), replayed from a graph of
-// `reps` repetitions after one warm-up replay.
+// `reps` repetitions: the fastest of three replays after a warm-up replay.
 int shg_internal_time_graph(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, const std::function<int()> &one,
                             float *ms);
