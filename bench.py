@@ -30,9 +30,9 @@ sys.path.insert(0, ROOT)
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256) from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE, each scaled by what the counter reads for a copy of known size
-# (profiles/thermal_step_256x256_pmc.md); None until measured.
-TRAFFIC_THERMAL_STEP = None
+# (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy of known size
+# (profiles/thermal_step_256x256_pmc.md: 9066 KiB with the narrow column layout).
+TRAFFIC_THERMAL_STEP = 9066 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 
 

@@ -181,13 +181,23 @@ typedef struct shg_adjoint_result {
     int32_t segments;
 } shg_adjoint_result;
 
-/* Discrete adjoint of the plain z-march (Solver::CWES, src/headers/Solver.h:245-276) for
+/* Discrete adjoint of the z-march (Solver::CWES, src/headers/Solver.h:245-276) for
  * J = eta.  Checkpoints every `checkpoint_interval` steps (<= 0: ceil(sqrt(nz-1))), with
  * per-segment recomputation.  d_eta_d_fh (nx*ny complex64, may be null) receives the
  * gradient with respect to the z = 0 pump field, dJ = Re sum conj(g) dA.  The focus and
  * the pump part of the temperature gradient assume the z = 0 pump is setPump at the
- * context's power, waist, focus and temp (plus additive noise). */
+ * context's power, waist, focus and temperature (plus additive noise).
+ * On a THERMAL context the march runs at the context's current temperature field, which
+ * is held fixed (the heat source is not updated); the temperature derivatives are those
+ * of a uniform shift of that field (the propagators, powers and pump at the grid-centre
+ * temperature of slice 0, as the forward evaluates them). */
 int shg_adjoint(shg_ctx *ctx, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res);
+
+/* After shg_adjoint on a THERMAL context (no reference counterpart): d eta / d T of every
+ * heat voxel (nx*ny*heat_nz doubles, IDX layout) with the rest of the field held fixed,
+ * crystal path only (coupling, phase mismatch through DKint, TPA; the propagator and
+ * power terms sit on the grid-centre voxel of slice 0).  Sums to d_eta_d_temp_crystal. */
+int shg_adjoint_temperature_map(shg_ctx *ctx, double *map);
 
 /* ----- Batched sweeps ---------------------------------------------------------------
  * Replace the reference's sweep driver, which runs one `./SHG` process per parameter

@@ -607,37 +607,48 @@ void oracle_set_q(const shg_params *p, const float *Tslice, const double *fh, co
 // 41-54) at a given temperature T (hz slices): the fields of the z = 0 plane in fh/sh are
 // replaced by the Pump_out/Signal_out state (before the last step); Q (hz slices, in/out)
 // receives the heat source of every step.  Propagators at T of the grid centre, slice 0.
-void oracle_thermal_cwes(const shg_params *p, const float *T, double *fh, double *sh, float *Q) {
-    State<float> st(*p);
-    const K<float> &k = st.k;
+extern "C++" {
+template <class R> void thermal_cwes(const shg_params *p, const float *T, double *fh, double *sh, float *Q) {
+    State<R> st(*p);
+    const K<R> &k = st.k;
     const int NX = k.NX, NY = k.NY, NZ = k.NZ;
     const int HZ = p->heat_nz > 0 ? p->heat_nz : NZ;
     const size_t n2 = (size_t)NX * NY, n3 = n2 * HZ;
     st.Tf.assign(T, T + n3);
     st.Q.assign(Q, Q + n3);
-    st.DK.assign(n2 * NZ, 0.f);
-    st.DKint.assign(n2, 0.f);
+    st.DK.assign(n2 * NZ, (R)0);
+    st.DKint.assign(n2, (R)0);
     for (int s = 0; s < NZ; s++) {
         const int hs = heat_slice(s, NZ, HZ);
         for (size_t i = 0; i < n2; i++) {
-            const float Tv = st.Tf[(size_t)hs * n2 + i];
+            const R Tv = st.Tf[(size_t)hs * n2 + i];
             st.DK[(size_t)s * n2 + i] = 2 * k.PI * (2 * k.n(k.lp, Tv) / k.lp - k.n(k.ls, Tv) / k.ls + 1 / (k.Lambda));
         }
     }
-    const float Tc = st.Tf[(size_t)(NY / 2) * NX + NX / 2];
-    set_propagators<float>(k, Tc, st.Hp.data(), st.Hs.data());
+    const R Tc = st.Tf[(size_t)(NY / 2) * NX + NX / 2];
+    set_propagators<R>(k, Tc, st.Hp.data(), st.Hs.data());
     fields_in(st.P, fh); fields_in(st.S, sh);
-    std::vector<cx<float>> outP, outS;
-    float z = 0.f;
+    std::vector<cx<R>> outP, outS;
+    R z = 0;
     for (int s = 0; s < NZ; s++) {
         if (s == NZ - 1) { outP = st.P; outS = st.S; }
         const int hs = heat_slice(s, NZ, HZ);
-        ssfm<float>(st, z, 0.f, p->scheme, true, s, hs);
-        set_q<float>(st, hs);
+        ssfm<R>(st, z, (R)0, p->scheme, true, s, hs);
+        set_q<R>(st, hs);
         z += k.dz;
     }
     fields_out(outP, fh); fields_out(outS, sh);
-    std::memcpy(Q, st.Q.data(), n3 * sizeof(float));
+    for (size_t i = 0; i < n3; i++) Q[i] = (float)st.Q[i];
+}
+}  // extern "C++"
+
+void oracle_thermal_cwes(const shg_params *p, const float *T, double *fh, double *sh, float *Q) {
+    thermal_cwes<float>(p, T, fh, sh, Q);
+}
+
+// The same march in FP64 throughout (temperatures taken from the float field).
+void oracle_thermal_cwes64(const shg_params *p, const float *T, double *fh, double *sh, float *Q) {
+    thermal_cwes<double>(p, T, fh, sh, Q);
 }
 
 // THERMAL Solver::run (src/headers/Solver.h:289-340) in float.

@@ -85,6 +85,7 @@ def load_library(path=LIB_PATH):
         "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_set_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_adjoint": (ctypes.c_int, [vp, ctypes.c_int, fp, ctypes.POINTER(AdjointResult)]),
+        "shg_adjoint_temperature_map": (ctypes.c_int, [vp, dp]),
         "shg_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
@@ -114,6 +115,10 @@ def _check(rc):
 
 def _fptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
 def make_pump(params, t_centre=None):
@@ -205,6 +210,12 @@ class Context:
         g = np.empty((self.params.ny, self.params.nx), np.complex64) if want_field else None
         _check(self.lib.shg_adjoint(self.h, checkpoint_interval, _fptr(g) if want_field else None, ctypes.byref(r)))
         return r.as_dict(), g
+
+    def adjoint_temperature_map(self):
+        """After adjoint() on a THERMAL context: d eta / d T per heat voxel (heat_shape())."""
+        m = np.empty(self.heat_shape(), np.float64)
+        _check(self.lib.shg_adjoint_temperature_map(self.h, _dptr(m)))
+        return m
 
     # THERMAL
     def heat_shape(self):

@@ -19,73 +19,11 @@
 
 #include "adjoint_kernels.h"
 #include "fft_lds.h"
+#include "adjoint_cell.h"
 #include "nl_step.h"
 #include "pass_cfg.h"
 
 namespace shg {
-
-__device__ __forceinline__ float2 cj(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
-__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 axpy2(float s, float2 a, float2 b) { return make_float2(b.x + s * a.x, b.y + s * a.y); }
-__device__ __forceinline__ float2 scl2(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
-
-struct AdjAcc {
-    double c1r, c1i, c2r, c2i, tpa;
-};
-
-__device__ __forceinline__ void acc_zero(AdjAcc &a) { a.c1r = a.c1i = a.c2r = a.c2i = a.tpa = 0.0; }
-
-// Adjoint of the right-hand side f(P, S) at (P, S) applied to (gP, gS).
-__device__ __forceinline__ void rhs_adj(float2 P, float2 S, float2 gP, float2 gS, float4 c, const NlConst &k,
-                                        float2 &Pb, float2 &Sb, AdjAcc &acc) {
-    const float2 c1 = f2(c.x, c.y), c2 = f2(c.z, c.w);
-    const float s2 = S.x * S.x + S.y * S.y;
-    // Pb = -lossP gP + c1 S conj(gP) + 2 conj(c2) conj(P) gS
-    Pb = add2(add2(scl2(-k.lossP, gP), c_mul(c_mul(c1, S), cj(gP))), scl2(2.0f, c_mul(c_mul(cj(c2), cj(P)), gS)));
-    // Sb = conj(c1) P gP - (lossS + 2 tpa |S|^2) gS - tpa S^2 conj(gS)
-    Sb = add2(add2(c_mul(c_mul(cj(c1), P), gP), scl2(-(k.lossS + 2.0f * k.tpa * s2), gS)),
-              scl2(-k.tpa, c_mul(c_mul(S, S), cj(gS))));
-    // coefficient sensitivities: c1b += conj(S conj P) gP, c2b += conj(P^2) gS, tpab += -|S|^2 Re(conj(gS) S)
-    const float2 u = c_mul(cj(S), P);
-    const float2 w = c_mul(cj(c_mul(P, P)), gS);
-    const float2 v = c_mul(u, gP);
-    acc.c1r += (double)v.x; acc.c1i += (double)v.y;
-    acc.c2r += (double)w.x; acc.c2i += (double)w.y;
-    acc.tpa += -(double)s2 * ((double)gS.x * S.x + (double)gS.y * S.y);
-}
-
-// RK4^T at forward input (P, S): (lP, lS) is the adjoint of the output on entry and of the
-// input on exit.
-__device__ __forceinline__ void rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, float4 c, const NlConst &k,
-                                        AdjAcc &acc) {
-    float2 k1p, k1s, k2p, k2s, k3p, k3s;
-    nl_rhs(P, S, c, k.lossP, k.lossS, k.tpa, k1p, k1s);
-    const float2 y2p = axpy2(0.5f, k1p, P), y2s = axpy2(0.5f, k1s, S);
-    nl_rhs(y2p, y2s, c, k.lossP, k.lossS, k.tpa, k2p, k2s);
-    const float2 y3p = axpy2(0.5f, k2p, P), y3s = axpy2(0.5f, k2s, S);
-    nl_rhs(y3p, y3s, c, k.lossP, k.lossS, k.tpa, k3p, k3s);
-    const float2 y4p = add2(P, k3p), y4s = add2(S, k3s);
-    const float h6 = k.h6;
-    float2 ybp = lP, ybs = lS;
-    float2 kb1p = scl2(h6, lP), kb1s = scl2(h6, lS);
-    float2 kb2p = scl2(2.0f * h6, lP), kb2s = scl2(2.0f * h6, lS);
-    float2 kb3p = kb2p, kb3s = kb2s;
-    const float2 kb4p = kb1p, kb4s = kb1s;
-    float2 rp, rs;
-    rhs_adj(y4p, y4s, kb4p, kb4s, c, k, rp, rs, acc);
-    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
-    kb3p = add2(kb3p, rp); kb3s = add2(kb3s, rs);
-    rhs_adj(y3p, y3s, kb3p, kb3s, c, k, rp, rs, acc);
-    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
-    kb2p = axpy2(0.5f, rp, kb2p); kb2s = axpy2(0.5f, rs, kb2s);
-    rhs_adj(y2p, y2s, kb2p, kb2s, c, k, rp, rs, acc);
-    ybp = add2(ybp, rp); ybs = add2(ybs, rs);
-    kb1p = axpy2(0.5f, rp, kb1p); kb1s = axpy2(0.5f, rs, kb1s);
-    rhs_adj(P, S, kb1p, kb1s, c, k, rp, rs, acc);
-    lP = add2(ybp, rp);
-    lS = add2(ybs, rs);
-}
 
 // Sum of the five accumulators over the workgroup (wave shuffles, then LDS across the
 // waves); thread 0 writes them to `out`.  All threads of the group must call it.
@@ -121,15 +59,7 @@ template <> struct AdjLayout<128> { using Row = RowCfgT<128, 4, 1>; using Col = 
 template <> struct AdjLayout<256> { using Row = RowCfgT<256, 4, 1>; using Col = ColCfgT<256, 4, 8>; };
 template <> struct AdjLayout<512> { using Row = RowCfgT<512, 4, 2>; using Col = ColCfgT<512, 4, 4>; };
 
-// The adjoint row pass is wide: one row per workgroup of W waves; the first N/E lanes
-// (at most one wave) run the transforms and every thread steps N/(64 W) cells of the row
-// between them, so the recomputed RK4 stages and their transposes spread over W times as
-// many wavefronts as the transforms need.
-template <int N> struct AdjWide;
-template <> struct AdjWide<64>  { static constexpr int E = 4, W = 1; };
-template <> struct AdjWide<128> { static constexpr int E = 4, W = 2; };
-template <> struct AdjWide<256> { static constexpr int E = 4, W = 4; };
-template <> struct AdjWide<512> { static constexpr int E = 8, W = 8; };
+
 
 // Forward row pass with tape (single instance, modes as RowMode without AUTO).
 // START at s0: ckpt = input (if ckpt is set), then RK4_s0, R_bwd.

@@ -29,6 +29,16 @@ struct AdjointBuffers {
     hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;  // captured forward / adjoint for interval k
     int g_row_v = -1, g_col_v = -1;            // plain-pass layouts the forward graph was captured with
     std::vector<std::complex<double>> pump_df, pump_dn;   // d(pump)/d(focus), d(pump)/d(n_p) per cell
+    double pump_T = -1e300;                    // temperature the pump derivative fields were built at
+    // THERMAL (temperature held fixed)
+    double *Ap = nullptr, *As = nullptr;       // per heat voxel dJ/dn_p, dJ/dn_s; Ap then holds dJ/dT
+    double *Gc = nullptr, *Dc = nullptr;       // per cell DKint suffix sum and pending part
+    double *dpart = nullptr, *dsum = nullptr;  // block sums of the dJ/dT map and their total
+    float *dkt = nullptr;                      // two tapes of DKint per step (K steps each)
+    float *dkc = nullptr;                      // DKint at every checkpoint (step s0 - 1)
+    int th_k = 0, th_nseg = 0;                 // interval and segment count the THERMAL buffers hold
+    bool map_valid = false;
+    double map_centre_extra = 0.0;             // propagator and power terms, at the grid-centre voxel
 };
 
 struct shg_ctx {
@@ -69,6 +79,7 @@ struct shg_ctx {
 // Shared helpers of the library's translation units (defined in shg_capi.cpp).
 int shg_internal_fail(int code, const std::string &msg);
 int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count);
+shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s);
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
 double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);
 // Captures the work `enqueue` issues on `st` (and on streams forked from it through

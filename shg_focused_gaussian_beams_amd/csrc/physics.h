@@ -7,6 +7,8 @@
 // (src/headers/PhaseMatching.h:63), so reproducing its rounding matters at the 0.06 rad
 // level over 30 mm: every value below is produced with the same float/double typing.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -34,21 +36,22 @@ inline float index_spplt(float lambda, float temp) {
 // MgO:PPLN (5 mol% MgO:CLN) extraordinary index, Gayer et al., Appl. Phys. B 91, 343
 // (2008): n^2 = a1 + b1 f + (a2 + b2 f)/(l^2 - (a3 + b3 f)^2) + (a4 + b4 f)/(l^2 - a5^2)
 // - a6 l^2, f = (T - 24.5)(T + 570.82).  Not in the reference; double throughout.
-inline double index_ppln(double lambda, double temp) {
+__host__ __device__ inline double index_ppln(double lambda, double temp) {
     const double f = (temp - 24.5) * (temp + 570.82);
     const double l2 = lambda * lambda, p = 0.2020 + 6.113e-8 * f;
     return std::sqrt(5.756 + 2.860e-6 * f + (0.0983 + 4.700e-8 * f) / (l2 - p * p) +
                      (189.32 + 1.516e-4 * f) / (l2 - 12.52 * 12.52) - 1.32e-2 * l2);
 }
 
-// Double-precision index for the FP64 power reductions (same formulas, no float rounding).
-inline double index_spplt_d(double lambda, double temp) {
+// Double-precision index for the FP64 power reductions and the temperature derivatives
+// of the adjoint (same formulas, no float rounding; also compiled for the device).
+__host__ __device__ inline double index_spplt_d(double lambda, double temp) {
     const double tk = temp + 273.15, b = 3.483933e-8 * tk * tk, c = 1.607839e-8 * tk * tk;
     const double l2 = lambda * lambda, cc = 0.185087 + c;
     return std::sqrt(4.502483 + (0.007294 + b) / (l2 - cc * cc) + 0.073423 / (l2 - 0.199595 * 0.199595) +
                      0.001 / (l2 - 7.99724 * 7.99724) - 0.02357 * l2);
 }
-inline double index_d(int crystal, double lambda, double temp) {
+__host__ __device__ inline double index_d(int crystal, double lambda, double temp) {
     return crystal == SHG_CRYSTAL_PPLN ? index_ppln(lambda, temp) : index_spplt_d(lambda, temp);
 }
 

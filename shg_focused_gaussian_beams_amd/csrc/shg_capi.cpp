@@ -183,6 +183,7 @@ shg::ThermStep therm_step(const shg_ctx *c, int s) {
     for (int i = 0; i < 3; i++) st.zfac[i] = (double)(k.dz * incs[i]) + (s + 1.0) * (double)k.dz;
     st.sp1 = s + 1.0;
     st.slice = shg::phys::heat_slice(s, c->p.nz, c->hz);
+    st.step = s;
     return st;
 }
 
@@ -239,7 +240,9 @@ void release(shg_ctx *c) {
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
                       (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
-                      (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam})
+                      (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam, (void *)c->adj.Ap, (void *)c->adj.As,
+                      (void *)c->adj.Gc, (void *)c->adj.Dc, (void *)c->adj.dpart, (void *)c->adj.dsum,
+                      (void *)c->adj.dkt, (void *)c->adj.dkc})
         if (ptr) (void)hipFree(ptr);
     for (hipGraphExec_t g : {c->adj.g_fwd, c->adj.g_adj})
         if (g) (void)hipGraphExecDestroy(g);
@@ -335,6 +338,7 @@ int timed_march(shg_ctx *c) {
 }  // namespace
 
 int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count) { return enqueue_march_plain(c, first, count); }
+shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s) { return therm_step(c, s); }
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh) {
     return reduce_powers(c, U, sum_fh, sum_sh);
 }

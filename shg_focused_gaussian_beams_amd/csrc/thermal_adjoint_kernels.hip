@@ -1,0 +1,216 @@
+// thermal_adjoint_kernels.hip — discrete adjoint of the THERMAL z-march (src/headers/
+// Solver.h:44-63,193-233,245-276) with the temperature field held fixed.
+//
+// The THERMAL nonlinear step differs from the plain one per cell: n_p, n_s come from the
+// cell's heat voxel, the phase phi_k = DKint_s * zfac_k / (s + 1) differs between the RK4
+// stage groups (1 | 2-3 | 4), and DKint_s is the running sum of dk over the executed
+// steps.  The transpose of each RK4 keeps the three stage groups' coefficient
+// sensitivities apart and turns them, per cell, into
+//   dJ/dkappa_p, dJ/dkappa_s, dJ/dtpa  -> dJ/dn_p, dJ/dn_s of the step's voxel,
+//   dJ/dphi_k                          -> dJ/dDKint_s.
+// Since DKint_s = sum_{s' <= s} dk(slice(s')), the dk of step s' sees G_s' = sum over
+// s >= s' of dJ/dDKint_s: the adjoint runs steps last first, so G is a running sum per
+// cell and every step adds G (d dk / d n) to its voxel.
+#include <hip/hip_runtime.h>
+
+#include "../../include/shg.h"
+#include "adjoint_cell.h"
+#include "adjoint_kernels.h"
+#include "fft_lds.h"
+#include "physics.h"
+#include "thermal_adjoint_kernels.h"
+#include "thermal_cell.h"
+
+namespace shg {
+
+// RK4^T of one THERMAL RK4 application (rk4_thermal) at forward input (P, S): updates the
+// adjoint (lP, lS) and returns the cell's sensitivities to DKint (dDK) and, through the
+// coupling constants and the TPA coefficient, to its indices (dnp, dns).
+__device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, const CellPhys &c,
+                                              float dkint, const ThermStep &st, const ThermConst &k, double &dDK,
+                                              double &dnp, double &dns) {
+    float sn[3], cs[3];
+    float4 cf[3];
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        cf[g] = phase_coef(c, dkint, st.zfac[g], st.sp1);   // (kp sin, kp cos, -ks sin, ks cos)
+        const float phi = (float)(((double)dkint * st.zfac[g]) / st.sp1);
+        sincosf(phi, &sn[g], &cs[g]);
+    }
+    AdjAcc acc[3];
+#pragma unroll
+    for (int g = 0; g < 3; g++) acc_zero(acc[g]);
+    rk4_adj3(P, S, lP, lS, cf[0], cf[1], cf[2], k.lossP, k.lossS, c.tpa, k.h6, acc[0], acc[1], acc[2]);
+    double dkp = 0.0, dks = 0.0, dphi_dk = 0.0;
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        // c1 = i kp e^{-i phi}: dc1/dkp = (sin, cos), dc1/dphi = -i c1;
+        // c2 = i ks e^{+i phi}: dc2/dks = (-sin, cos), dc2/dphi = +i c2.
+        dkp += acc[g].c1r * sn[g] + acc[g].c1i * cs[g];
+        dks += -acc[g].c2r * sn[g] + acc[g].c2i * cs[g];
+        const double dphi = acc[g].c1r * cf[g].y - acc[g].c1i * cf[g].x - acc[g].c2r * cf[g].w + acc[g].c2i * cf[g].z;
+        dphi_dk += dphi * st.zfac[g] / st.sp1;
+    }
+    dDK += dphi_dk;
+    // kp = pi dQ / (n_p lp), ks = pi dQ / (2 n_s ls), tpa = tpa_base n_s / 2
+    dnp += dkp * (-(double)c.kp / (double)c.np);
+    dns += dks * (-(double)c.ks / (double)c.ns) + (acc[0].tpa + acc[1].tpa + acc[2].tpa) * 0.5 * k.tpa_base;
+}
+
+template <int N>
+__global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const InstDesc *__restrict__ descs,
+                                                                     const float2 *__restrict__ twg, int mode,
+                                                                     ThermAdjArgs a) {
+    constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    static_assert(T <= 64 && N == THREADS, "the transforms run in one wavefront; one cell per thread");
+    __shared__ float2 lds[2][L];
+    __shared__ float2 tw[N];
+    float4 *U = descs[0].U;
+    const int tid = threadIdx.x;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + tid;
+    const int ls = a.cur.step - a.s0;
+    // the cell's inputs, loaded before the transform hides their latency
+    const float2 nn_c = a.NT[(size_t)a.cur.slice * n2 + idx];
+    float2 nn_p = make_float2(0.f, 0.f);
+    float4 A;
+    float dk_c, dk_p = 0.f;
+    if (mode == ADJ_START) {
+        A = a.a2[(size_t)ls * n2 + idx];
+        dk_c = a.dk[(size_t)ls * n2 + idx];
+    } else if (mode == ADJ_END) {
+        A = a.a0[idx];
+        dk_c = a.dk[(size_t)ls * n2 + idx];
+    } else {
+        A = a.a2[(size_t)(ls - 1) * n2 + idx];
+        dk_p = a.dk[(size_t)(ls - 1) * n2 + idx];
+        dk_c = a.dk[(size_t)ls * n2 + idx];
+        nn_p = a.NT[(size_t)a.prev.slice * n2 + idx];
+    }
+    double G = a.G[idx], Dp = a.Dp[idx];
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    __syncthreads();
+    auto sync = [] { wave_sync(); };
+    if (tid < T) {
+        float2 fa[E], fb[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const float4 v = U[base + tid + j * T];
+            fa[j] = make_float2(v.x, v.y);
+            fb[j] = make_float2(v.z, v.w);
+        }
+        if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            lds[0][lds_pad(tid + j * T)] = fa[j];
+            lds[1][lds_pad(tid + j * T)] = fb[j];
+        }
+    }
+    __syncthreads();
+    {
+        const ThermConst &k = a.k;
+        // d dk / d n_p = 4 pi / lp, d dk / d n_s = -2 pi / ls (src/headers/PhaseMatching.h:47)
+        const double wp = 4.0 * (double)k.pi / (double)k.lp, ws = -2.0 * (double)k.pi / (double)k.ls;
+        const int o = lds_pad(tid);
+        float2 P = lds[0][o], S = lds[1][o];
+        double *Apc = a.Ap + (size_t)a.cur.slice * n2 + idx, *Asc = a.As + (size_t)a.cur.slice * n2 + idx;
+        const CellPhys cc = cell_phys(k, nn_c);
+        if (mode == ADJ_START) {
+            double dDK = 0.0, dnp = 0.0, dns = 0.0;
+            therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), P, S, cc, dk_c, a.cur, k, dDK, dnp, dns);
+            *Apc += dnp;
+            *Asc += dns;
+            Dp = dDK;
+        } else {
+            float2 Ap_ = f2(A.x, A.y), As_ = f2(A.z, A.w);
+            CellPhys cp{};
+            if (mode == ADJ_MID) {
+                cp = cell_phys(k, nn_p);
+                rk4_thermal(Ap_, As_, cp, dk_p, a.prev, k);          // A_s from A''_{s-1}
+            }
+            double dDK = 0.0, dnp = 0.0, dns = 0.0;
+            therm_rk4_adj(Ap_, As_, P, S, cc, dk_c, a.cur, k, dDK, dnp, dns);
+            G += Dp + dDK;                                          // dJ/dDKint_s complete
+            *Apc += dnp + G * wp;
+            *Asc += dns + G * ws;
+            Dp = 0.0;
+            if (mode == ADJ_MID) {
+                double dDK2 = 0.0, dnp2 = 0.0, dns2 = 0.0;
+                therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), P, S, cp, dk_p, a.prev, k, dDK2, dnp2, dns2);
+                a.Ap[(size_t)a.prev.slice * n2 + idx] += dnp2;
+                a.As[(size_t)a.prev.slice * n2 + idx] += dns2;
+                Dp = dDK2;
+            }
+        }
+        lds[0][o] = P;
+        lds[1][o] = S;
+        a.G[idx] = G;
+        a.Dp[idx] = Dp;
+    }
+    __syncthreads();
+    if (tid < T) {
+        float2 fa[E], fb[E];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            fa[j] = lds[0][lds_pad(tid + j * T)];
+            fb[j] = lds[1][lds_pad(tid + j * T)];
+        }
+        if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
+#pragma unroll
+        for (int j = 0; j < E; j++) U[base + tid + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(double *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_therm_dtmap(double *__restrict__ Ap, const double *__restrict__ As,
+                                                     const float *__restrict__ T, int64_t n3, int crystal, double lp,
+                                                     double ls, double *__restrict__ part) {
+    const double h = 1e-3;
+    double sum = 0.0;
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n3; v += (int64_t)gridDim.x * 256) {
+        const double t = (double)T[v];
+        const double dnp = (phys::index_d(crystal, lp, t + h) - phys::index_d(crystal, lp, t - h)) / (2 * h);
+        const double dns = (phys::index_d(crystal, ls, t + h) - phys::index_d(crystal, ls, t - h)) / (2 * h);
+        const double m = Ap[v] * dnp + As[v] * dns;
+        Ap[v] = m;
+        sum += m;
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_therm_adj_row(int n, const InstDesc *d, const float2 *tw, int mode, const ThermAdjArgs &a,
+                                hipStream_t st) {
+    if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
+    switch (n) {
+        case 64: hipLaunchKernelGGL(k_therm_adj_row<64>, dim3(64), dim3(64 * AdjWide<64>::W), 0, st, d, tw, mode, a); break;
+        case 128: hipLaunchKernelGGL(k_therm_adj_row<128>, dim3(128), dim3(64 * AdjWide<128>::W), 0, st, d, tw, mode, a); break;
+        case 256: hipLaunchKernelGGL(k_therm_adj_row<256>, dim3(256), dim3(64 * AdjWide<256>::W), 0, st, d, tw, mode, a); break;
+        case 512: hipLaunchKernelGGL(k_therm_adj_row<512>, dim3(512), dim3(64 * AdjWide<512>::W), 0, st, d, tw, mode, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double *p, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_therm_dtmap(double *Ap, const double *As, const float *T, int64_t n3, int crystal, double lp,
+                              double ls, double *part, int nparts, hipStream_t st) {
+    if (n3 <= 0 || nparts <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_therm_dtmap, dim3(nparts), dim3(256), 0, st, Ap, As, T, n3, crystal, lp, ls, part);
+    return hipGetLastError();
+}
+
+}  // namespace shg

@@ -36,16 +36,24 @@ struct ThermStep {
     double zfac[3];
     double sp1;                                   // s + 1
     int slice;                                    // heat slice of step s
+    int step;                                     // s
 };
 
 enum ThermRowFlags : int {
-    TROW_FWD = 1, TROW_NPREV = 2, TROW_QPREV = 4, TROW_SNAP = 8, TROW_NCUR = 16, TROW_BWD = 32, TROW_FIRST = 64
+    TROW_FWD = 1, TROW_NPREV = 2, TROW_QPREV = 4, TROW_SNAP = 8, TROW_NCUR = 16, TROW_BWD = 32, TROW_FIRST = 64,
+    TROW_TAPE = 128
 };
 
+// With TROW_TAPE the pass also records, for the adjoint, the fields before the second RK4
+// of step prev.step into tape_a2[prev.step - tape_s0] and DKint of step cur.step into
+// tape_dk[cur.step - tape_s0] (n*n entries per step).
 struct ThermRowArgs {
     ThermStep prev, cur;
     ThermConst k;
     int flags;
+    int tape_s0;
+    float4 *tape_a2;
+    float *tape_dk;
 };
 
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,

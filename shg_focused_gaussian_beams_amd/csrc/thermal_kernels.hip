@@ -11,6 +11,7 @@
 
 #include "fft_lds.h"
 #include "nl_step.h"
+#include "thermal_cell.h"
 #include "pass_cfg.h"
 #include "thermal_kernels.h"
 #include "../../include/shg.h"
@@ -91,33 +92,6 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
-// ---------------------------------------------------------------------------
-// Refractive index with the reference's typing (src/headers/Crystal.h:55-71).
-__device__ __forceinline__ float index_dev(int crystal, float L, float T) {
-#pragma clang fp contract(off)
-    if (crystal == SHG_CRYSTAL_PPLN) {
-        const double t = (double)T, l2 = (double)L * (double)L;
-        const double f = (t - 24.5) * (t + 570.82);
-        const double p = 0.2020 + 6.113e-8 * f;
-        return (float)sqrt(5.756 + 2.860e-6 * f + (0.0983 + 4.700e-8 * f) / (l2 - p * p) +
-                           (189.32 + 1.516e-4 * f) / (l2 - 12.52 * 12.52) - 1.32e-2 * l2);
-    }
-    const float A = 4.502483f, B = 0.007294f, C = 0.185087f, D = -0.02357f;
-    const float E = 0.073423f, F = 0.199595f, G = 0.001f, H = 7.99724f;
-    const double tk = (double)T + 273.15;
-    const float b = (float)(3.483933e-8 * (tk * tk));
-    const float c = (float)(1.607839e-8 * (tk * tk));
-    const double l2 = (double)L * (double)L;
-    const float bb = B + b, cc = C + c;
-    const double s = (double)A + (double)bb / (l2 - (double)cc * (double)cc) + (double)E / (l2 - (double)F * (double)F) +
-                     (double)G / (l2 - (double)H * (double)H) + (double)D * l2;
-    return (float)sqrt(s);
-}
-
-struct CellPhys {
-    float np, ns, kp, ks, dk, tpa;
-};
-
 // Refractive indices of every heat voxel, {n_p(T), n_s(T)}: the temperature is frozen
 // during a march, so the FP64 Sellmeier evaluation runs once per voxel instead of twice
 // per cell per row pass (the values are the same floats).
@@ -127,48 +101,6 @@ __global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T
     if (i >= n) return;
     const float t = T[i];
     tab[i] = make_float2(index_dev(crystal, lp, t), index_dev(crystal, ls, t));
-}
-
-// The coupling magnitudes, the phase mismatch (src/headers/PhaseMatching.h:47) and the
-// TPA coefficient from the cell's indices, with the reference's float operations.
-__device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float2 nn) {
-#pragma clang fp contract(off)
-    CellPhys c;
-    c.np = nn.x;
-    c.ns = nn.y;
-    c.kp = ((1.0f * k.pi) * k.dq) / (c.np * k.lp);
-    c.ks = ((0.5f * k.pi) * k.dq) / (c.ns * k.ls);
-    c.dk = (2.0f * k.pi) * (((2.0f * c.np) / k.lp - c.ns / k.ls) + k.inv_period);
-    c.tpa = (float)(0.5 * (k.tpa_base * (double)c.ns));
-    return c;
-}
-
-// (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}), phi = (float)(DKint * zfac / (s+1)).
-__device__ __forceinline__ float4 phase_coef(const CellPhys &c, float dkint, double zfac, double sp1) {
-    const float phi = (float)(((double)dkint * zfac) / sp1);
-    float sn, cs;
-    sincosf(phi, &sn, &cs);
-    return make_float4(c.kp * sn, c.kp * cs, -c.ks * sn, c.ks * cs);
-}
-
-// THERMAL solverRK4 (src/headers/Solver.h:197-213): stage increments 0, .25, .25, .5.
-__device__ __forceinline__ void rk4_thermal(float2 &P, float2 &S, const CellPhys &c, float dkint, const ThermStep &st,
-                                            const ThermConst &k) {
-    const float4 ca = phase_coef(c, dkint, st.zfac[0], st.sp1);
-    const float4 cb = phase_coef(c, dkint, st.zfac[1], st.sp1);
-    const float4 cc = phase_coef(c, dkint, st.zfac[2], st.sp1);
-    rk4_cell(P, S, ca, cb, cc, k.lossP, k.lossS, c.tpa, k.h6);
-}
-
-// setQ (src/headers/Tfield.h:277-290): Q = ap Ip + as Is + bs Is^2 with I_j = c eps0 n_j |A_j|^2 / 2.
-__device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys &c, float2 P, float2 S) {
-#pragma clang fp contract(off)
-    const float p2 = P.x * P.x + P.y * P.y, s2 = S.x * S.x + S.y * S.y;
-    const double ip = k.q_p * (double)c.np * (double)p2;
-    const double is = k.q_s * (double)c.ns * (double)s2;
-    const float i2 = (float)(k.q_i * (double)c.ns * (double)s2);
-    const float tpa = k.beta_s * (i2 * i2);
-    return (float)(ip + is + (double)tpa);
 }
 
 // Row pass of the THERMAL z-march (flags in ThermRowFlags):
@@ -184,12 +116,14 @@ __device__ __forceinline__ float heat_source(const ThermConst &k, const CellPhys
 //
 // One cell of the pass between the transforms (idx: cell index in the plane), from its
 // already loaded phase accumulator and index-table entries.
-__device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx, size_t sprev, float dkprev, float2 nnp,
-                                               float2 nnc, float4 *__restrict__ Uout, float *__restrict__ Q,
+__device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx, size_t n2, size_t sprev, float dkprev,
+                                               float2 nnp, float2 nnc, float4 *__restrict__ Uout, float *__restrict__ Q,
                                                float *__restrict__ DKint, const ThermRowArgs &a) {
     const ThermConst &k = a.k;
     if (a.flags & (TROW_NPREV | TROW_QPREV)) {
         const CellPhys cp = cell_phys(k, nnp);
+        if ((a.flags & (TROW_TAPE | TROW_NPREV)) == (TROW_TAPE | TROW_NPREV))
+            a.tape_a2[(size_t)(a.prev.step - a.tape_s0) * n2 + idx] = make_float4(P.x, P.y, S.x, S.y);
         if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
         if (a.flags & TROW_QPREV) Q[sprev + idx] = heat_source(k, cp, P, S);
     }
@@ -198,18 +132,19 @@ __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx,
         const CellPhys cc = cell_phys(k, nnc);
         const float dk = dkprev + cc.dk;
         DKint[idx] = dk;
+        if (a.flags & TROW_TAPE) a.tape_dk[(size_t)(a.cur.step - a.tape_s0) * n2 + idx] = dk;
         rk4_thermal(P, S, cc, dk, a.cur, k);
     }
 }
 
 // The same, loading the cell's inputs itself.
-__device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, size_t sprev, size_t scur,
+__device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, size_t n2, size_t sprev, size_t scur,
                                            float4 *__restrict__ Uout, const float2 *__restrict__ NT,
                                            float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
     const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
     const float2 nnp = (a.flags & (TROW_NPREV | TROW_QPREV)) ? NT[sprev + idx] : make_float2(0.f, 0.f);
     const float2 nnc = (a.flags & TROW_NCUR) ? NT[scur + idx] : make_float2(0.f, 0.f);
-    therm_cell_pre(P, S, idx, sprev, dkprev, nnp, nnc, Uout, Q, DKint, a);
+    therm_cell_pre(P, S, idx, n2, sprev, dkprev, nnp, nnc, Uout, Q, DKint, a);
 }
 
 // Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
@@ -254,7 +189,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
     for (int j = 0; j < E; j++) {
         const int o = lds_pad(t + j * TT);
         float2 P = la[o], S = lb[o];
-        therm_cell(P, S, base + t + j * TT, sprev, scur, Uout, NT, Q, DKint, a);
+        therm_cell(P, S, base + t + j * TT, n2, sprev, scur, Uout, NT, Q, DKint, a);
         la[o] = P;
         lb[o] = S;
     }
@@ -318,7 +253,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     for (int q = 0; q < CPT; q++) {
         const int cc = tid + q * THREADS, o = lds_pad(cc);
         float2 P = lds[0][o], S = lds[1][o];
-        therm_cell_pre(P, S, base + cc, sprev, dkp[q], nnp[q], nnc[q], Uout, Q, DKint, a);
+        therm_cell_pre(P, S, base + cc, n2, sprev, dkp[q], nnp[q], nnc[q], Uout, Q, DKint, a);
         lds[0][o] = P;
         lds[1][o] = S;
     }

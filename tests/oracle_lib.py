@@ -40,6 +40,7 @@ def load(path=ORACLE_SO):
         "oracle_reduced_sum": (ctypes.c_double, [fp, fp, ctypes.c_int64]),
         "oracle_set_q": (None, [P, fp, dp, dp, fp]),
         "oracle_thermal_cwes": (None, [P, fp, dp, dp, fp]),
+        "oracle_thermal_cwes64": (None, [P, fp, dp, dp, fp]),
         "oracle_thermal_run": (ctypes.c_int, [P, ctypes.c_int, dp, dp, dp, dp, fp, fp, i64p]),
     }
     for name, (res, args) in sigs.items():
@@ -132,14 +133,16 @@ def set_q(p, Tslice, fh, sh):
     return q
 
 
-def thermal_cwes(p, T, fh0, sh0, Q=None):
-    """One THERMAL z-march at temperature T; returns (fh_out, sh_out, Q)."""
+def thermal_cwes(p, T, fh0, sh0, Q=None, prec=32):
+    """One THERMAL z-march at temperature T (FP32 restatement, or FP64 with prec=64);
+    returns (fh_out, sh_out, Q)."""
     hz = p.heat_nz if p.heat_nz > 0 else p.nz
     T = np.ascontiguousarray(T, np.float32).reshape(hz, p.ny, p.nx)
     Q = np.zeros_like(T) if Q is None else np.ascontiguousarray(Q, np.float32).copy()
     fh = _c128(fh0).copy()
     sh = _c128(sh0).copy()
-    load().oracle_thermal_cwes(ctypes.byref(p), _fp(T), _dp(fh), _dp(sh), _fp(Q))
+    fn = load().oracle_thermal_cwes if prec == 32 else load().oracle_thermal_cwes64
+    fn(ctypes.byref(p), _fp(T), _dp(fh), _dp(sh), _fp(Q))
     return fh, sh, Q
 
 

@@ -273,3 +273,18 @@ def test_heat_source_terms():
     # each term alone
     q_p = orc.set_q(p.copy(alpha_s=0.0, beta_s=0.0), Ts, fh, sh)
     assert np.abs(q_p - p.alpha_p * ip).max() / (p.alpha_p * ip).max() < 1e-5
+
+
+def test_thermal_fp64_restatement_matches_fp32_where_dk_agrees():
+    # At 26.913 C the float and double phase mismatch agree to 7e-7 rad over the crystal,
+    # so the FP32 and FP64 THERMAL marches must agree to float rounding (2e-6 here; at
+    # 27 C the float dk offset alone separates them by 5e-3 on the SH).
+    p = config1_params(nx=64, ny=64, nz=12, thermal=1, heat_nz=4, temp=26.913, power=40.0, waist=30.0,
+                       lx=240.0, ly=240.0, alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7)
+    T = np.full((4, 64, 64), 26.913, np.float32)
+    fh0 = orc.pump(p, 64).astype(np.complex128)
+    sh0 = np.zeros_like(fh0)
+    f32, s32, q32 = orc.thermal_cwes(p, T, fh0, sh0, prec=32)
+    f64, s64, q64 = orc.thermal_cwes(p, T, fh0, sh0, prec=64)
+    assert orc.rel_l2(f32, f64) < 1e-5 and orc.rel_l2(s32, s64) < 1e-5, (orc.rel_l2(f32, f64), orc.rel_l2(s32, s64))
+    assert np.linalg.norm(q32 - q64) / np.linalg.norm(q64) < 1e-5
