@@ -111,3 +111,12 @@ def config2_params(**overrides):
     for k, v in overrides.items():
         setattr(p, k, v)
     return p
+
+
+def config5_params(**overrides):
+    """Large inverse-design instance: as config2 (THERMAL, w0 = 40 um, 8*w0 window) but
+    512x512 grid, nz = 1000, heat grid 512x512x200, P = 50 W."""
+    p = config2_params(nx=512, ny=512, nz=1000, heat_nz=200, power=50.0)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p

@@ -118,3 +118,33 @@ def test_thermal_checkpoint_interval_does_not_change_gradients(gpu):
         assert np.abs(g - g0).max() <= 1e-5 * np.abs(g0).max()
         assert np.abs(m - m0).max() <= 1e-5 * np.abs(m0).max()
         assert abs(r["d_eta_d_temp"] - r0["d_eta_d_temp"]) <= 1e-5 * abs(r0["d_eta_d_temp"])
+
+
+@pytest.mark.gpu
+def test_phase_mask_ascent_increases_efficiency(gpu):
+    from shg_focused_gaussian_beams_amd.inverse import ascend_phase_mask, mask_gradient
+    p = _params(nz=20)
+    T = _smooth_T(p)
+    pump = gpu.make_pump(p, float(T[0, p.ny // 2, p.nx // 2]))
+    mask0 = np.random.default_rng(5).uniform(-0.1, 0.1, pump.shape)
+    with gpu.Context(p) as ctx:
+        ctx.set_temperature(T)
+        # the mask gradient is the chain rule of the field gradient: check it by FD once
+        fld = (pump * np.exp(1j * mask0)).astype(np.complex64)
+        ctx.set_fields(fld, np.zeros_like(fld))
+        res, g = ctx.adjoint()
+        d = mask_gradient(g, fld)
+        v = np.random.default_rng(6).standard_normal(pump.shape)
+        e = 1e-2
+        etas = []
+        for sgn in (1, -1):
+            f = (pump * np.exp(1j * (mask0 + sgn * e * v))).astype(np.complex64)
+            ctx.set_fields(f, np.zeros_like(f))
+            ctx.forward()
+            etas.append(ctx.powers()[2])
+        fd = (etas[0] - etas[1]) / (2 * e)
+        print(f"\n  mask: fd {fd:+.6e} adjoint {np.sum(d * v):+.6e}", flush=True)
+        assert abs(fd - np.sum(d * v)) <= 5e-3 * abs(fd)
+        mask, hist = ascend_phase_mask(ctx, pump, mask0, 4, step=0.05)
+    print("  eta history", ["%.6e" % x for x in hist])
+    assert hist[-1] > hist[0]
