@@ -1,4 +1,4 @@
-"""Builds the HIP library (libshg.so) and the SHG command-line program in-tree.
+"""Builds the HIP library (libshg.so) and the SHG and SHG_sweep programs in-tree.
 
 hipcc compiles for gfx950 only.  Packed FP32 instructions are disabled
 (-target-feature -packed-fp32-ops) for every HIP translation unit.
@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libshg.so")
 CLI = os.path.join(HERE, "SHG")
+SWEEP = os.path.join(HERE, "SHG_sweep")
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -31,9 +32,8 @@ def build(verbose=True):
         srcs += ["-x", "hip", os.path.join(CSRC, s)]
     _run([hipcc, *HIP_FLAGS, "-shared", "-o", LIB + ".tmp", *srcs])
     os.replace(LIB + ".tmp", LIB)
-    cli_src = os.path.join(CSRC, "shg_cli.cpp")
-    if os.path.exists(cli_src):
-        _run(["g++", "-O2", "-std=c++17", "-o", CLI, cli_src, "-L" + HERE, "-lshg",
+    for exe, src in ((CLI, "shg_cli.cpp"), (SWEEP, "shg_sweep.cpp")):
+        _run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(CSRC, src), "-L" + HERE, "-lshg",
               "-Wl,-rpath,$ORIGIN", "-I" + os.path.join(HERE, "..", "include")])
     return LIB
 

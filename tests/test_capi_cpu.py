@@ -83,6 +83,15 @@ def test_device_fft_index_logic_in_host_build(tmp_path):
                     assert np.abs(o - ref).max() / np.abs(ref).max() < 1e-6, (n, sign, mode)
 
 
+def test_sweep_rejects_bad_command_lines(tmp_path):
+    exe = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG_sweep")
+    if not os.path.exists(exe):
+        pytest.skip("SHG_sweep program not built")
+    for bad in (["--power=a:b"], ["--power=5:1:1"], ["--nope=1"], ["stray"]):
+        r = subprocess.run([exe] + bad, capture_output=True, text=True, cwd=tmp_path)
+        assert r.returncode == 1 and ("usage" in r.stderr or "option" in r.stderr), (bad, r.stderr)
+
+
 def test_cli_rejects_bad_command_lines(tmp_path):
     exe = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG")
     if not os.path.exists(exe):

@@ -74,3 +74,34 @@ def test_cli_thermal_matches_library(gpu, tmp_path):
     assert max(e) < 1e-6, e
     assert np.abs(t5 - T[5]).max() <= 1e-4 * np.abs(T[5]).max()   # 6 significant digits in the file
     assert "Global iterations" in out
+
+
+SWEEP = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG_sweep")
+
+
+@pytest.mark.gpu
+def test_sweep_program_matches_single_solves(gpu, tmp_path):
+    # the SHG.sh loops (power x waist) as one batched launch; each point's folder holds
+    # the files one SHG run would write
+    if not os.path.exists(SWEEP):
+        pytest.fail("SHG_sweep program not built")
+    args = ["--power=5,10", "--waist=28:31:3", "--t1=27", "--nz=20", "--folder=" + str(tmp_path / "Sims")] + COMMON
+    print("\nSHG_sweep " + " ".join(args), flush=True)
+    r = subprocess.run([SWEEP] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for P in (5, 10):
+        for w in (28, 31):
+            d = tmp_path / "Sims" / f"sPPLT_P_{P}_W_T1_27_WAIST_{w}"
+            assert (d / f"sPPLT_P_{P}_W_T1_27_WAIST_{w}.txt").exists()
+            fh, sh = _read(d, "Pump_out"), _read(d, "Signal_out")
+            p = default_params(nx=64, ny=64, nz=20, lx=248.0, ly=248.0, qpm_period=8.0265, deff=1e-5,
+                               power=float(P), waist=float(w), temp=27.0, t_peltier1=27.0, t_peltier2=27.0)
+            p.focus = 0.5 * float(np.float32(p.lz))
+            pump = gpu.make_pump(p)
+            with gpu.Context(p) as ctx:
+                ctx.set_fields(pump, np.zeros_like(pump))
+                ctx.forward()
+                rfh, rsh = ctx.get_fields()
+            e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
+            print(f"  P={P} w={w}: relL2 FH {e[0]:.2e} SH {e[1]:.2e}")
+            assert max(e) < 1e-5, e
