@@ -128,6 +128,7 @@ shg::ThermStep therm_step(const shg_ctx *c, int s);
 static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
     shg::ThermRowArgs a{};
     a.k = c->tc;
+    a.ct = c->ctab;
     a.prev = therm_step(c, 0);
     a.cur = therm_step(c, 1);
     a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
@@ -146,8 +147,7 @@ int tune_variants(shg_ctx *c, int reps) {
     const int nrow = c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
     shg::ThermRowArgs a{};
     if (c->p.thermal) {
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
-                                        c->tc.lp, c->tc.ls, c->stream));
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
         a = therm_mid_args(c);
     }
     float best = 1e30f;
@@ -182,6 +182,7 @@ shg::ThermStep therm_step(const shg_ctx *c, int s) {
     const float incs[3] = {0.0f, 0.25f, 0.5f};     // 0.5 * {0, .5, 1} (src/headers/Solver.h:201-212)
     for (int i = 0; i < 3; i++) st.zfac[i] = (double)(k.dz * incs[i]) + (s + 1.0) * (double)k.dz;
     st.sp1 = s + 1.0;
+    for (int i = 0; i < 3; i++) st.zr[i] = st.zfac[i] / st.sp1;
     st.slice = shg::phys::heat_slice(s, c->p.nz, c->hz);
     st.step = s;
     return st;
@@ -194,6 +195,7 @@ int enqueue_march_thermal(shg_ctx *c) {
     const int nz = c->p.nz;
     shg::ThermRowArgs a{};
     a.k = c->tc;
+    a.ct = c->ctab;
     a.cur = therm_step(c, 0);
     a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
     const float2 *NT = c->ntab;
@@ -238,7 +240,7 @@ void release(shg_ctx *c) {
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
                       (void *)c->d_coef, (void *)c->d_desc,
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
-                      (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
+                      (void *)c->ntab, (void *)c->ctab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
                       (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam, (void *)c->adj.Ap, (void *)c->adj.As,
                       (void *)c->adj.Gc, (void *)c->adj.Dc, (void *)c->adj.dpart, (void *)c->adj.dsum,
@@ -320,8 +322,7 @@ int timed_march(shg_ctx *c) {
     if (c->use_graph && !c->gexec) RC_TRY(shg_internal_capture(c->stream, enqueue, &c->gexec));
     HIP_TRY(hipEventRecord(c->e0, c->stream));
     if (c->p.thermal)
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
-                                        c->tc.lp, c->tc.ls, c->stream));
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
     if (c->use_graph) {
         HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
     } else {
@@ -435,6 +436,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
             hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
             hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
             hipMalloc(&c->ntab, n3 * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&c->ctab, n3 * sizeof(float4)) != hipSuccess ||
             hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
             hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
@@ -577,8 +579,7 @@ int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
     if (c->p.thermal)
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc.crystal,
-                                        c->tc.lp, c->tc.ls, c->stream));
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
     const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
     const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
     const shg::ThermRowArgs a = c->p.thermal ? therm_mid_args(c) : shg::ThermRowArgs{};

@@ -33,8 +33,8 @@ __device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, fl
     float4 cf[3];
 #pragma unroll
     for (int g = 0; g < 3; g++) {
-        cf[g] = phase_coef(c, dkint, st.zfac[g], st.sp1);   // (kp sin, kp cos, -ks sin, ks cos)
-        const float phi = (float)(((double)dkint * st.zfac[g]) / st.sp1);
+        cf[g] = phase_coef(c, dkint, st.zr[g]);   // (kp sin, kp cos, -ks sin, ks cos)
+        const float phi = (float)((double)dkint * st.zr[g]);
         sincosf(phi, &sn[g], &cs[g]);
     }
     AdjAcc acc[3];
@@ -49,7 +49,7 @@ __device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, fl
         dkp += acc[g].c1r * sn[g] + acc[g].c1i * cs[g];
         dks += -acc[g].c2r * sn[g] + acc[g].c2i * cs[g];
         const double dphi = acc[g].c1r * cf[g].y - acc[g].c1i * cf[g].x - acc[g].c2r * cf[g].w + acc[g].c2i * cf[g].z;
-        dphi_dk += dphi * st.zfac[g] / st.sp1;
+        dphi_dk += dphi * st.zr[g];
     }
     dDK += dphi_dk;
     // kp = pi dQ / (n_p lp), ks = pi dQ / (2 n_s ls), tpa = tpa_base n_s / 2

@@ -50,9 +50,23 @@ __device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float2 nn) {
     return c;
 }
 
-// (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}), phi = (float)(DKint * zfac / (s+1)).
-__device__ __forceinline__ float4 phase_coef(const CellPhys &c, float dkint, double zfac, double sp1) {
-    const float phi = (float)(((double)dkint * zfac) / sp1);
+// The same values read from the per-march table: nn = {n_p, n_s}, cf = {kp, ks, dk, tpa}.
+__device__ __forceinline__ CellPhys cell_from_table(float2 nn, float4 cf) {
+    CellPhys c;
+    c.np = nn.x;
+    c.ns = nn.y;
+    c.kp = cf.x;
+    c.ks = cf.y;
+    c.dk = cf.z;
+    c.tpa = cf.w;
+    return c;
+}
+
+// (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}), phi = (float)(DKint * zfac / (s+1)) with
+// zr = zfac / (s+1) precomputed in FP64 (the float phase is the same but for rare
+// one-ulp double roundings).
+__device__ __forceinline__ float4 phase_coef(const CellPhys &c, float dkint, double zr) {
+    const float phi = (float)((double)dkint * zr);
     float sn, cs;
     sincosf(phi, &sn, &cs);
     return make_float4(c.kp * sn, c.kp * cs, -c.ks * sn, c.ks * cs);
@@ -61,9 +75,9 @@ __device__ __forceinline__ float4 phase_coef(const CellPhys &c, float dkint, dou
 // THERMAL solverRK4 (src/headers/Solver.h:197-213): stage increments 0, .25, .25, .5.
 __device__ __forceinline__ void rk4_thermal(float2 &P, float2 &S, const CellPhys &c, float dkint, const ThermStep &st,
                                             const ThermConst &k) {
-    const float4 ca = phase_coef(c, dkint, st.zfac[0], st.sp1);
-    const float4 cb = phase_coef(c, dkint, st.zfac[1], st.sp1);
-    const float4 cc = phase_coef(c, dkint, st.zfac[2], st.sp1);
+    const float4 ca = phase_coef(c, dkint, st.zr[0]);
+    const float4 cb = phase_coef(c, dkint, st.zr[1]);
+    const float4 cc = phase_coef(c, dkint, st.zr[2]);
     rk4_cell(P, S, ca, cb, cc, k.lossP, k.lossS, c.tpa, k.h6);
 }
 

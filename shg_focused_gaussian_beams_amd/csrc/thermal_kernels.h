@@ -34,6 +34,7 @@ struct ThermConst {
 // DKint_s * zfac[k] / (s + 1) with zfac[k] = dz*inc_k + (s+1)*dz, inc in {0, .25, .5}.
 struct ThermStep {
     double zfac[3];
+    double zr[3];                                 // zfac / (s + 1)
     double sp1;                                   // s + 1
     int slice;                                    // heat slice of step s
     int step;                                     // s
@@ -50,6 +51,7 @@ enum ThermRowFlags : int {
 struct ThermRowArgs {
     ThermStep prev, cur;
     ThermConst k;
+    const float4 *ct;                             // {kp, ks, dk, tpa} per heat voxel (launch_index_table)
     int flags;
     int tape_s0;
     float4 *tape_a2;
@@ -59,9 +61,11 @@ struct ThermRowArgs {
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                          double *part, int nparts, hipStream_t st);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
-// {n_p(T), n_s(T)} of every heat voxel (T is frozen during one march, so the row pass
-// reads the two indices instead of evaluating the FP64 Sellmeier fits per cell and step).
-hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, int crystal, float lp, float ls, hipStream_t st);
+// Per heat voxel: tab = {n_p(T), n_s(T)} and ct = {kp, ks, dk, tpa} with the reference's
+// float operations (T is frozen during one march, so the row pass reads them instead of
+// evaluating the FP64 Sellmeier fits and four float divisions per cell and step).
+hipError_t launch_index_table(const float *T, float2 *tab, float4 *ct, int64_t n, const ThermConst &k,
+                              hipStream_t st);
 // Row-pass layouts per size (the tuner times all of them): variants 0-3 are <E, W> row
 // layouts, 4-5 wide layouts (one row per workgroup, one cell per thread or two).
 constexpr int kThermRowVariantCount = 6;

@@ -92,15 +92,19 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
-// Refractive indices of every heat voxel, {n_p(T), n_s(T)}: the temperature is frozen
-// during a march, so the FP64 Sellmeier evaluation runs once per voxel instead of twice
-// per cell per row pass (the values are the same floats).
-__global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T, float2 *__restrict__ tab, int64_t n,
-                                                     int crystal, float lp, float ls) {
+// Refractive indices of every heat voxel, {n_p(T), n_s(T)}, and the coupling constants,
+// phase mismatch and TPA coefficient derived from them: the temperature is frozen during
+// a march, so the FP64 Sellmeier fits and the float divisions run once per voxel instead
+// of twice per cell per row pass (the values are the same floats).
+__global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T, float2 *__restrict__ tab,
+                                                     float4 *__restrict__ ct, int64_t n, ThermConst k) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float t = T[i];
-    tab[i] = make_float2(index_dev(crystal, lp, t), index_dev(crystal, ls, t));
+    const float2 nn = make_float2(index_dev(k.crystal, k.lp, t), index_dev(k.crystal, k.ls, t));
+    const CellPhys c = cell_phys(k, nn);
+    tab[i] = nn;
+    ct[i] = make_float4(c.kp, c.ks, c.dk, c.tpa);
 }
 
 // Row pass of the THERMAL z-march (flags in ThermRowFlags):
@@ -117,11 +121,12 @@ __global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T
 // One cell of the pass between the transforms (idx: cell index in the plane), from its
 // already loaded phase accumulator and index-table entries.
 __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx, size_t n2, size_t sprev, float dkprev,
-                                               float2 nnp, float2 nnc, float4 *__restrict__ Uout, float *__restrict__ Q,
+                                               float2 nnp, float4 cfp, float2 nnc, float4 cfc,
+                                               float4 *__restrict__ Uout, float *__restrict__ Q,
                                                float *__restrict__ DKint, const ThermRowArgs &a) {
     const ThermConst &k = a.k;
     if (a.flags & (TROW_NPREV | TROW_QPREV)) {
-        const CellPhys cp = cell_phys(k, nnp);
+        const CellPhys cp = cell_from_table(nnp, cfp);
         if ((a.flags & (TROW_TAPE | TROW_NPREV)) == (TROW_TAPE | TROW_NPREV))
             a.tape_a2[(size_t)(a.prev.step - a.tape_s0) * n2 + idx] = make_float4(P.x, P.y, S.x, S.y);
         if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
@@ -129,7 +134,7 @@ __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx,
     }
     if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
     if (a.flags & TROW_NCUR) {
-        const CellPhys cc = cell_phys(k, nnc);
+        const CellPhys cc = cell_from_table(nnc, cfc);
         const float dk = dkprev + cc.dk;
         DKint[idx] = dk;
         if (a.flags & TROW_TAPE) a.tape_dk[(size_t)(a.cur.step - a.tape_s0) * n2 + idx] = dk;
@@ -142,9 +147,12 @@ __device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, siz
                                            float4 *__restrict__ Uout, const float2 *__restrict__ NT,
                                            float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
     const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
-    const float2 nnp = (a.flags & (TROW_NPREV | TROW_QPREV)) ? NT[sprev + idx] : make_float2(0.f, 0.f);
-    const float2 nnc = (a.flags & TROW_NCUR) ? NT[scur + idx] : make_float2(0.f, 0.f);
-    therm_cell_pre(P, S, idx, n2, sprev, dkprev, nnp, nnc, Uout, Q, DKint, a);
+    const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
+    const float2 nnp = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
+    const float2 nnc = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
+    const float4 cfp = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 cfc = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    therm_cell_pre(P, S, idx, n2, sprev, dkprev, nnp, cfp, nnc, cfc, Uout, Q, DKint, a);
 }
 
 // Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
@@ -222,12 +230,16 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
     float dkp[CPT];
     float2 nnp[CPT], nnc[CPT];
+    float4 cfp[CPT], cfc[CPT];
+    const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const size_t idx = base + tid + q * THREADS;
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
-        nnp[q] = (a.flags & (TROW_NPREV | TROW_QPREV)) ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        nnc[q] = (a.flags & TROW_NCUR) ? NT[scur + idx] : make_float2(0.f, 0.f);
+        nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
+        nnc[q] = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
+        cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        cfc[q] = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
@@ -253,7 +265,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     for (int q = 0; q < CPT; q++) {
         const int cc = tid + q * THREADS, o = lds_pad(cc);
         float2 P = lds[0][o], S = lds[1][o];
-        therm_cell_pre(P, S, base + cc, n2, sprev, dkp[q], nnp[q], nnc[q], Uout, Q, DKint, a);
+        therm_cell_pre(P, S, base + cc, n2, sprev, dkp[q], nnp[q], cfp[q], nnc[q], cfc[q], Uout, Q, DKint, a);
         lds[0][o] = P;
         lds[1][o] = S;
     }
@@ -292,9 +304,10 @@ hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, int crystal, float lp, float ls, hipStream_t st) {
+hipError_t launch_index_table(const float *T, float2 *tab, float4 *ct, int64_t n, const ThermConst &k,
+                              hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_index_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, tab, n, crystal, lp, ls);
+    hipLaunchKernelGGL(k_index_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, tab, ct, n, k);
     return hipGetLastError();
 }
 
