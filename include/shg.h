@@ -243,7 +243,7 @@ int shg_batch_last_ms(shg_batch *b, double *ms);
  * rows or columns per workgroup).  shg_forward / shg_batch_run tune them on first use
  * (timing each (row, column) pair, replayed as the march runs them, on the context's own
  * data) unless the environment sets SHG_NO_TUNE=1; these calls tune explicitly or pin a
- * choice.  Rows 0..3 (THERMAL contexts: 0..5), columns 0..4. */
+ * choice.  Rows 0..4 (THERMAL contexts: 0..6), columns 0..7. */
 int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
 int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 

@@ -84,7 +84,8 @@ template <int N> struct LdsLen { static constexpr int value = N + N / 16; };
 // One Stockham stage, in registers.  Butterfly i = t + m*T takes x[i + r*N/R] (register
 // m + r*M), twiddles it by w^(r*k) with k = i mod Ns and leaves y_r in register m + r*M.
 // `tw` is the LDS table W_N^j = exp(-2 pi i j/N).
-template <int N, int E, int R, int Ns, int SIGN>
+// TWO = false transforms `a` only (`b` is neither read nor written).
+template <int N, int E, int R, int Ns, int SIGN, bool TWO = true>
 __device__ __forceinline__ void stage_compute(float2 (&a)[E], float2 (&b)[E], int t, const float2 *tw) {
     constexpr int T = N / E;
     constexpr int M = E / R;
@@ -94,7 +95,10 @@ __device__ __forceinline__ void stage_compute(float2 (&a)[E], float2 (&b)[E], in
         const int k = i & (Ns - 1);
         float2 xa[R], xb[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) { xa[r] = a[m + r * M]; xb[r] = b[m + r * M]; }
+        for (int r = 0; r < R; r++) {
+            xa[r] = a[m + r * M];
+            if constexpr (TWO) xb[r] = b[m + r * M];
+        }
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -102,19 +106,22 @@ __device__ __forceinline__ void stage_compute(float2 (&a)[E], float2 (&b)[E], in
                 float2 w = tw[e];
                 if (SIGN > 0) w.y = -w.y;
                 xa[r] = c_mul(xa[r], w);
-                xb[r] = c_mul(xb[r], w);
+                if constexpr (TWO) xb[r] = c_mul(xb[r], w);
             }
         }
         dft<R, SIGN>(xa);
-        dft<R, SIGN>(xb);
+        if constexpr (TWO) dft<R, SIGN>(xb);
 #pragma unroll
-        for (int r = 0; r < R; r++) { a[m + r * M] = xa[r]; b[m + r * M] = xb[r]; }
+        for (int r = 0; r < R; r++) {
+            a[m + r * M] = xa[r];
+            if constexpr (TWO) b[m + r * M] = xb[r];
+        }
     }
 }
 
 // Writes a stage's outputs to their autosort positions (i-k)*R + k + r*Ns.  After the
 // last stage that position is t + (m + r*M)*T, i.e. the registers are already in order.
-template <int N, int E, int R, int Ns, typename P>
+template <int N, int E, int R, int Ns, bool TWO = true, typename P>
 __device__ __forceinline__ void stage_scatter(const float2 (&a)[E], const float2 (&b)[E], P la, P lb, int t, int st) {
     constexpr int T = N / E;
     constexpr int M = E / R;
@@ -127,19 +134,19 @@ __device__ __forceinline__ void stage_scatter(const float2 (&a)[E], const float2
         for (int r = 0; r < R; r++) {
             const int o = lds_pad(ob + r * Ns) * st;
             la[o] = a[m + r * M];
-            lb[o] = b[m + r * M];
+            if constexpr (TWO) lb[o] = b[m + r * M];
         }
     }
 }
 
-template <int N, int E, typename P>
+template <int N, int E, bool TWO = true, typename P>
 __device__ __forceinline__ void lds_gather(float2 (&a)[E], float2 (&b)[E], P la, P lb, int t, int st) {
     constexpr int T = N / E;
 #pragma unroll
     for (int j = 0; j < E; j++) {
         const int o = lds_pad(t + j * T) * st;
         a[j] = la[o];
-        b[j] = lb[o];
+        if constexpr (TWO) b[j] = lb[o];
     }
 }
 
@@ -152,12 +159,12 @@ template <> struct FftPlan<512> { static constexpr int E = 8;  };   // 8 x 8 x 8
 
 // Exchange between two stages: barrier (earlier readers of this LDS are done), scatter,
 // barrier, gather.
-template <int N, int E, int R, int Ns, typename P, typename Sync>
+template <int N, int E, int R, int Ns, bool TWO = true, typename P, typename Sync>
 __device__ __forceinline__ void exchange(float2 (&a)[E], float2 (&b)[E], P la, P lb, int t, int st, Sync sync) {
     sync();
-    stage_scatter<N, E, R, Ns>(a, b, la, lb, t, st);
+    stage_scatter<N, E, R, Ns, TWO>(a, b, la, lb, t, st);
     sync();
-    lds_gather<N, E>(a, b, la, lb, t, st);
+    lds_gather<N, E, TWO>(a, b, la, lb, t, st);
 }
 
 // Radix plan for length N with E elements per thread: radix E while it fits, then the
@@ -173,15 +180,15 @@ __host__ __device__ constexpr int plan_stride(int N, int E, int s) {
     return ns;
 }
 
-template <int N, int E, int S, int SIGN, typename P, typename Sync>
+template <int N, int E, int S, int SIGN, bool TWO = true, typename P, typename Sync>
 __device__ __forceinline__ void fft_stages(float2 (&a)[E], float2 (&b)[E], P la, P lb, int t, int st,
                                            const float2 *tw, Sync sync) {
     constexpr int R = plan_radix(N, E, S), Ns = plan_stride(N, E, S);
     static_assert(R >= 2 && E % R == 0, "radix must divide the elements per thread");
-    stage_compute<N, E, R, Ns, SIGN>(a, b, t, tw);
+    stage_compute<N, E, R, Ns, SIGN, TWO>(a, b, t, tw);
     if constexpr (Ns * R < N) {
-        exchange<N, E, R, Ns>(a, b, la, lb, t, st, sync);
-        fft_stages<N, E, S + 1, SIGN>(a, b, la, lb, t, st, tw, sync);
+        exchange<N, E, R, Ns, TWO>(a, b, la, lb, t, st, sync);
+        fft_stages<N, E, S + 1, SIGN, TWO>(a, b, la, lb, t, st, tw, sync);
     }
 }
 
@@ -196,6 +203,13 @@ __device__ __forceinline__ void fft2fe(float2 (&a)[E], float2 (&b)[E], P la, P l
     fft_stages<N, E, 0, SIGN>(a, b, la, lb, t, st, tw, sync);
 }
 
+// The same for one field.
+template <int N, int E, int SIGN, typename P, typename Sync>
+__device__ __forceinline__ void fft1fe(float2 (&a)[E], P la, int t, int st, const float2 *tw, Sync sync) {
+    static_assert(N >= 64 && N <= 512 && (N & (N - 1)) == 0, "supported lengths: 64..512, powers of two");
+    fft_stages<N, E, 0, SIGN, false>(a, a, la, la, t, st, tw, sync);
+}
+
 // Ordering point for LDS exchanges inside one wavefront (a transform run by at most 64
 // lanes of one wave): a wave's LDS operations complete in issue order, so only the
 // compiler's reordering has to be fenced.
@@ -204,6 +218,23 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+#endif
+
+#ifndef SHG_HOST_EMULATION
+// In-place transform of one field held in natural order in LDS (l[pad(o)], o < N) by the
+// T = N/E lanes t of one wavefront: gather x[t + j T], transform, write back in order.
+template <int N, int E, int SIGN>
+__device__ __forceinline__ void fft_lds_field(float2 *l, int t, const float2 *tw) {
+    constexpr int T = N / E;
+    static_assert(T <= 64, "one wavefront per transform");
+    float2 x[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) x[j] = l[lds_pad(t + j * T)];
+    fft1fe<N, E, SIGN>(x, l, t, 1, tw, [] { wave_sync(); });
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < E; j++) l[lds_pad(t + j * T)] = x[j];
 }
 #endif
 

@@ -37,8 +37,9 @@ enum RowMode : int {
     ROW_AUTO = 3     // ROW_MID if s + 1 < steps of the instance, else ROW_END
 };
 
-// `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..3, columns 0..4).
-constexpr int kRowVariantCount = 4, kColVariantCount = 5;
+// `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..4, columns 0..7;
+// row 4 and columns 5..7 are the split layouts).
+constexpr int kRowVariantCount = 5, kColVariantCount = 8;
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
                            int variant = 0);
 hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st, int variant = 0);

@@ -108,6 +108,91 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
         U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
 
+// ---------------------------------------------------------------------------
+// Split layouts: the workgroup loads its cells one per thread (coalesced, spread over all
+// its waves) into LDS in natural order; each 1D transform of one field of one line runs
+// on its own wavefront (so the two fields' transforms run side by side instead of one
+// wave doing both); the per-cell work again runs one cell per thread.  Lower latency per
+// pass for fields too small to fill the GPU in the register layouts.
+
+// Row pass, one row per workgroup of W >= 2 waves: waves 0 and 1 transform FH and SH.
+template <int N, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_row_split(const InstDesc *__restrict__ descs,
+                                                      const float2 *__restrict__ twg, int step, int mode) {
+    constexpr int T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    static_assert(W >= 2 && T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2][L];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[blockIdx.y];
+    if (mode == ROW_AUTO) mode = step + 1 < d.steps ? ROW_MID : ROW_END;
+    const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
+    float4 *Ur = d.U + (size_t)blockIdx.x * N;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    for (int c = tid; c < N; c += THREADS) {
+        const float4 v = Ur[c];
+        lds[0][lds_pad(c)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(c)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    if (mode != ROW_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
+    __syncthreads();
+    const float4 c0 = d.coef[step];
+    const float4 c1 = d.coef[mode == ROW_START ? step : (mode == ROW_MID ? step + 1 : step)];
+    for (int c = tid; c < N; c += THREADS) {
+        float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+        if (mode != ROW_START) rk4(P, S, c0, d.nl);
+        if (mode != ROW_END) rk4(P, S, c1, d.nl);
+        lds[0][lds_pad(c)] = P;
+        lds[1][lds_pad(c)] = S;
+    }
+    __syncthreads();
+    if (mode != ROW_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
+    __syncthreads();
+    for (int c = tid; c < N; c += THREADS) {
+        const float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+        Ur[c] = make_float4(P.x, P.y, S.x, S.y);
+    }
+}
+
+// Column pass over CW adjacent columns per workgroup of 2*CW waves: loads and stores move
+// CW*16-byte row segments; wave k transforms field k / CW of column k % CW.
+template <int N, int E, int CW>
+__global__ __launch_bounds__(128 * CW) void k_col_split(const InstDesc *__restrict__ descs,
+                                                        const float2 *__restrict__ twg) {
+    constexpr int T = N / E, THREADS = 128 * CW, L = LdsLen<N>::value;
+    static_assert(T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2 * CW][L];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[blockIdx.y];
+    float4 *__restrict__ U = d.U;
+    const float2 *__restrict__ G = d.G;
+    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63;
+    const int x0 = blockIdx.x * CW;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    for (int i = tid; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW;
+        const float4 v = U[(size_t)y * N + x0 + c];
+        lds[c][lds_pad(y)] = make_float2(v.x, v.y);
+        lds[CW + c][lds_pad(y)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
+    __syncthreads();
+    for (int i = tid; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW, x = x0 + c;
+        lds[c][lds_pad(y)] = c_mul(c_mul(G[x], G[N + y]), lds[c][lds_pad(y)]);
+        lds[CW + c][lds_pad(y)] = c_mul(c_mul(G[2 * N + x], G[3 * N + y]), lds[CW + c][lds_pad(y)]);
+    }
+    __syncthreads();
+    if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
+    __syncthreads();
+    for (int i = tid; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW;
+        const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
+        U[(size_t)y * N + x0 + c] = make_float4(P.x, P.y, S.x, S.y);
+    }
+}
+
 // FP64 sums of |FH|^2 and |SH|^2: per-block partials, then one block folds them.
 // Deterministic (fixed reduction tree).
 __global__ __launch_bounds__(256) void k_power_partial(const float4 *__restrict__ U, int64_t n, double2 *part) {
@@ -183,8 +268,21 @@ static hipError_t col_launch(const InstDesc *d, int ninst, const float2 *tw, hip
     return hipGetLastError();
 }
 
+template <int N, int E, int W>
+static hipError_t row_split_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
+    hipLaunchKernelGGL((k_row_split<N, E, W>), dim3(N, ninst), dim3(64 * W), 0, st, d, tw, step, mode);
+    return hipGetLastError();
+}
+template <int N, int E, int CW>
+static hipError_t col_split_launch(const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
+    hipLaunchKernelGGL((k_col_split<N, E, CW>), dim3(N / CW, ninst), dim3(128 * CW), 0, st, d, tw);
+    return hipGetLastError();
+}
+
 #define ROWV(N, E, W) return row_launch<N, E, W>(d, ninst, tw, step, mode, st)
 #define COLV(N, E, CW) return col_launch<N, E, CW>(d, ninst, tw, st)
+#define ROWS(N, E, W) return row_split_launch<N, E, W>(d, ninst, tw, step, mode, st)
+#define COLS(N, E, CW) return col_split_launch<N, E, CW>(d, ninst, tw, st)
 
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
                            int variant) {
@@ -194,18 +292,22 @@ hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 64 * 16 + 1: ROWV(64, 4, 1);
         case 64 * 16 + 2: ROWV(64, 8, 4);
         case 64 * 16 + 3: ROWV(64, 16, 1);
+        case 64 * 16 + 4: ROWS(64, 4, 2);
         case 128 * 16 + 0: ROWV(128, 16, 1);
         case 128 * 16 + 1: ROWV(128, 8, 1);
         case 128 * 16 + 2: ROWV(128, 4, 1);
         case 128 * 16 + 3: ROWV(128, 16, 2);
+        case 128 * 16 + 4: ROWS(128, 4, 2);
         case 256 * 16 + 0: ROWV(256, 16, 1);
         case 256 * 16 + 1: ROWV(256, 8, 1);
         case 256 * 16 + 2: ROWV(256, 4, 1);
         case 256 * 16 + 3: ROWV(256, 16, 2);
+        case 256 * 16 + 4: ROWS(256, 4, 4);
         case 512 * 16 + 0: ROWV(512, 8, 1);
         case 512 * 16 + 1: ROWV(512, 16, 1);
         case 512 * 16 + 2: ROWV(512, 4, 2);
         case 512 * 16 + 3: ROWV(512, 8, 4);
+        case 512 * 16 + 4: ROWS(512, 8, 8);
         default: return hipErrorInvalidValue;
     }
 }
@@ -218,21 +320,33 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 64 * 16 + 2: COLV(64, 8, 16);
         case 64 * 16 + 3: COLV(64, 4, 4);
         case 64 * 16 + 4: COLV(64, 4, 16);
+        case 64 * 16 + 5: COLS(64, 4, 1);
+        case 64 * 16 + 6: COLS(64, 4, 2);
+        case 64 * 16 + 7: COLS(64, 4, 4);
         case 128 * 16 + 0: COLV(128, 16, 8);
         case 128 * 16 + 1: COLV(128, 8, 8);
         case 128 * 16 + 2: COLV(128, 4, 8);
         case 128 * 16 + 3: COLV(128, 4, 2);
         case 128 * 16 + 4: COLV(128, 4, 4);
+        case 128 * 16 + 5: COLS(128, 4, 1);
+        case 128 * 16 + 6: COLS(128, 4, 2);
+        case 128 * 16 + 7: COLS(128, 4, 4);
         case 256 * 16 + 0: COLV(256, 16, 8);
         case 256 * 16 + 1: COLV(256, 8, 8);
         case 256 * 16 + 2: COLV(256, 4, 8);
         case 256 * 16 + 3: COLV(256, 4, 1);
         case 256 * 16 + 4: COLV(256, 4, 2);
+        case 256 * 16 + 5: COLS(256, 4, 1);
+        case 256 * 16 + 6: COLS(256, 4, 2);
+        case 256 * 16 + 7: COLS(256, 4, 4);
         case 512 * 16 + 0: COLV(512, 8, 4);
         case 512 * 16 + 1: COLV(512, 16, 4);
         case 512 * 16 + 2: COLV(512, 4, 4);
         case 512 * 16 + 3: COLV(512, 8, 1);
         case 512 * 16 + 4: COLV(512, 8, 2);
+        case 512 * 16 + 5: COLS(512, 8, 1);
+        case 512 * 16 + 6: COLS(512, 8, 2);
+        case 512 * 16 + 7: COLS(512, 8, 4);
         default: return hipErrorInvalidValue;
     }
 }

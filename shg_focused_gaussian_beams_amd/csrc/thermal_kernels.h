@@ -67,8 +67,9 @@ hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipS
 hipError_t launch_index_table(const float *T, float2 *tab, float4 *ct, int64_t n, const ThermConst &k,
                               hipStream_t st);
 // Row-pass layouts per size (the tuner times all of them): variants 0-3 are <E, W> row
-// layouts, 4-5 wide layouts (one row per workgroup, one cell per thread or two).
-constexpr int kThermRowVariantCount = 6;
+// layouts, 4-5 wide layouts (one row per workgroup, one cell per thread or two), 6 the
+// split layout (the two fields transformed by two waves side by side).
+constexpr int kThermRowVariantCount = 7;
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
                                    float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0);
 // Grid size the Jacobi launcher uses (number of residual partials it writes).

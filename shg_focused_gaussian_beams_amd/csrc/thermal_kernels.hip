@@ -283,6 +283,63 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     }
 }
 
+// Split layout <N, E, W>, W >= 2: one row per workgroup; the row is loaded one cell per
+// thread into LDS, waves 0 and 1 transform FH and SH side by side, every thread steps its
+// cells, and the row is stored one cell per thread.
+template <int N, int E, int W>
+__global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__ U, float4 *__restrict__ Uout,
+                                                            const float2 *__restrict__ twg,
+                                                            const float2 *__restrict__ NT, float *__restrict__ Q,
+                                                            float *__restrict__ DKint, ThermRowArgs a) {
+    constexpr int T = N / E, THREADS = 64 * W, L = LdsLen<N>::value, CPT = (N + THREADS - 1) / THREADS;
+    static_assert(W >= 2 && T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2][L];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
+    const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
+    float dkp[CPT];
+    float2 nnp[CPT], nnc[CPT];
+    float4 cfp[CPT], cfc[CPT];
+    const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * THREADS;
+        const size_t idx = base + (c < N ? c : 0);
+        dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
+        nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
+        nnc[q] = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
+        cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        cfc[q] = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    for (int c = tid; c < N; c += THREADS) {
+        const float4 v = U[base + c];
+        lds[0][lds_pad(c)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(c)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    if ((a.flags & TROW_FWD) && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * THREADS;
+        if (c < N) {
+            float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+            therm_cell_pre(P, S, base + c, n2, sprev, dkp[q], nnp[q], cfp[q], nnc[q], cfc[q], Uout, Q, DKint, a);
+            lds[0][lds_pad(c)] = P;
+            lds[1][lds_pad(c)] = S;
+        }
+    }
+    __syncthreads();
+    if ((a.flags & TROW_BWD) && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
+    __syncthreads();
+    for (int c = tid; c < N; c += THREADS) {
+        const float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+        U[base + c] = make_float4(P.x, P.y, S.x, S.y);
+    }
+}
+
 // ---------------------------------------------------------------------------
 int jacobi_blocks(int nx, int ny, int hz) { return (nx / 64) * ((ny + 3) / 4) * hz; }
 
@@ -327,7 +384,15 @@ static hipError_t therm_wide_launch(float4 *U, float4 *Uout, const float2 *tw, c
     return hipGetLastError();
 }
 
+template <int N, int E, int W>
+static hipError_t therm_split_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
+                                     float *DKint, const ThermRowArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL((k_therm_row_split<N, E, W>), dim3(N), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
+    return hipGetLastError();
+}
+
 #define THRV(N, E, W) return therm_row_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
+#define THRS(N, E, W) return therm_split_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
 #define THRW(N, E, W) return therm_wide_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
 
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
@@ -340,28 +405,33 @@ hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 
         case 64 * 16 + 3: THRV(64, 8, 2);
         case 64 * 16 + 4: THRW(64, 4, 1);
         case 64 * 16 + 5: THRW(64, 8, 1);
+        case 64 * 16 + 6: THRS(64, 4, 2);
         case 128 * 16 + 0: THRV(128, 16, 1);
         case 128 * 16 + 1: THRV(128, 8, 1);
         case 128 * 16 + 2: THRV(128, 4, 1);
         case 128 * 16 + 3: THRV(128, 4, 2);
         case 128 * 16 + 4: THRW(128, 4, 2);
         case 128 * 16 + 5: THRW(128, 4, 1);
+        case 128 * 16 + 6: THRS(128, 4, 2);
         case 256 * 16 + 0: THRV(256, 16, 1);
         case 256 * 16 + 1: THRV(256, 8, 1);
         case 256 * 16 + 2: THRV(256, 4, 1);
         case 256 * 16 + 3: THRV(256, 4, 4);
         case 256 * 16 + 4: THRW(256, 4, 4);
         case 256 * 16 + 5: THRW(256, 4, 2);
+        case 256 * 16 + 6: THRS(256, 4, 4);
         case 512 * 16 + 0: THRV(512, 8, 1);
         case 512 * 16 + 1: THRV(512, 4, 2);
         case 512 * 16 + 2: THRV(512, 16, 1);
         case 512 * 16 + 3: THRV(512, 8, 2);
         case 512 * 16 + 4: THRW(512, 8, 8);
         case 512 * 16 + 5: THRW(512, 8, 4);
+        case 512 * 16 + 6: THRS(512, 8, 8);
         default: return hipErrorInvalidValue;
     }
 }
 #undef THRV
 #undef THRW
+#undef THRS
 
 }  // namespace shg

@@ -4,6 +4,7 @@
 // stdin-free arguments: generates a seeded field, runs rows or columns mode, prints
 // the transformed values for comparison with numpy.
 //   usage: fft_emul <N> <sign +1|-1> <mode row|col> <seed> [E elements per thread, 0 = default plan]
+//          [one: transform the first field only (fft1fe); the second must come back unchanged]
 #include <barrier>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,7 @@ int main(int argc, char **argv) {
     std::vector<float2> lds_a((size_t)L * CW), lds_b((size_t)L * CW);
     std::vector<float2> outA(A.size()), outB(B.size());
     int E = argc > 5 ? std::atoi(argv[5]) : 0;
+    const bool one = argc > 6 && std::string(argv[6]) == "one";
     if (E == 0) E = (N == 64 || N == 512) ? 8 : 16;
     if (E != 4 && E != 8 && E != 16) return 2;
     const int T = N / E;
@@ -61,8 +63,13 @@ int main(int argc, char **argv) {
             constexpr int e = decltype(EE)::value;
             float2 a[e], b[e];
             for (int j = 0; j < e; j++) { a[j] = A[(size_t)(t + j * T) * CW + c]; b[j] = B[(size_t)(t + j * T) * CW + c]; }
-            if (sign > 0) shg::fft2fe<n, e, +1>(a, b, la, lb, t, CW, tw.data(), sync);
-            else shg::fft2fe<n, e, -1>(a, b, la, lb, t, CW, tw.data(), sync);
+            if (one) {
+                if (sign > 0) shg::fft1fe<n, e, +1>(a, la, t, CW, tw.data(), sync);
+                else shg::fft1fe<n, e, -1>(a, la, t, CW, tw.data(), sync);
+            } else {
+                if (sign > 0) shg::fft2fe<n, e, +1>(a, b, la, lb, t, CW, tw.data(), sync);
+                else shg::fft2fe<n, e, -1>(a, b, la, lb, t, CW, tw.data(), sync);
+            }
             for (int j = 0; j < e; j++) { outA[(size_t)(t + j * T) * CW + c] = a[j]; outB[(size_t)(t + j * T) * CW + c] = b[j]; }
         };
         auto runE = [&](auto NN) {

@@ -81,6 +81,15 @@ def test_device_fft_index_logic_in_host_build(tmp_path):
                     o = (d[:, k + 4] + 1j * d[:, k + 5]).reshape(n, cw)
                     ref = np.fft.ifft(a, axis=0) * n if sign > 0 else np.fft.fft(a, axis=0)
                     assert np.abs(o - ref).max() / np.abs(ref).max() < 1e-6, (n, sign, mode)
+            # the single-field transform (fft1fe): first field transformed, second untouched
+            if e in (4, 8) and n // e <= 64:
+                out = subprocess.run([str(exe), str(n), str(sign), "row", "12", str(e), "one"], check=True,
+                                     capture_output=True, text=True).stdout
+                d = np.array([list(map(float, l.split())) for l in out.strip().split("\n")])
+                a, o = d[:, 0] + 1j * d[:, 1], d[:, 4] + 1j * d[:, 5]
+                ref = np.fft.ifft(a) * n if sign > 0 else np.fft.fft(a)
+                assert np.abs(o - ref).max() / np.abs(ref).max() < 1e-6, (n, sign, "one")
+                assert np.array_equal(d[:, 2:4], d[:, 6:8])
 
 
 def test_sweep_rejects_bad_command_lines(tmp_path):
