@@ -78,15 +78,16 @@ __device__ __forceinline__ void rk4_adj3(float2 P, float2 S, float2 &lP, float2 
     lS = add2(ybs, rs);
 }
 
-// The adjoint row passes are wide: one row per workgroup of W waves; the first N/E lanes
-// (at most one wave) run the transforms and every thread steps N/(64 W) cells of the row
-// between them, so the recomputed RK4 stages and their transposes spread over W times as
-// many wavefronts as the transforms need.
+// Layouts of the tape and adjoint passes (split layouts, as the forward's): a row per
+// workgroup of W >= 2 waves, loaded and stored one cell per thread through LDS, the two
+// fields transformed by waves 0 and 1, the per-cell work (RK4^T with its recomputed
+// stages and FP64 accumulators) on every thread; CW columns per workgroup of 2*CW waves.
 template <int N> struct AdjWide;
-template <> struct AdjWide<64>  { static constexpr int E = 4, W = 1; };
+template <> struct AdjWide<64>  { static constexpr int E = 4, W = 2; };
 template <> struct AdjWide<128> { static constexpr int E = 4, W = 2; };
 template <> struct AdjWide<256> { static constexpr int E = 4, W = 4; };
 template <> struct AdjWide<512> { static constexpr int E = 8, W = 8; };
+template <int N> struct AdjCol { static constexpr int E = N == 512 ? 8 : 4, CW = 2; };
 
 // The plain build's RK4^T (one coefficient for all stages).
 __device__ __forceinline__ void rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, float4 c, const NlConst &k,

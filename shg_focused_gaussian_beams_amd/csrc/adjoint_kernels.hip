@@ -50,158 +50,142 @@ __device__ __forceinline__ void block_store(AdjAcc a, double *out) {
 }
 
 // ---------------------------------------------------------------------------
-// Layouts of the tape and adjoint passes: four elements per thread, which gives the most
-// wavefronts per pass (one per 256-point row) and leaves room for the adjoint's
-// recomputed RK4 stages and FP64 accumulators.
-template <int N> struct AdjLayout;
-template <> struct AdjLayout<64>  { using Row = RowCfgT<64, 4, 1>;  using Col = ColCfgT<64, 4, 8>; };
-template <> struct AdjLayout<128> { using Row = RowCfgT<128, 4, 1>; using Col = ColCfgT<128, 4, 8>; };
-template <> struct AdjLayout<256> { using Row = RowCfgT<256, 4, 1>; using Col = ColCfgT<256, 4, 8>; };
-template <> struct AdjLayout<512> { using Row = RowCfgT<512, 4, 2>; using Col = ColCfgT<512, 4, 4>; };
-
-
-
 // Forward row pass with tape (single instance, modes as RowMode without AUTO).
 // START at s0: ckpt = input (if ckpt is set), then RK4_s0, R_bwd.
 // MID at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s, RK4_{s+1}, R_bwd.
 // END at s   : R_fwd, tape_a2[s-s0] = A''_s, RK4_s  (real-space state of the segment end).
 template <int N>
-__global__ __launch_bounds__(AdjLayout<N>::Row::THREADS) void k_row_tape(const InstDesc *__restrict__ descs,
-                                                                          const float2 *__restrict__ twg, int step,
-                                                                          int mode, TapeArgs tp) {
-    using C = typename AdjLayout<N>::Row;
-    constexpr int E = C::E, T = C::T;
-    __shared__ float2 lds[2][C::ROWS][C::L];
+__global__ __launch_bounds__(64 * AdjWide<N>::W) void k_row_tape(const InstDesc *__restrict__ descs,
+                                                                const float2 *__restrict__ twg, int step, int mode,
+                                                                TapeArgs tp) {
+    constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
+    static_assert(W >= 2 && T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
-    const int lane = threadIdx.x;
-    const int t = lane % T, rr = lane / T;
-    const int row = blockIdx.x * C::ROWS + rr;
-    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
-    __syncthreads();
-    const size_t n2 = (size_t)N * N, base = (size_t)row * N;
-    float2 fa[E], fb[E];
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-        const float4 v = d.U[base + t + j * T];
-        fa[j] = make_float2(v.x, v.y);
-        fb[j] = make_float2(v.z, v.w);
-    }
-    auto sync = [] { __syncthreads(); };
+    const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
     const int ls = step - tp.s0;
-    if (mode == ROW_START) {
-        if (tp.ckpt)
-#pragma unroll
-            for (int j = 0; j < E; j++) tp.ckpt[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
-    } else {
-        fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
-        const float4 c = d.coef[step];
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            if (tp.a2) tp.a2[(size_t)ls * n2 + base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
-            rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    for (int c = tid; c < N; c += THREADS) {
+        const float4 v = d.U[base + c];
+        lds[0][lds_pad(c)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(c)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+    if (mode != ROW_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
+    __syncthreads();
+    const float4 c0 = d.coef[step];
+    const float4 c1 = d.coef[mode == ROW_MID ? step + 1 : step];
+    for (int c = tid; c < N; c += THREADS) {
+        float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+        if (mode == ROW_START) {
+            if (tp.ckpt) tp.ckpt[base + c] = make_float4(P.x, P.y, S.x, S.y);
+        } else {
+            if (tp.a2) tp.a2[(size_t)ls * n2 + base + c] = make_float4(P.x, P.y, S.x, S.y);
+            rk4_cell(P, S, c0, c0, c0, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
         }
+        if (mode != ROW_END) rk4_cell(P, S, c1, c1, c1, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
+        lds[0][lds_pad(c)] = P;
+        lds[1][lds_pad(c)] = S;
     }
-    if (mode != ROW_END) {
-        const float4 c = d.coef[mode == ROW_START ? step : step + 1];
-#pragma unroll
-        for (int j = 0; j < E; j++) rk4_cell(fa[j], fb[j], c, c, c, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);
-        fft2fe<N, E, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
+    __syncthreads();
+    if (mode != ROW_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
+    __syncthreads();
+    for (int c = tid; c < N; c += THREADS) {
+        const float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
+        d.U[base + c] = make_float4(P.x, P.y, S.x, S.y);
     }
-#pragma unroll
-    for (int j = 0; j < E; j++) d.U[base + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+}
+
+// Columns x0 .. x0+CW-1 of U into LDS (field k/CW of column k%CW at lds[k]).
+template <int N, int CW, int THREADS>
+__device__ __forceinline__ void cols_to_lds(const float4 *__restrict__ U, int x0, float2 (*lds)[LdsLen<N>::value]) {
+    for (int i = threadIdx.x; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW;
+        const float4 v = U[(size_t)y * N + x0 + c];
+        lds[c][lds_pad(y)] = make_float2(v.x, v.y);
+        lds[CW + c][lds_pad(y)] = make_float2(v.z, v.w);
+    }
+}
+template <int N, int CW, int THREADS>
+__device__ __forceinline__ void lds_to_cols(float4 *__restrict__ U, int x0, float2 (*lds)[LdsLen<N>::value]) {
+    for (int i = threadIdx.x; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW;
+        const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
+        U[(size_t)y * N + x0 + c] = make_float4(P.x, P.y, S.x, S.y);
+    }
 }
 
 // Forward column pass with tape: tape_spec[s-s0] = the 2D spectrum before x H.
 template <int N>
-__global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_col_tape(const InstDesc *__restrict__ descs,
-                                                                          const float2 *__restrict__ twg, int step,
-                                                                          TapeArgs tp) {
-    using C = typename AdjLayout<N>::Col;
-    constexpr int E = C::E, T = C::T, CW = C::CW;
-    __shared__ float2 lds[2][C::L * CW];
+__global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc *__restrict__ descs,
+                                                                 const float2 *__restrict__ twg, int step, TapeArgs tp) {
+    constexpr int E = AdjCol<N>::E, CW = AdjCol<N>::CW, T = N / E, THREADS = 128 * CW, L = LdsLen<N>::value;
+    static_assert(T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
-    const int tid = threadIdx.x;
-    const int c = tid % CW, t = tid / CW;
-    const int x = blockIdx.x * CW + c;
-    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
+    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = blockIdx.x * CW;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
     __syncthreads();
-    float2 fa[E], fb[E];
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-        const float4 v = d.U[(size_t)(t + j * T) * N + x];
-        fa[j] = make_float2(v.x, v.y);
-        fb[j] = make_float2(v.z, v.w);
-    }
-    auto sync = [] { __syncthreads(); };
-    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
+    __syncthreads();
     const size_t soff = (size_t)(step - tp.s0) * N * N;
-    const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-        const int y = t + j * T;
-        if (tp.spec) tp.spec[soff + (size_t)y * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
-        fa[j] = c_mul(c_mul(gxp, d.G[N + y]), fa[j]);
-        fb[j] = c_mul(c_mul(gxs, d.G[3 * N + y]), fb[j]);
+    for (int i = tid; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW, x = x0 + c;
+        const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
+        if (tp.spec) tp.spec[soff + (size_t)y * N + x] = make_float4(P.x, P.y, S.x, S.y);
+        lds[c][lds_pad(y)] = c_mul(c_mul(d.G[x], d.G[N + y]), P);
+        lds[CW + c][lds_pad(y)] = c_mul(c_mul(d.G[2 * N + x], d.G[3 * N + y]), S);
     }
-    fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
-#pragma unroll
-    for (int j = 0; j < E; j++)
-        d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    __syncthreads();
+    if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
+    __syncthreads();
+    lds_to_cols<N, CW, THREADS>(d.U, x0, lds);
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's),
-// wide layout (AdjWide).
+// Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's).
 // ADJ_START at s: RK4_s^T at A''_s, R_bwd.
 // ADJ_MID   at s: R_fwd, RK4_s^T at A_s = RK4_{s-1}(A''_{s-1}), RK4_{s-1}^T at A''_{s-1}, R_bwd.
 // ADJ_END   at s: R_fwd, RK4_s^T at A_s = tp.a (the segment's checkpoint); leaves the
 //                 real-space lambda of the segment start.
 // g1[(s*N + row)*5] gets the row's sensitivities of the first RK4 of step s (at A_s),
-// g2[...] those of the second (at A''_s).  One cell per thread between the transforms.
+// g2[...] those of the second (at A''_s).
 template <int N>
 __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *__restrict__ descs,
                                                                const float2 *__restrict__ twg, int step, int mode,
                                                                TapeArgs tp, double *__restrict__ g1,
                                                                double *__restrict__ g2) {
     constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
-    static_assert(T <= 64 && N == THREADS, "the transforms run in one wavefront; one cell per thread");
+    static_assert(W >= 2 && T <= 64 && THREADS >= N, "one wavefront per field transform; one cell per thread");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
-    const int tid = threadIdx.x;
-    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + tid;
+    const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
+    const bool cell = tid < N;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + (cell ? tid : 0);
     const int ls = step - tp.s0;
     // the cell's taped forward state, loaded before the transform hides its latency:
     // START: A''_s;  MID: A''_{s-1};  END: A_s (the checkpoint)
     const float4 tape = mode == ADJ_END ? tp.a[idx] : tp.a2[(size_t)(mode == ADJ_START ? ls : ls - 1) * n2 + idx];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
-    __syncthreads();
-    auto sync = [] { wave_sync(); };
-    if (tid < T) {
-        float2 fa[E], fb[E];
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            const float4 v = d.U[base + tid + j * T];
-            fa[j] = make_float2(v.x, v.y);
-            fb[j] = make_float2(v.z, v.w);
-        }
-        if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            lds[0][lds_pad(tid + j * T)] = fa[j];
-            lds[1][lds_pad(tid + j * T)] = fb[j];
-        }
+    if (cell) {
+        const float4 v = d.U[idx];
+        lds[0][lds_pad(tid)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(tid)] = make_float2(v.z, v.w);
     }
+    __syncthreads();
+    if (mode != ADJ_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
     __syncthreads();
     AdjAcc acc1, acc2;
     acc_zero(acc1);
     acc_zero(acc2);
     const float4 c_s = d.coef[step];
     const float4 c_p = d.coef[step > 0 ? step - 1 : 0];
-    {
+    if (cell) {
         const int o = lds_pad(tid);
         float2 P = lds[0][o], S = lds[1][o];
         if (mode == ADJ_START) {
@@ -224,16 +208,11 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
         block_store<W>(acc2, g2 + ((size_t)(step - 1) * N + blockIdx.x) * 5);
     }
     __syncthreads();
-    if (tid < T) {
-        float2 fa[E], fb[E];
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            fa[j] = lds[0][lds_pad(tid + j * T)];
-            fb[j] = lds[1][lds_pad(tid + j * T)];
-        }
-        if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
-#pragma unroll
-        for (int j = 0; j < E; j++) d.U[base + tid + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    if (mode != ADJ_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
+    __syncthreads();
+    if (cell) {
+        const float2 P = lds[0][lds_pad(tid)], S = lds[1][lds_pad(tid)];
+        d.U[idx] = make_float4(P.x, P.y, S.x, S.y);
     }
 }
 
@@ -241,59 +220,48 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
 // sum Re(conj(lambda_hat) (Gx Gy) x_hat * i (a_p, a_s) (qx + qy)) over the block's cells,
 // the sensitivity of J to a temperature change of the propagator (d log H/dT = i a q).
 template <int N>
-__global__ __launch_bounds__(AdjLayout<N>::Col::THREADS) void k_adj_col(const InstDesc *__restrict__ descs,
-                                                                         const float2 *__restrict__ twg, int step,
-                                                                         TapeArgs tp, const float2 *__restrict__ qxy,
-                                                                         float a_p, float a_s,
-                                                                         double *__restrict__ gH) {
-    using C = typename AdjLayout<N>::Col;
-    constexpr int E = C::E, T = C::T, CW = C::CW;
-    __shared__ float2 lds[2][C::L * CW];
+__global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc *__restrict__ descs,
+                                                                const float2 *__restrict__ twg, int step, TapeArgs tp,
+                                                                const float2 *__restrict__ qxy, float a_p, float a_s,
+                                                                double *__restrict__ gH) {
+    constexpr int E = AdjCol<N>::E, CW = AdjCol<N>::CW, T = N / E, THREADS = 128 * CW, L = LdsLen<N>::value;
+    static_assert(T <= 64, "one wavefront per field transform");
+    __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
-    __shared__ double red[C::THREADS / 64];
+    __shared__ double red[THREADS / 64];
     const InstDesc d = descs[0];
-    const int tid = threadIdx.x;
-    const int c = tid % CW, t = tid / CW;
-    const int x = blockIdx.x * CW + c;
-    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
+    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = blockIdx.x * CW;
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
     __syncthreads();
-    float2 fa[E], fb[E];
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-        const float4 v = d.U[(size_t)(t + j * T) * N + x];
-        fa[j] = make_float2(v.x, v.y);
-        fb[j] = make_float2(v.z, v.w);
-    }
-    auto sync = [] { __syncthreads(); };
-    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
+    __syncthreads();
     const size_t soff = (size_t)(step - tp.s0) * N * N;
-    const float2 gxp = d.G[x], gxs = d.G[2 * N + x];
-    const float qx = qxy[x].x;
     double gsum = 0.0;
-#pragma unroll
-    for (int j = 0; j < E; j++) {
-        const int y = t + j * T;
-        const float q = qx + qxy[y].y;
-        const float2 hp = c_mul(gxp, d.G[N + y]), hs = c_mul(gxs, d.G[3 * N + y]);
+    for (int i = tid; i < CW * N; i += THREADS) {
+        const int y = i / CW, c = i % CW, x = x0 + c;
+        const float q = qxy[x].x + qxy[y].y;
+        const float2 hp = c_mul(d.G[x], d.G[N + y]), hs = c_mul(d.G[2 * N + x], d.G[3 * N + y]);
         const float4 xs = tp.spec[soff + (size_t)y * N + x];
+        const float2 lp = lds[c][lds_pad(y)], lsg = lds[CW + c][lds_pad(y)];
         // Re(conj(l) * h * xs * i a q) = -a q Im(conj(l) h xs)
-        const float2 tp_ = c_mul(c_mul(cj(fa[j]), hp), f2(xs.x, xs.y));
-        const float2 ts_ = c_mul(c_mul(cj(fb[j]), hs), f2(xs.z, xs.w));
+        const float2 tp_ = c_mul(c_mul(cj(lp), hp), f2(xs.x, xs.y));
+        const float2 ts_ = c_mul(c_mul(cj(lsg), hs), f2(xs.z, xs.w));
         gsum += -(double)(a_p * q) * (double)tp_.y - (double)(a_s * q) * (double)ts_.y;
-        fa[j] = c_mul(cj(hp), fa[j]);
-        fb[j] = c_mul(cj(hs), fb[j]);
+        lds[c][lds_pad(y)] = c_mul(cj(hp), lp);
+        lds[CW + c][lds_pad(y)] = c_mul(cj(hs), lsg);
     }
-    fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
-#pragma unroll
-    for (int j = 0; j < E; j++)
-        d.U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    __syncthreads();
+    if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
+    __syncthreads();
+    lds_to_cols<N, CW, THREADS>(d.U, x0, lds);
     for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = gsum;
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
-        for (int w = 0; w < C::THREADS / 64; w++) s += red[w];
-        gH[(size_t)step * gridDim.x + blockIdx.x] = s;
+        double sum = 0.0;
+        for (int w = 0; w < THREADS / 64; w++) sum += red[w];
+        gH[(size_t)step * gridDim.x + blockIdx.x] = sum;
     }
 }
 
@@ -335,8 +303,7 @@ __global__ __launch_bounds__(64) void k_adj_reduce(const double *__restrict__ g1
 }
 
 // ---------------------------------------------------------------------------
-template <int N> static int row_blocks() { return N / AdjLayout<N>::Row::ROWS; }
-template <int N> static int col_blocks() { return N / AdjLayout<N>::Col::CW; }
+template <int N> static int col_blocks() { return N / AdjCol<N>::CW; }
 
 #define SHG_DISPATCH(n, CALL)                \
     switch (n) {                             \
@@ -361,14 +328,14 @@ int adj_col_parts(int n) {
 hipError_t launch_row_tape(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
                            hipStream_t st) {
     if (mode != ROW_START && mode != ROW_MID && mode != ROW_END) return hipErrorInvalidValue;
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_row_tape<NN>, dim3(row_blocks<NN>()), dim3(AdjLayout<NN>::Row::THREADS), 0,
-                                       st, d, tw, step, mode, tp));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_row_tape<NN>, dim3(NN), dim3(64 * AdjWide<NN>::W), 0, st, d, tw, step, mode,
+                                       tp));
     return hipGetLastError();
 }
 
 hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(AdjLayout<NN>::Col::THREADS), 0,
-                                       st, d, tw, step, tp));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0, st, d,
+                                       tw, step, tp));
     return hipGetLastError();
 }
 
@@ -382,8 +349,8 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(AdjLayout<NN>::Col::THREADS), 0,
-                                       st, d, tw, step, tp, qxy, a_p, a_s, gH));
+    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0, st, d,
+                                       tw, step, tp, qxy, a_p, a_s, gH));
     return hipGetLastError();
 }
 

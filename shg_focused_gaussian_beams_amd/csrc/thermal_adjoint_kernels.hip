@@ -62,12 +62,13 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
                                                                      const float2 *__restrict__ twg, int mode,
                                                                      ThermAdjArgs a) {
     constexpr int E = AdjWide<N>::E, W = AdjWide<N>::W, T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
-    static_assert(T <= 64 && N == THREADS, "the transforms run in one wavefront; one cell per thread");
+    static_assert(W >= 2 && T <= 64 && THREADS >= N, "one wavefront per field transform; one cell per thread");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
     float4 *U = descs[0].U;
-    const int tid = threadIdx.x;
-    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + tid;
+    const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
+    const bool cell = tid < N;
+    const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + (cell ? tid : 0);
     const int ls = a.cur.step - a.s0;
     // the cell's inputs, loaded before the transform hides their latency
     const float2 nn_c = a.NT[(size_t)a.cur.slice * n2 + idx];
@@ -88,26 +89,15 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
     }
     double G = a.G[idx], Dp = a.Dp[idx];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
-    __syncthreads();
-    auto sync = [] { wave_sync(); };
-    if (tid < T) {
-        float2 fa[E], fb[E];
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            const float4 v = U[base + tid + j * T];
-            fa[j] = make_float2(v.x, v.y);
-            fb[j] = make_float2(v.z, v.w);
-        }
-        if (mode != ADJ_START) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            lds[0][lds_pad(tid + j * T)] = fa[j];
-            lds[1][lds_pad(tid + j * T)] = fb[j];
-        }
+    if (cell) {
+        const float4 v = U[idx];
+        lds[0][lds_pad(tid)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(tid)] = make_float2(v.z, v.w);
     }
     __syncthreads();
-    {
+    if (mode != ADJ_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
+    __syncthreads();
+    if (cell) {
         const ThermConst &k = a.k;
         // d dk / d n_p = 4 pi / lp, d dk / d n_s = -2 pi / ls (src/headers/PhaseMatching.h:47)
         const double wp = 4.0 * (double)k.pi / (double)k.lp, ws = -2.0 * (double)k.pi / (double)k.ls;
@@ -148,16 +138,11 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
         a.Dp[idx] = Dp;
     }
     __syncthreads();
-    if (tid < T) {
-        float2 fa[E], fb[E];
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            fa[j] = lds[0][lds_pad(tid + j * T)];
-            fb[j] = lds[1][lds_pad(tid + j * T)];
-        }
-        if (mode != ADJ_END) fft2fe<N, E, +1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
-#pragma unroll
-        for (int j = 0; j < E; j++) U[base + tid + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+    if (mode != ADJ_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
+    __syncthreads();
+    if (cell) {
+        const float2 P = lds[0][lds_pad(tid)], S = lds[1][lds_pad(tid)];
+        U[idx] = make_float4(P.x, P.y, S.x, S.y);
     }
 }
 
