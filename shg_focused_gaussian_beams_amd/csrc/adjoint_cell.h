@@ -3,8 +3,9 @@
 //
 // The per-cell maps are not holomorphic (conj(P), |S|^2), so each stage uses the real
 // adjoint zbar = conj(dg/dz) gbar + (dg/dconj z) conj(gbar) with the gradient convention
-// dJ = Re sum conj(lambda) dA.  FP64 accumulators collect the sensitivities of J to the
-// coefficients: dJ = Re(conj(c1b) dc1) + Re(conj(c2b) dc2) + tpab dtpa.
+// dJ = Re sum conj(lambda) dA.  Accumulators collect the sensitivities of J to the
+// coefficients, dJ = Re(conj(c1b) dc1) + Re(conj(c2b) dc2) + tpab dtpa: in float over the
+// four stages of one RK4 of one cell, in FP64 beyond.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,9 +26,21 @@ struct AdjAcc {
 
 __device__ __forceinline__ void acc_zero(AdjAcc &a) { a.c1r = a.c1i = a.c2r = a.c2i = a.tpa = 0.0; }
 
+// Per-cell, per-RK4 sums of the coefficient sensitivities (four stage terms at most; the
+// caller folds them into the FP64 AdjAcc once per RK4).
+struct AdjAccF {
+    float c1r, c1i, c2r, c2i, tpa;
+};
+
+__device__ __forceinline__ void accf_zero(AdjAccF &a) { a.c1r = a.c1i = a.c2r = a.c2i = a.tpa = 0.0f; }
+__device__ __forceinline__ void acc_add(AdjAcc &a, const AdjAccF &b) {
+    a.c1r += (double)b.c1r; a.c1i += (double)b.c1i; a.c2r += (double)b.c2r; a.c2i += (double)b.c2i;
+    a.tpa += (double)b.tpa;
+}
+
 // Adjoint of the right-hand side f(P, S) (nl_rhs) at (P, S) applied to (gP, gS).
 __device__ __forceinline__ void rhs_adj(float2 P, float2 S, float2 gP, float2 gS, float4 c, float lossP, float lossS,
-                                        float tpa, float2 &Pb, float2 &Sb, AdjAcc &acc) {
+                                        float tpa, float2 &Pb, float2 &Sb, AdjAccF &acc) {
     const float2 c1 = f2(c.x, c.y), c2 = f2(c.z, c.w);
     const float s2 = S.x * S.x + S.y * S.y;
     // Pb = -lossP gP + c1 S conj(gP) + 2 conj(c2) conj(P) gS
@@ -39,15 +52,15 @@ __device__ __forceinline__ void rhs_adj(float2 P, float2 S, float2 gP, float2 gS
     const float2 u = c_mul(cj(S), P);
     const float2 w = c_mul(cj(c_mul(P, P)), gS);
     const float2 v = c_mul(u, gP);
-    acc.c1r += (double)v.x; acc.c1i += (double)v.y;
-    acc.c2r += (double)w.x; acc.c2i += (double)w.y;
-    acc.tpa += -(double)s2 * ((double)gS.x * S.x + (double)gS.y * S.y);
+    acc.c1r += v.x; acc.c1i += v.y;
+    acc.c2r += w.x; acc.c2i += w.y;
+    acc.tpa += -s2 * (gS.x * S.x + gS.y * S.y);
 }
 
 // RK4^T (rk4_cell) at forward input (P, S) with stage coefficients ca (stage 1), cb
 // (stages 2-3), cc (stage 4): (lP, lS) is the adjoint of the output on entry and of the
 // input on exit.  The sensitivities of the three coefficients go to a1, a23, a4 (one
-// accumulator may be passed for all three).
+// accumulator may be passed for all three), summed in float over the stages of this RK4.
 __device__ __forceinline__ void rk4_adj3(float2 P, float2 S, float2 &lP, float2 &lS, float4 ca, float4 cb, float4 cc,
                                          float lossP, float lossS, float tpa, float h6, AdjAcc &a1, AdjAcc &a23,
                                          AdjAcc &a4) {
@@ -64,18 +77,25 @@ __device__ __forceinline__ void rk4_adj3(float2 P, float2 S, float2 &lP, float2 
     float2 kb3p = kb2p, kb3s = kb2s;
     const float2 kb4p = kb1p, kb4s = kb1s;
     float2 rp, rs;
-    rhs_adj(y4p, y4s, kb4p, kb4s, cc, lossP, lossS, tpa, rp, rs, a4);
+    AdjAccF f1, f23, f4;
+    accf_zero(f1);
+    accf_zero(f23);
+    accf_zero(f4);
+    rhs_adj(y4p, y4s, kb4p, kb4s, cc, lossP, lossS, tpa, rp, rs, f4);
     ybp = add2(ybp, rp); ybs = add2(ybs, rs);
     kb3p = add2(kb3p, rp); kb3s = add2(kb3s, rs);
-    rhs_adj(y3p, y3s, kb3p, kb3s, cb, lossP, lossS, tpa, rp, rs, a23);
+    rhs_adj(y3p, y3s, kb3p, kb3s, cb, lossP, lossS, tpa, rp, rs, f23);
     ybp = add2(ybp, rp); ybs = add2(ybs, rs);
     kb2p = axpy2(0.5f, rp, kb2p); kb2s = axpy2(0.5f, rs, kb2s);
-    rhs_adj(y2p, y2s, kb2p, kb2s, cb, lossP, lossS, tpa, rp, rs, a23);
+    rhs_adj(y2p, y2s, kb2p, kb2s, cb, lossP, lossS, tpa, rp, rs, f23);
     ybp = add2(ybp, rp); ybs = add2(ybs, rs);
     kb1p = axpy2(0.5f, rp, kb1p); kb1s = axpy2(0.5f, rs, kb1s);
-    rhs_adj(P, S, kb1p, kb1s, ca, lossP, lossS, tpa, rp, rs, a1);
+    rhs_adj(P, S, kb1p, kb1s, ca, lossP, lossS, tpa, rp, rs, f1);
     lP = add2(ybp, rp);
     lS = add2(ybs, rs);
+    acc_add(a1, f1);
+    acc_add(a23, f23);
+    acc_add(a4, f4);
 }
 
 // Layouts of the tape and adjoint passes (split layouts, as the forward's): a row per
