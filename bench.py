@@ -29,6 +29,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Kernel of each tuned layout at 256 x 256 (the launch tables of ssfm_kernels.hip and
+# thermal_kernels.hip), for the roofline line's label.
+THERM_ROW_256 = {0: "k_therm_row_pass<256,16,1>", 1: "k_therm_row_pass<256,8,1>", 2: "k_therm_row_pass<256,4,1>",
+                 3: "k_therm_row_pass<256,4,4>", 4: "k_therm_row_wide<256,4,4>", 5: "k_therm_row_wide<256,4,2>",
+                 6: "k_therm_row_split<256,4,4>"}
+COL_256 = {0: "k_col_pass<256,16,8>", 1: "k_col_pass<256,8,8>", 2: "k_col_pass<256,4,8>", 3: "k_col_pass<256,4,1>",
+           4: "k_col_pass<256,4,2>", 5: "k_col_split<256,4,1>", 6: "k_col_split<256,4,2>",
+           7: "k_col_split<256,4,4>"}
+
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256) from rocprofv3 --pmc FETCH_SIZE
@@ -127,6 +136,7 @@ def bench_thermal(shg, args):
     wall = time.perf_counter() - t0
     cells = ctx.cell_steps()
     _, _, eta = ctx.powers()
+    variants = ctx.variants()
     row_ms, col_ms = ctx.profile_kernels(args.profile_reps)
     # Jacobi throughput on the heat grid
     nj = 200
@@ -137,7 +147,8 @@ def bench_thermal(shg, args):
     vox = p.nx * p.ny * p.heat_nz
     ctx.close()
     return dict(p=p, T=T, fh0=fh0, sh0=sh0, wall=wall, steps=args.steps, dev_ms=dev / args.steps, cells=cells,
-                eta=eta, row_ms=row_ms, col_ms=col_ms, stats=st, loop_s=loop_s, jac_s=jac_s, vox=vox)
+                eta=eta, row_ms=row_ms, col_ms=col_ms, stats=st, loop_s=loop_s, jac_s=jac_s, vox=vox,
+                variants=variants)
 
 
 def bench_plain(shg, args, p):
@@ -153,10 +164,11 @@ def bench_plain(shg, args, p):
         ctx.forward()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
+    variants = ctx.variants()
     row_ms, col_ms = ctx.profile_kernels(args.profile_reps)
     cells = ctx.cell_steps()
     ctx.close()
-    return dict(s_per_solve=wall, cells=cells, row_ms=row_ms, col_ms=col_ms)
+    return dict(s_per_solve=wall, cells=cells, row_ms=row_ms, col_ms=col_ms, variants=variants)
 
 
 def bench_batch(shg, args):
@@ -351,7 +363,9 @@ def main():
                    "thermal_loop": {**th["stats"], "wall_s": th["loop_s"]},
                    "jacobi_voxel_sweeps_per_s": th["vox"] / th["jac_s"],
                    "jacobi_GBps": th["vox"] * B_JACOBI / th["jac_s"] / 1e9},
-        "roofline": {"bound": "hbm", "kernel": "fused THERMAL z-step = k_therm_row_wide + k_col_pass (256x256)",
+        "roofline": {"bound": "hbm", "kernel": "fused THERMAL z-step = %s + %s (256x256)" % (
+                         THERM_ROW_256.get(th["variants"][0], "?"), COL_256.get(th["variants"][1], "?")),
+                     "variants": th["variants"],
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": TRAFFIC_THERMAL_STEP,
                      "bytes_per_cell_step": B_THERMAL, "step_us": step_s * 1e6,
@@ -363,7 +377,7 @@ def main():
         pl = bench_plain(shg, args, shg.config2_params(thermal=0))
         out["config"]["plain_256x256_nz400"] = {
             "cell_steps_per_s": pl["cells"] / pl["s_per_solve"], "ms_per_solve": pl["s_per_solve"] * 1e3,
-            "row_pass_us": pl["row_ms"] * 1e3, "col_pass_us": pl["col_ms"] * 1e3,
+            "row_pass_us": pl["row_ms"] * 1e3, "col_pass_us": pl["col_ms"] * 1e3, "variants": pl["variants"],
             "step_GBps_algorithmic": B_PLAIN * 65536 / ((pl["row_ms"] + pl["col_ms"]) * 1e-3) / 1e9}
         log("batched config 3 ...")
         bt = bench_batch(shg, args)

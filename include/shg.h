@@ -246,6 +246,8 @@ int shg_batch_last_ms(shg_batch *b, double *ms);
  * choice.  Rows 0..4 (THERMAL contexts: 0..6), columns 0..7. */
 int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
 int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
+/* The layouts the next march runs (-1 while untuned). */
+int shg_get_variants(const shg_ctx *ctx, int *row_variant, int *col_variant);
 
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass (the tuned layouts), each replayed `reps` times back to back from a

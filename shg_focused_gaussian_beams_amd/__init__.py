@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
         "shg_adjoint_temperature_map": (ctypes.c_int, [vp, dp]),
         "shg_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "shg_get_variants": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
         "shg_batch_destroy": (None, [vp]),
         "shg_batch_init_fields": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint64]),
@@ -202,6 +203,12 @@ class Context:
 
     def set_variants(self, row_variant, col_variant):
         _check(self.lib.shg_set_variants(self.h, row_variant, col_variant))
+
+    def variants(self):
+        """(row, column) layouts the next march runs; (-1, -1) before tuning."""
+        r, c = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.shg_get_variants(self.h, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
 
     def adjoint(self, checkpoint_interval=0, want_field=True):
         """Efficiency and its gradients (discrete adjoint).  Returns (result dict,

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
     __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
     const InstDesc d = descs[0];
-    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = blockIdx.x * CW;
+    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
     __syncthreads();
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     __shared__ float2 tw[N];
     __shared__ double red[THREADS / 64];
     const InstDesc d = descs[0];
-    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = blockIdx.x * CW;
+    const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
     __syncthreads();

@@ -8,6 +8,14 @@
 
 namespace shg {
 
+// Workgroups are handed to the 8 XCDs round-robin (XCD = linear workgroup id mod 8), and
+// every XCD has its own L2.  Workgroups whose lines share 128-byte cache lines (adjacent
+// column groups) are given consecutive logical indices on the same XCD: logical group
+// (b mod 8) * (nb / 8) + b / 8 for workgroup b of nb (identity unless 8 divides nb).
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 template <int N, int E_, int W> struct RowCfgT {
     static constexpr int E = E_;
     static constexpr int T = N / E;

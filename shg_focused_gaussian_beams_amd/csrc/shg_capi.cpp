@@ -576,6 +576,13 @@ int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
     return SHG_OK;
 }
 
+int shg_get_variants(const shg_ctx *c, int *row_variant, int *col_variant) {
+    if (!c || !row_variant || !col_variant) return fail(SHG_E_INVALID, "shg_get_variants: bad argument");
+    *row_variant = c->row_v;
+    *col_variant = c->col_v;
+    return SHG_OK;
+}
+
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
     if (c->p.thermal)

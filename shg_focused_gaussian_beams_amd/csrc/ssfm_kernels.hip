@@ -83,7 +83,7 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
     const float2 *__restrict__ G = d.G;
     const int tid = threadIdx.x;
     const int c = tid % CW, t = tid / CW;
-    const int x = blockIdx.x * CW + c;
+    const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
     for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
     __syncthreads();
     float2 fa[E], fb[E];
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(128 * CW) void k_col_split(const InstDesc *__restri
     float4 *__restrict__ U = d.U;
     const float2 *__restrict__ G = d.G;
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63;
-    const int x0 = blockIdx.x * CW;
+    const int x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int i = tid; i < CW * N; i += THREADS) {
         const int y = i / CW, c = i % CW;
