@@ -28,6 +28,13 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st);
 hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st);
+// d eta / d A0 = lambda_FH + direct * A0_FH with direct = -2 cp J / P_in and
+// J = cs * red->y / P_in (red: the FP64 output power sums of the forward), written to
+// grad (float2 per cell); part[b] = block sums of Re(conj(g) pdf), Re(conj(g) pdn) (FP64,
+// with g in FP64) for the focus and pump-index sensitivities.  nparts blocks of 256.
+hipError_t launch_field_grad(const float4 *lam, const float4 *U0, const double2 *red, double cs, double cp, double pin,
+                             const double2 *pdf, const double2 *pdn, float2 *grad, double2 *part, int nparts,
+                             int64_t n, hipStream_t st);
 hipError_t launch_copy_cells(float4 *dst, const float4 *src, int64_t n, hipStream_t st);
 // Per-step sums of the gradient partials: out[s*6 + 0..4] from g1 + g2, out[s*6 + 5] from gH.
 hipError_t launch_adj_reduce(const double *g1, const double *g2, int rparts, const double *gH, int cparts, int steps,

@@ -194,9 +194,10 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
             rk4_adj(f2(tape.x, tape.y), f2(tape.z, tape.w), P, S, c_s, d.nl, acc1);
         } else {
             float2 Ap = f2(tape.x, tape.y), As = f2(tape.z, tape.w);
-            rk4_cell(Ap, As, c_p, c_p, c_p, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6);   // A_s
+            Rk4Stages kept;
+            rk4_keep(Ap, As, c_p, c_p, c_p, d.nl.lossP, d.nl.lossS, d.nl.tpa, d.nl.h6, kept);   // A_s
             rk4_adj(Ap, As, P, S, c_s, d.nl, acc1);
-            rk4_adj(f2(tape.x, tape.y), f2(tape.z, tape.w), P, S, c_p, d.nl, acc2);
+            rk4_adj_kept(f2(tape.x, tape.y), f2(tape.z, tape.w), kept, P, S, c_p, d.nl, acc2);
         }
         lds[0][o] = P;
         lds[1][o] = S;
@@ -273,6 +274,31 @@ __global__ __launch_bounds__(256) void k_adj_seed(const float4 *__restrict__ U, 
     if (i >= n) return;
     const float4 v = U[i];
     L[i] = make_float4(0.0f, 0.0f, scale * v.z, scale * v.w);
+}
+
+__global__ __launch_bounds__(256) void k_field_grad(const float4 *__restrict__ lam, const float4 *__restrict__ U0,
+                                                    const double2 *__restrict__ red, double cs, double cp, double pin,
+                                                    const double2 *__restrict__ pdf, const double2 *__restrict__ pdn,
+                                                    float2 *__restrict__ grad, double2 *__restrict__ part, int64_t n) {
+    const double J = cs * red->y / pin, direct = -J / pin * 2.0 * cp;
+    double sf = 0.0, sn = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float4 l = lam[i], u = U0[i];
+        const double gr = (double)l.x + direct * (double)u.x, gi = (double)l.y + direct * (double)u.y;
+        grad[i] = make_float2((float)gr, (float)gi);
+        const double2 f = pdf[i], m = pdn[i];
+        sf += gr * f.x + gi * f.y;
+        sn += gr * m.x + gi * m.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sf += __shfl_down(sf, off, 64);
+        sn += __shfl_down(sn, off, 64);
+    }
+    __shared__ double2 w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = make_double2(sf, sn);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = make_double2(w[0].x + w[1].x + w[2].x + w[3].x, w[0].y + w[1].y + w[2].y + w[3].y);
 }
 
 // dst[i] = src[i] (field copies inside captured graphs: kernels, not copy nodes).
@@ -365,6 +391,14 @@ hipError_t launch_copy_cells(float4 *dst, const float4 *src, int64_t n, hipStrea
     if (n <= 0) return hipSuccess;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
     hipLaunchKernelGGL(k_copy_cells, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_grad(const float4 *lam, const float4 *U0, const double2 *red, double cs, double cp, double pin,
+                             const double2 *pdf, const double2 *pdn, float2 *grad, double2 *part, int nparts,
+                             int64_t n, hipStream_t st) {
+    if (n <= 0 || nparts <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_field_grad, dim3(nparts), dim3(256), 0, st, lam, U0, red, cs, cp, pin, pdf, pdn, grad, part, n);
     return hipGetLastError();
 }
 

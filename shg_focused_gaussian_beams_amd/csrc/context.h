@@ -28,8 +28,14 @@ struct AdjointBuffers {
     hipEvent_t ev_free[2] = {nullptr, nullptr};  // tape slot k read by the adjoint
     hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;  // captured forward / adjoint for interval k
     int g_row_v = -1, g_col_v = -1;            // plain-pass layouts the forward graph was captured with
-    std::vector<std::complex<double>> pump_df, pump_dn;   // d(pump)/d(focus), d(pump)/d(n_p) per cell
+    double2 *pump_df = nullptr, *pump_dn = nullptr;  // d(pump)/d(focus), d(pump)/d(n_p) per cell (device)
     double pump_T = -1e300;                    // temperature the pump derivative fields were built at
+    float2 *grad = nullptr;                    // d eta / d FH(z = 0) per cell
+    double2 *gpart = nullptr, *gsum = nullptr; // block sums of its pump projections and their total
+    hipEvent_t ev_end = nullptr;               // end of the adjoint (timing)
+    // pinned host staging of the results: the gradient field, then the FP64 scalars
+    void *host = nullptr;
+    size_t host_bytes = 0;
     // THERMAL (temperature held fixed)
     double *Ap = nullptr, *As = nullptr;       // per heat voxel dJ/dn_p, dJ/dn_s; Ap then holds dJ/dT
     double *Gc = nullptr, *Dc = nullptr;       // per cell DKint suffix sum and pending part

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,26 @@ namespace {
 
 using cd = std::complex<double>;
 constexpr int kMapParts = 1024;                // block sums of the THERMAL dJ/dT map
+constexpr int kGradParts = 256;                // block sums of the pump projections
+
+// Layout of the pinned result staging: the gradient field (n2 float2), the output power
+// sums, the pump projections, the dJ/dT map total, the per-step sums (steps x 6).
+struct HostOffsets {
+    size_t grad, sums, gsum, dsum, gs, end;
+};
+HostOffsets host_offsets(size_t n2, int steps) {
+    HostOffsets o;
+    o.grad = 0;
+    o.sums = n2 * sizeof(float2);
+    o.gsum = o.sums + sizeof(double2);
+    o.dsum = o.gsum + sizeof(double2);
+    o.gs = o.dsum + sizeof(double2);
+    o.end = o.gs + (size_t)steps * 6 * sizeof(double);
+    return o;
+}
+template <class T> T *host_at(AdjointBuffers &b, size_t off) {
+    return reinterpret_cast<T *>(static_cast<char *>(b.host) + off);
+}
 
 int ensure_buffers(shg_ctx *c, int K, int nseg) {
     AdjointBuffers &b = c->adj;
@@ -56,8 +77,17 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
             hipMalloc(&b.gH, (size_t)M * shg::adj_col_parts(c->n) * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.gs, (size_t)M * 6 * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.qxy, (size_t)c->n * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess)
+            hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess ||
+            hipMalloc(&b.grad, n2 * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&b.gpart, kGradParts * sizeof(double2)) != hipSuccess ||
+            hipMalloc(&b.gsum, sizeof(double2)) != hipSuccess)
             return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+        b.host_bytes = host_offsets(n2, M).end;
+        if (hipHostMalloc(&b.host, b.host_bytes, hipHostMallocDefault) != hipSuccess) {
+            b.host = nullptr;
+            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: pinned host allocation failed");
+        }
+        AHIP_TRY(hipEventCreate(&b.ev_end));
         AHIP_TRY(hipStreamCreateWithFlags(&b.ts, hipStreamNonBlocking));
         for (hipEvent_t *e : {&b.ev_u, &b.ev_tape[0], &b.ev_tape[1], &b.ev_free[0], &b.ev_free[1]})
             AHIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
@@ -112,56 +142,65 @@ double dn_dT(int crystal, double lam, double T) {
     return (shg::phys::index_d(crystal, lam, T + h) - shg::phys::index_d(crystal, lam, T - h)) / (2 * h);
 }
 
-// d eta / d A0 = lambda_0 (FH part) + the d(1/P_in)/dA0 term, written to d_eta_d_fh (if
-// set), and its projections on the derivative fields of the reference pump (setPump,
-// src/headers/Efields.h:74-104, differentiated analytically at temperature T): the
-// focus sensitivity and the temperature sensitivity through n_p(T) in Ap0 and zR.
-int field_and_pump_gradients(shg_ctx *c, double J, double pin, double cp, double np, double dnp, double T,
-                             float *d_eta_d_fh, double *d_focus, double *dT_pump) {
+// The derivative fields of the reference pump (setPump, src/headers/Efields.h:74-104,
+// differentiated analytically at temperature T) with respect to the focus and to n_p:
+// they depend on the context's parameters and T only, so they are built once per T and
+// kept on the device.
+int ensure_pump_fields(shg_ctx *c, double np, double T) {
     AdjointBuffers &b = c->adj;
     const int n = c->n;
     const size_t n2 = (size_t)n * n;
-    std::vector<float> lam(4 * n2), u0(4 * n2);
-    AHIP_TRY(hipMemcpy(lam.data(), b.lam, n2 * sizeof(float4), hipMemcpyDeviceToHost));
-    AHIP_TRY(hipMemcpy(u0.data(), c->U0, n2 * sizeof(float4), hipMemcpyDeviceToHost));
-    std::vector<cd> grad(n2);
-    const double direct = -J / pin * 2.0 * cp;
-    for (size_t i = 0; i < n2; i++) grad[i] = cd(lam[4 * i], lam[4 * i + 1]) + direct * cd(u0[4 * i], u0[4 * i + 1]);
-    if (d_eta_d_fh)
-        for (size_t i = 0; i < n2; i++) {
-            d_eta_d_fh[2 * i] = (float)grad[i].real();
-            d_eta_d_fh[2 * i + 1] = (float)grad[i].imag();
-        }
+    if (b.pump_df && b.pump_T == T) return SHG_OK;
+    if (!b.pump_df && (hipMalloc(&b.pump_df, n2 * sizeof(double2)) != hipSuccess ||
+                       hipMalloc(&b.pump_dn, n2 * sizeof(double2)) != hipSuccess))
+        return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
     const shg::phys::Consts k(c->p);
     const double eps0 = 8.8541878128e-6, cl = 299.792458;
     const double w0 = c->p.waist, f = c->p.focus, lam0 = (double)k.lp;
     const double zr = M_PI * np * w0 * w0 / lam0;
     const double ap0 = std::sqrt(4.0 * c->p.power / (eps0 * cl * M_PI * np * w0 * w0));
     const cd Mq(1.0, -f / zr);
-    // the derivative fields depend on the context's parameters and T only: built once
-    std::vector<cd> &pdf = b.pump_df, &pdn = b.pump_dn;
-    if (pdf.size() != n2 || b.pump_T != T) {
-        pdf.resize(n2);
-        pdn.resize(n2);
-        for (int iy = 0; iy < n; iy++)
-            for (int ix = 0; ix < n; ix++) {
-                const double xx = (ix - 0.5 * n) * (double)k.dx, yy = (iy - 0.5 * n) * (double)k.dy;
-                const double r2 = xx * xx + yy * yy;
-                const cd A = ap0 / Mq * std::exp(-r2 / (w0 * w0 * Mq));
-                const cd dA_dM = A * (-1.0 / Mq + r2 / (w0 * w0 * Mq * Mq));
-                pdf[(size_t)iy * n + ix] = dA_dM * cd(0.0, -1.0 / zr);
-                pdn[(size_t)iy * n + ix] = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
-            }
-        b.pump_T = T;
-    }
-    double df = 0.0, dt = 0.0;
-    for (size_t i = 0; i < n2; i++) {
-        df += grad[i].real() * pdf[i].real() + grad[i].imag() * pdf[i].imag();
-        dt += (grad[i].real() * pdn[i].real() + grad[i].imag() * pdn[i].imag()) * dnp;
-    }
-    *d_focus = df;
-    *dT_pump = dt;
+    std::vector<cd> pdf(n2), pdn(n2);
+    for (int iy = 0; iy < n; iy++)
+        for (int ix = 0; ix < n; ix++) {
+            const double xx = (ix - 0.5 * n) * (double)k.dx, yy = (iy - 0.5 * n) * (double)k.dy;
+            const double r2 = xx * xx + yy * yy;
+            const cd A = ap0 / Mq * std::exp(-r2 / (w0 * w0 * Mq));
+            const cd dA_dM = A * (-1.0 / Mq + r2 / (w0 * w0 * Mq * Mq));
+            pdf[(size_t)iy * n + ix] = dA_dM * cd(0.0, -1.0 / zr);
+            pdn[(size_t)iy * n + ix] = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
+        }
+    AHIP_TRY(hipMemcpy(b.pump_df, pdf.data(), n2 * sizeof(double2), hipMemcpyHostToDevice));
+    AHIP_TRY(hipMemcpy(b.pump_dn, pdn.data(), n2 * sizeof(double2), hipMemcpyHostToDevice));
+    b.pump_T = T;
     return SHG_OK;
+}
+
+// After the adjoint march (lambda_0 in b.lam): d eta / d A0 = lambda_0 (FH part) + the
+// d(1/P_in)/dA0 term, and its projections on the pump derivative fields, on the device;
+// then the results are copied to the pinned staging (the field only when wanted).  The
+// caller synchronises the stream before reading them.
+int enqueue_gradient_results(shg_ctx *c, double cs, double cp, bool want_field, hipStream_t st) {
+    AdjointBuffers &b = c->adj;
+    const size_t n2 = (size_t)c->n * c->n;
+    const HostOffsets o = host_offsets(n2, c->steps);
+    AHIP_TRY(shg::launch_field_grad(b.lam, c->U0, c->red, cs, cp, c->p_fh_in, b.pump_df, b.pump_dn, b.grad, b.gpart,
+                                    kGradParts, (int64_t)n2, st));
+    AHIP_TRY(shg::launch_fold2(b.gpart, kGradParts, b.gsum, st));
+    AHIP_TRY(hipEventRecord(b.ev_end, st));
+    AHIP_TRY(hipMemcpyAsync(host_at<double2>(b, o.sums), c->red, sizeof(double2), hipMemcpyDeviceToHost, st));
+    AHIP_TRY(hipMemcpyAsync(host_at<double2>(b, o.gsum), b.gsum, sizeof(double2), hipMemcpyDeviceToHost, st));
+    AHIP_TRY(hipMemcpyAsync(host_at<double>(b, o.gs), b.gs, (size_t)c->steps * 6 * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    if (want_field)
+        AHIP_TRY(hipMemcpyAsync(host_at<float2>(b, o.grad), b.grad, n2 * sizeof(float2), hipMemcpyDeviceToHost, st));
+    return SHG_OK;
+}
+
+void copy_field_out(shg_ctx *c, float *d_eta_d_fh) {
+    if (!d_eta_d_fh) return;
+    const size_t n2 = (size_t)c->n * c->n;
+    std::memcpy(d_eta_d_fh, host_at<float2>(c->adj, host_offsets(n2, c->steps).grad), n2 * sizeof(float2));
 }
 
 // ----- THERMAL, temperature held fixed -------------------------------------------
@@ -240,6 +279,18 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
     b.map_valid = false;
     hipStream_t st = c->stream;
     const int cb = shg::adj_col_parts(n);
+    const shg::phys::Consts k(c->p);
+    const double T = c->t_centre;                              // propagators, powers, pump
+    const double eps0 = 8.8541878128e-6, cl = 299.792458;
+    const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
+    const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
+    const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
+    const double pin = c->p_fh_in;
+    const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
+    const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
+    const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
+    const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
+    ARC_TRY(ensure_pump_fields(c, np, T));
     auto slot = [&](int j) { return (size_t)(nseg > 1 ? (j & 1) : 0); };
     auto tape_of = [&](int j) {
         const size_t off = slot(j) * K * n2;
@@ -269,27 +320,10 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
     }
     AHIP_TRY(shg::launch_power(c->U, (int64_t)n2, c->part, c->nparts, c->red, st));
     AHIP_TRY(hipEventRecord(c->e1, st));
-    double2 sums;
-    AHIP_TRY(hipMemcpyAsync(&sums, c->red, sizeof(double2), hipMemcpyDeviceToHost, st));
-    AHIP_TRY(hipStreamSynchronize(st));
-    float fwd_ms = 0.f;
-    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
-
-    const shg::phys::Consts k(c->p);
-    const double T = c->t_centre;                              // propagators, powers, pump
-    const double eps0 = 8.8541878128e-6, cl = 299.792458;
-    const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
-    const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
-    const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
-    const double pin = c->p_fh_in, J = cs * sums.y / pin;
-    const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
-    const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
-    const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
-    const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
 
     // 2. seed, then per segment, last first: the adjoint on `st`, the previous segment's
-    //    taped recomputation on `ts`
-    AHIP_TRY(hipEventRecord(c->e0, st));
+    //    taped recomputation on `ts` (no host round trip after the forward: the seed
+    //    scale needs P_in only, and eta is read back with the gradients)
     AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
     for (double *ptr : {b.Ap, b.As})
         AHIP_TRY(shg::launch_fill_zero(ptr, (int64_t)n3, st));
@@ -336,18 +370,20 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
     AHIP_TRY(shg::launch_therm_dtmap(b.Ap, b.As, c->T[c->tcur], (int64_t)n3, c->p.crystal, (double)k.lp,
                                      (double)k.ls, b.dpart, kMapParts, st));
     AHIP_TRY(shg::launch_sum_partials(b.dpart, kMapParts, b.dsum, st));
-    AHIP_TRY(hipEventRecord(c->e1, st));
-    AHIP_TRY(hipEventSynchronize(c->e1));
-    float adj_ms = 0.f;
-    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e0, c->e1));
+    ARC_TRY(enqueue_gradient_results(c, cs, cp, d_eta_d_fh != nullptr, st));
+    const HostOffsets o = host_offsets(n2, M);
+    AHIP_TRY(hipMemcpyAsync(host_at<double>(b, o.dsum), b.dsum, sizeof(double), hipMemcpyDeviceToHost, st));
+    AHIP_TRY(hipStreamSynchronize(st));
+    float fwd_ms = 0.f, adj_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e1, b.ev_end));
 
     // 3. host assembly (FP64)
-    std::vector<double> gs((size_t)M * 6);
-    double dT_map = 0.0;
-    AHIP_TRY(hipMemcpy(gs.data(), b.gs, gs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    AHIP_TRY(hipMemcpy(&dT_map, b.dsum, sizeof(double), hipMemcpyDeviceToHost));
-    double d_focus = 0.0, dT_pump = 0.0;
-    ARC_TRY(field_and_pump_gradients(c, J, pin, cp, np, dnp, T, d_eta_d_fh, &d_focus, &dT_pump));
+    const double J = cs * host_at<double2>(b, o.sums)->y / pin;
+    const double *gs = host_at<double>(b, o.gs);
+    const double dT_map = *host_at<double>(b, o.dsum);
+    const double d_focus = host_at<double2>(b, o.gsum)->x, dT_pump = host_at<double2>(b, o.gsum)->y * dnp;
+    copy_field_out(c, d_eta_d_fh);
     double dT_H = 0.0;
     for (int s = 0; s < M; s++) dT_H += gs[(size_t)s * 6 + 5];
     const double dT_powers = J * (dns / ns - dnp / np);
@@ -390,14 +426,30 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
                              write_ckpt ? b.ckpt + (size_t)j * n2 : nullptr, j * K, 0};
     };
 
-    // 1. forward: checkpoints of segments 0 .. nseg-2, the last segment taped, the powers
+    // 1. forward: one uninterrupted march whose row passes at the segment boundaries
+    //    leave the checkpoints (the state after step jK - 1), the last segment taped, the
+    //    powers
     auto enqueue_forward = [&]() -> int {
         AHIP_TRY(shg::launch_copy_cells(c->U, c->U0, (int64_t)n2, st));
-        for (int j = 0; j < nseg - 1; j++) {
-            AHIP_TRY(shg::launch_copy_cells(b.ckpt + (size_t)j * n2, c->U, (int64_t)n2, st));
-            ARC_TRY(shg_internal_enqueue_march_plain(c, j * K, K));
+        if (nseg == 1) {
+            ARC_TRY(enqueue_tape(c, tape_of(0, true), M, st));
+        } else {
+            const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+            const int s_tape = (nseg - 1) * K;             // first step of the taped segment
+            AHIP_TRY(shg::launch_copy_cells(b.ckpt, c->U0, (int64_t)n2, st));
+            AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, 0, shg::ROW_START, st, rv));
+            for (int s = 0; s < s_tape; s++) {
+                AHIP_TRY(shg::launch_col_pass(n, c->d_desc, 1, c->tw, st, cv));
+                float4 *snap = (s + 1) % K == 0 ? b.ckpt + (size_t)((s + 1) / K) * n2 : nullptr;
+                AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, s, shg::ROW_MID, st, rv, snap));
+            }
+            const shg::TapeArgs tp = tape_of(nseg - 1, false);
+            for (int s = s_tape; s < M; s++) {
+                AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
+                AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < M - 1 ? shg::ROW_MID : shg::ROW_END, tp,
+                                              st));
+            }
         }
-        ARC_TRY(enqueue_tape(c, tape_of(nseg - 1, true), M, st));
         AHIP_TRY(shg::launch_power(c->U, (int64_t)n2, c->part, c->nparts, c->red, st));
         return SHG_OK;
     };
@@ -452,6 +504,10 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
         b.g_col_v = cv;
     }
 
+    // forward, seed and adjoint back to back: the seed scale needs P_in only, and eta is
+    // read back with the gradients, so the host waits once
+    const double pin = c->p_fh_in;
+    ARC_TRY(ensure_pump_fields(c, np, T));
     AHIP_TRY(hipEventRecord(c->e0, st));
     if (c->use_graph) {
         AHIP_TRY(hipGraphLaunch(b.g_fwd, st));
@@ -459,30 +515,24 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
         ARC_TRY(enqueue_forward());
     }
     AHIP_TRY(hipEventRecord(c->e1, st));
-    double2 sums;
-    AHIP_TRY(hipMemcpyAsync(&sums, c->red, sizeof(double2), hipMemcpyDeviceToHost, st));
-    AHIP_TRY(hipStreamSynchronize(st));
-    float fwd_ms = 0.f;
-    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
-    const double pin = c->p_fh_in, J = cs * sums.y / pin;
-
-    AHIP_TRY(hipEventRecord(c->e0, st));
     AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
     if (c->use_graph) {
         AHIP_TRY(hipGraphLaunch(b.g_adj, st));
     } else {
         ARC_TRY(enqueue_adjoint());
     }
-    AHIP_TRY(hipEventRecord(c->e1, st));
-    AHIP_TRY(hipEventSynchronize(c->e1));
-    float adj_ms = 0.f;
-    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e0, c->e1));
+    ARC_TRY(enqueue_gradient_results(c, cs, cp, d_eta_d_fh != nullptr, st));
+    AHIP_TRY(hipStreamSynchronize(st));
+    float fwd_ms = 0.f, adj_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e1, b.ev_end));
 
     // 3. gradients on the host (FP64)
-    std::vector<double> gs((size_t)M * 6);
-    AHIP_TRY(hipMemcpy(gs.data(), b.gs, gs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double d_focus = 0.0, dT_pump = 0.0;
-    ARC_TRY(field_and_pump_gradients(c, J, pin, cp, np, dnp, T, d_eta_d_fh, &d_focus, &dT_pump));
+    const HostOffsets o = host_offsets(n2, M);
+    const double J = cs * host_at<double2>(b, o.sums)->y / pin;
+    const double *gs = host_at<double>(b, o.gs);
+    const double d_focus = host_at<double2>(b, o.gsum)->x, dT_pump = host_at<double2>(b, o.gsum)->y * dnp;
+    copy_field_out(c, d_eta_d_fh);
 
     // temperature through the step coefficients and the TPA coefficient
     const double kap = (double)k.kappa_p((float)T), kas = (double)k.kappa_s((float)T);

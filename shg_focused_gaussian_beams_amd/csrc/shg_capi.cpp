@@ -244,11 +244,14 @@ void release(shg_ctx *c) {
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
                       (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam, (void *)c->adj.Ap, (void *)c->adj.As,
                       (void *)c->adj.Gc, (void *)c->adj.Dc, (void *)c->adj.dpart, (void *)c->adj.dsum,
-                      (void *)c->adj.dkt, (void *)c->adj.dkc})
+                      (void *)c->adj.dkt, (void *)c->adj.dkc, (void *)c->adj.pump_df, (void *)c->adj.pump_dn,
+                      (void *)c->adj.grad, (void *)c->adj.gpart, (void *)c->adj.gsum})
         if (ptr) (void)hipFree(ptr);
+    if (c->adj.host) (void)hipHostFree(c->adj.host);
     for (hipGraphExec_t g : {c->adj.g_fwd, c->adj.g_adj})
         if (g) (void)hipGraphExecDestroy(g);
-    for (hipEvent_t e : {c->adj.ev_u, c->adj.ev_tape[0], c->adj.ev_tape[1], c->adj.ev_free[0], c->adj.ev_free[1]})
+    for (hipEvent_t e : {c->adj.ev_u, c->adj.ev_tape[0], c->adj.ev_tape[1], c->adj.ev_free[0], c->adj.ev_free[1],
+                         c->adj.ev_end})
         if (e) (void)hipEventDestroy(e);
     if (c->adj.ts) (void)hipStreamDestroy(c->adj.ts);
     if (c->e0) (void)hipEventDestroy(c->e0);

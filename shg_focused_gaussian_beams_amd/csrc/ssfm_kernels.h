@@ -40,10 +40,14 @@ enum RowMode : int {
 // `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..4, columns 0..7;
 // row 4 and columns 5..7 are the split layouts).
 constexpr int kRowVariantCount = 5, kColVariantCount = 8;
+// A set `snap` (one instance only) receives the real-space state after step `step` (the
+// fields after its second RK4) from a MID or END pass: the checkpoints of the adjoint.
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
-                           int variant = 0);
+                           int variant = 0, float4 *snap = nullptr);
 hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw, hipStream_t st, int variant = 0);
 hipError_t launch_power(const float4 *U, int64_t ncell, double2 *part, int nparts, double2 *out, hipStream_t st);
+// out = sum of part[0 .. nparts) (componentwise, FP64; one block).
+hipError_t launch_fold2(const double2 *part, int nparts, double2 *out, hipStream_t st);
 // Per-instance FP64 sums of |FH|^2 and |SH|^2: out[i] for table entry i (one block each).
 hipError_t launch_power_batch(const InstDesc *d, const int *ncell, int ninst, double2 *out, hipStream_t st);
 

@@ -31,7 +31,7 @@ __device__ __forceinline__ void rk4(float2 &P, float2 &S, float4 c, const NlCons
 // layout <N, E, W> as in pass_cfg.h.
 template <int N, int E, int W>
 __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict__ descs, const float2 *__restrict__ twg,
-                                                     int step, int mode) {
+                                                     int step, int mode, float4 *__restrict__ snap) {
     using C = RowCfgT<N, E, W>;
     constexpr int T = C::T;
     __shared__ float2 lds[2][C::ROWS][C::L];
@@ -57,6 +57,10 @@ __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict_
         const float4 c = d.coef[step];
 #pragma unroll
         for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
+        if (snap)
+#pragma unroll
+            for (int j = 0; j < E; j++)
+                snap[(size_t)row * N + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
     }
     if (mode != ROW_END) {
         const float4 c = d.coef[mode == ROW_START ? step : step + 1];
@@ -118,7 +122,8 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
 // Row pass, one row per workgroup of W >= 2 waves: waves 0 and 1 transform FH and SH.
 template <int N, int E, int W>
 __global__ __launch_bounds__(64 * W) void k_row_split(const InstDesc *__restrict__ descs,
-                                                      const float2 *__restrict__ twg, int step, int mode) {
+                                                      const float2 *__restrict__ twg, int step, int mode,
+                                                      float4 *__restrict__ snap) {
     constexpr int T = N / E, THREADS = 64 * W, L = LdsLen<N>::value;
     static_assert(W >= 2 && T <= 64, "one wavefront per field transform");
     __shared__ float2 lds[2][L];
@@ -140,7 +145,10 @@ __global__ __launch_bounds__(64 * W) void k_row_split(const InstDesc *__restrict
     const float4 c1 = d.coef[mode == ROW_START ? step : (mode == ROW_MID ? step + 1 : step)];
     for (int c = tid; c < N; c += THREADS) {
         float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
-        if (mode != ROW_START) rk4(P, S, c0, d.nl);
+        if (mode != ROW_START) {
+            rk4(P, S, c0, d.nl);
+            if (snap) snap[(size_t)blockIdx.x * N + c] = make_float4(P.x, P.y, S.x, S.y);
+        }
         if (mode != ROW_END) rk4(P, S, c1, d.nl);
         lds[0][lds_pad(c)] = P;
         lds[1][lds_pad(c)] = S;
@@ -256,9 +264,11 @@ __global__ __launch_bounds__(256) void k_power_batch(const InstDesc *__restrict_
 // Launchers.  Variant tables (index 0 = default): rows {E, waves}, columns {E, CW}.
 
 template <int N, int E, int W>
-static hipError_t row_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
+static hipError_t row_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
+                             float4 *snap) {
     using C = RowCfgT<N, E, W>;
-    hipLaunchKernelGGL((k_row_pass<N, E, W>), dim3(N / C::ROWS, ninst), dim3(C::THREADS), 0, st, d, tw, step, mode);
+    hipLaunchKernelGGL((k_row_pass<N, E, W>), dim3(N / C::ROWS, ninst), dim3(C::THREADS), 0, st, d, tw, step, mode,
+                       snap);
     return hipGetLastError();
 }
 template <int N, int E, int CW>
@@ -269,8 +279,9 @@ static hipError_t col_launch(const InstDesc *d, int ninst, const float2 *tw, hip
 }
 
 template <int N, int E, int W>
-static hipError_t row_split_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st) {
-    hipLaunchKernelGGL((k_row_split<N, E, W>), dim3(N, ninst), dim3(64 * W), 0, st, d, tw, step, mode);
+static hipError_t row_split_launch(const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
+                                   float4 *snap) {
+    hipLaunchKernelGGL((k_row_split<N, E, W>), dim3(N, ninst), dim3(64 * W), 0, st, d, tw, step, mode, snap);
     return hipGetLastError();
 }
 template <int N, int E, int CW>
@@ -279,14 +290,15 @@ static hipError_t col_split_launch(const InstDesc *d, int ninst, const float2 *t
     return hipGetLastError();
 }
 
-#define ROWV(N, E, W) return row_launch<N, E, W>(d, ninst, tw, step, mode, st)
+#define ROWV(N, E, W) return row_launch<N, E, W>(d, ninst, tw, step, mode, st, snap)
 #define COLV(N, E, CW) return col_launch<N, E, CW>(d, ninst, tw, st)
-#define ROWS(N, E, W) return row_split_launch<N, E, W>(d, ninst, tw, step, mode, st)
+#define ROWS(N, E, W) return row_split_launch<N, E, W>(d, ninst, tw, step, mode, st, snap)
 #define COLS(N, E, CW) return col_split_launch<N, E, CW>(d, ninst, tw, st)
 
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
-                           int variant) {
+                           int variant, float4 *snap) {
     if (ninst <= 0 || ninst > 65535 || step < 0 || mode < ROW_START || mode > ROW_AUTO) return hipErrorInvalidValue;
+    if (snap && ninst != 1) return hipErrorInvalidValue;
     switch (n * 16 + variant) {
         case 64 * 16 + 0: ROWV(64, 8, 1);
         case 64 * 16 + 1: ROWV(64, 4, 1);
@@ -353,6 +365,12 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
 
 hipError_t launch_power(const float4 *U, int64_t ncell, double2 *part, int nparts, double2 *out, hipStream_t st) {
     hipLaunchKernelGGL(k_power_partial, dim3(nparts), dim3(256), 0, st, U, ncell, part);
+    hipLaunchKernelGGL(k_power_final, dim3(1), dim3(256), 0, st, part, nparts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold2(const double2 *part, int nparts, double2 *out, hipStream_t st) {
+    if (nparts <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_power_final, dim3(1), dim3(256), 0, st, part, nparts, out);
     return hipGetLastError();
 }

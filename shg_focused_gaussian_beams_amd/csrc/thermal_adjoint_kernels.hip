@@ -23,38 +23,59 @@
 
 namespace shg {
 
-// RK4^T of one THERMAL RK4 application (rk4_thermal) at forward input (P, S): updates the
-// adjoint (lP, lS) and returns the cell's sensitivities to DKint (dDK) and, through the
-// coupling constants and the TPA coefficient, to its indices (dnp, dns).
-__device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, const CellPhys &c,
-                                              float dkint, const ThermStep &st, const ThermConst &k, double &dDK,
-                                              double &dnp, double &dns) {
-    float sn[3], cs[3];
+// The three stage-group phases of one THERMAL RK4 (rk4_thermal): sin and cos of
+// phi_g = (float)(DKint * zr_g) and the coefficients (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}).
+struct ThermPhase {
     float4 cf[3];
+    float sn[3], cs[3];
+};
+
+__device__ __forceinline__ void therm_phase(const CellPhys &c, float dkint, const ThermStep &st, ThermPhase &ph) {
 #pragma unroll
     for (int g = 0; g < 3; g++) {
-        cf[g] = phase_coef(c, dkint, st.zr[g]);   // (kp sin, kp cos, -ks sin, ks cos)
         const float phi = (float)((double)dkint * st.zr[g]);
-        sincosf(phi, &sn[g], &cs[g]);
+        sincosf(phi, &ph.sn[g], &ph.cs[g]);
+        ph.cf[g] = make_float4(c.kp * ph.sn[g], c.kp * ph.cs[g], -c.ks * ph.sn[g], c.ks * ph.cs[g]);
     }
+}
+
+// RK4^T of one THERMAL RK4 application at forward input (P, S) with stage inputs `st`:
+// updates the adjoint (lP, lS) and returns the cell's sensitivities to DKint (dDK) and,
+// through the coupling constants and the TPA coefficient, to its indices (dnp, dns).
+__device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, const Rk4Stages &stg, float2 &lP, float2 &lS,
+                                              const CellPhys &c, const ThermPhase &ph, const ThermStep &st,
+                                              const ThermConst &k, double &dDK, double &dnp, double &dns) {
     AdjAcc acc[3];
 #pragma unroll
     for (int g = 0; g < 3; g++) acc_zero(acc[g]);
-    rk4_adj3(P, S, lP, lS, cf[0], cf[1], cf[2], k.lossP, k.lossS, c.tpa, k.h6, acc[0], acc[1], acc[2]);
+    rk4_adj_rev<false>(P, S, stg, lP, lS, ph.cf[0], ph.cf[1], ph.cf[2], k.lossP, k.lossS, c.tpa, k.h6, acc[0], acc[1],
+                       acc[2]);
     double dkp = 0.0, dks = 0.0, dphi_dk = 0.0;
 #pragma unroll
     for (int g = 0; g < 3; g++) {
         // c1 = i kp e^{-i phi}: dc1/dkp = (sin, cos), dc1/dphi = -i c1;
         // c2 = i ks e^{+i phi}: dc2/dks = (-sin, cos), dc2/dphi = +i c2.
-        dkp += acc[g].c1r * sn[g] + acc[g].c1i * cs[g];
-        dks += -acc[g].c2r * sn[g] + acc[g].c2i * cs[g];
-        const double dphi = acc[g].c1r * cf[g].y - acc[g].c1i * cf[g].x - acc[g].c2r * cf[g].w + acc[g].c2i * cf[g].z;
+        const float4 cf = ph.cf[g];
+        dkp += acc[g].c1r * ph.sn[g] + acc[g].c1i * ph.cs[g];
+        dks += -acc[g].c2r * ph.sn[g] + acc[g].c2i * ph.cs[g];
+        const double dphi = acc[g].c1r * cf.y - acc[g].c1i * cf.x - acc[g].c2r * cf.w + acc[g].c2i * cf.z;
         dphi_dk += dphi * st.zr[g];
     }
     dDK += dphi_dk;
     // kp = pi dQ / (n_p lp), ks = pi dQ / (2 n_s ls), tpa = tpa_base n_s / 2
     dnp += dkp * (-(double)c.kp / (double)c.np);
     dns += dks * (-(double)c.ks / (double)c.ns) + (acc[0].tpa + acc[1].tpa + acc[2].tpa) * 0.5 * k.tpa_base;
+}
+
+// The same, evaluating the phases and stages itself.
+__device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, float2 &lS, const CellPhys &c,
+                                              float dkint, const ThermStep &st, const ThermConst &k, double &dDK,
+                                              double &dnp, double &dns) {
+    ThermPhase ph;
+    therm_phase(c, dkint, st, ph);
+    Rk4Stages stg;
+    rk4_stages(P, S, ph.cf[0], ph.cf[1], k.lossP, k.lossS, c.tpa, stg);
+    therm_rk4_adj(P, S, stg, lP, lS, c, ph, st, k, dDK, dnp, dns);
 }
 
 template <int N>
@@ -114,9 +135,13 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
         } else {
             float2 Ap_ = f2(A.x, A.y), As_ = f2(A.z, A.w);
             CellPhys cp{};
+            ThermPhase php;
+            Rk4Stages kept;
             if (mode == ADJ_MID) {
+                // A_s from A''_{s-1} (rk4_thermal), keeping the phases and stages for RK4_{s-1}^T
                 cp = cell_phys(k, nn_p);
-                rk4_thermal(Ap_, As_, cp, dk_p, a.prev, k);          // A_s from A''_{s-1}
+                therm_phase(cp, dk_p, a.prev, php);
+                rk4_keep(Ap_, As_, php.cf[0], php.cf[1], php.cf[2], k.lossP, k.lossS, cp.tpa, k.h6, kept);
             }
             double dDK = 0.0, dnp = 0.0, dns = 0.0;
             therm_rk4_adj(Ap_, As_, P, S, cc, dk_c, a.cur, k, dDK, dnp, dns);
@@ -126,7 +151,7 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
             Dp = 0.0;
             if (mode == ADJ_MID) {
                 double dDK2 = 0.0, dnp2 = 0.0, dns2 = 0.0;
-                therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), P, S, cp, dk_p, a.prev, k, dDK2, dnp2, dns2);
+                therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), kept, P, S, cp, php, a.prev, k, dDK2, dnp2, dns2);
                 a.Ap[(size_t)a.prev.slice * n2 + idx] += dnp2;
                 a.As[(size_t)a.prev.slice * n2 + idx] += dns2;
                 Dp = dDK2;
