@@ -24,19 +24,17 @@
 namespace shg {
 
 // The three stage-group phases of one THERMAL RK4 (rk4_thermal): sin and cos of
-// phi_g = (float)(DKint * zr_g) and the coefficients (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}).
+// phi_g = (float)(DKint * zr_g) (phase_sincos3) and the coefficients (c1, c2) = (i kp e^{-i phi}, i ks e^{+i phi}).
 struct ThermPhase {
     float4 cf[3];
     float sn[3], cs[3];
 };
 
 __device__ __forceinline__ void therm_phase(const CellPhys &c, float dkint, const ThermStep &st, ThermPhase &ph) {
+    phase_sincos3(dkint, st, ph.sn, ph.cs);
 #pragma unroll
-    for (int g = 0; g < 3; g++) {
-        const float phi = (float)((double)dkint * st.zr[g]);
-        sincosf(phi, &ph.sn[g], &ph.cs[g]);
+    for (int g = 0; g < 3; g++)
         ph.cf[g] = make_float4(c.kp * ph.sn[g], c.kp * ph.cs[g], -c.ks * ph.sn[g], c.ks * ph.cs[g]);
-    }
 }
 
 // RK4^T of one THERMAL RK4 application at forward input (P, S) with stage inputs `st`:
