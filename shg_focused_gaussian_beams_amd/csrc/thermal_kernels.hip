@@ -232,14 +232,15 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     float2 nnp[CPT], nnc[CPT];
     float4 cfp[CPT], cfc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
+    const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const size_t idx = base + tid + q * THREADS;
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        nnc[q] = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
         cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
-        cfc[q] = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
+        cfc[q] = !cu ? make_float4(0.f, 0.f, 0.f, 0.f) : same ? cfp[q] : a.ct[scur + idx];
     }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
@@ -302,15 +303,16 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
     float2 nnp[CPT], nnc[CPT];
     float4 cfp[CPT], cfc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
+    const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const int c = tid + q * THREADS;
         const size_t idx = base + (c < N ? c : 0);
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        nnc[q] = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
         cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
-        cfc[q] = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
+        cfc[q] = !cu ? make_float4(0.f, 0.f, 0.f, 0.f) : same ? cfp[q] : a.ct[scur + idx];
     }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int c = tid; c < N; c += THREADS) {
