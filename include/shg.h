@@ -41,6 +41,14 @@ extern "C" {
 #define SHG_CRYSTAL_SPPLT   0    /* MgO:sPPLT, src/headers/Crystal.h:55-71 */
 #define SHG_CRYSTAL_PPLN    1    /* MgO:PPLN (Gayer et al. 2008); absent from the reference */
 
+/* Heat-equation boundary (THERMAL).  TOP_OPEN: heatEquationTopOpen with the two ovens
+ * under y = 0 (src/headers/Tfield.h:48-68, 113-221; the reference's run).  OVEN_SURROUNDED:
+ * kernelHeatEquationOvenSurrounded with the ovens on the four lateral faces
+ * (src/headers/Tfield.h:71-110, 224-258; reachable in the reference only through the
+ * commented call at src/headers/Solver.h:290). */
+#define SHG_BC_TOP_OPEN        0
+#define SHG_BC_OVEN_SURROUNDED 1
+
 #define SHG_SCHEME_REFERENCE 0   /* RK4(dz) - L(dz) - RK4(dz), src/headers/Solver.h:236-242 */
 #define SHG_SCHEME_STRANG    1   /* RK4(dz/2) - L(dz) - RK4(dz/2) */
 
@@ -75,6 +83,7 @@ typedef struct shg_params {
     int32_t max_outer;         /* outer fixed-point iterations; reference 5 */
     int32_t jacobi_max;        /* cap on sweeps per outer iteration; 0 = no cap */
     double  jacobi_tol;        /* mean squared update tolerance; 0 = reference 5e-4/(nx*ny*heat_nz) */
+    int32_t thermal_bc;        /* SHG_BC_* (default TOP_OPEN) */
 } shg_params;
 
 typedef struct shg_ctx shg_ctx;
@@ -121,6 +130,14 @@ int shg_march(shg_ctx *ctx, int first_step, int nsteps);
 
 /* Copies the current device fields to caller buffers (nx*ny complex64 each). */
 int shg_get_fields(shg_ctx *ctx, float *fh, float *sh);
+
+/* The save_only_last = 0 output of CWES (src/headers/Solver.h:257-262): runs the march of
+ * shg_forward from the shg_set_fields state (THERMAL: at the current temperature) and
+ * writes the fields before every step s = 0 .. nz-1 (the reference's Pump_<s>,
+ * Signal_<s>) to fh_steps + 2*s*nx*ny and sh_steps + 2*s*nx*ny (complex64).  Holds
+ * nz*nx*ny*16 bytes of device memory during the call.  Leaves the context as
+ * shg_forward does. */
+int shg_forward_snapshots(shg_ctx *ctx, float *fh_steps, float *sh_steps);
 
 /* FP64 powers: P_j = 0.5*eps0*c*n_j*sum|A_j|^2*dx*dy for the z = 0 pump and the
  * current SH, and eta = P_SH / P_FH(z=0).  New: the reference never reduces powers. */

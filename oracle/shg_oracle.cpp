@@ -112,11 +112,11 @@ template <class R> R n_index(int crystal, R L, R T) {
 // The reference's global constants (src/headers/Crystal.h:18-53, src/SHG.cpp:7-20)
 // at run-time grid sizes.  Each `const real_t X = <double expr>` is rounded to R once.
 template <class R> struct K {
-    int NX, NY, NZ, crystal;
+    int NX, NY, NZ, crystal, bc;
     R PI, C, EPS0, lp, ls, LX, LY, LZ, dx, dy, dz, deff, dQ, Lambda;
     R alpha_p, alpha_s, beta_s, rhop, rhos, k_th, h_conv;
     explicit K(const shg_params &p) {
-        NX = p.nx; NY = p.ny; NZ = p.nz; crystal = p.crystal;
+        NX = p.nx; NY = p.ny; NZ = p.nz; crystal = p.crystal; bc = p.thermal_bc;
         PI = (R)3.14159265358979323846;
         C = (R)(299792458 * 1E6 / 1E12);
         EPS0 = (R)(8.8541878128E-12 * 1E12 / 1E6);
@@ -380,9 +380,41 @@ template <class R> void set_q(State<R> &st, int hs) {
     }
 }
 
+// kernelHeatEquationOvenSurrounded (src/headers/Tfield.h:224-258): one Jacobi sweep with
+// the ovens on the four lateral faces (never written); inner points, and the z = 0 and
+// z = L faces (convective, Biot number h dz / k) inside the lateral faces.
+template <class R> void jacobi_sweep_oven(const K<R> &k, int NZ, R dz, R *Tf, const R *Ti, R T_inf, const R *Q) {
+    const int NX = k.NX, NY = k.NY;
+    const R dx = k.dx, dy = k.dy;
+    auto I = [&](int x, int y, int z) { return (size_t)z * NX * NY + (size_t)y * NX + x; };
+    const R Biz = k.h_conv * dz / k.k_th;
+    const R az = dz * dz / ((R)2 * dx * dx), bz = dz * dz / ((R)2 * dy * dy);
+    const R L2 = (R)1 / ((R)2 / (dx * dx) + (R)2 / (dy * dy) + (R)2 / (dz * dz));
+    for (int z = 0; z < NZ; z++)
+        for (int y = 1; y < NY - 1; y++)
+            for (int x = 1; x < NX - 1; x++) {
+                if (z > 0 && z < NZ - 1) {
+                    Tf[I(x, y, z)] = L2 * ((Ti[I(x + 1, y, z)] + Ti[I(x - 1, y, z)]) / (dx * dx) +
+                                           (Ti[I(x, y + 1, z)] + Ti[I(x, y - 1, z)]) / (dy * dy) +
+                                           (Ti[I(x, y, z + 1)] + Ti[I(x, y, z - 1)]) / (dz * dz) +
+                                           Q[I(x, y, z)] / k.k_th);
+                } else {
+                    const int zn = z == 0 ? z + 1 : z - 1;     // back (z = 0) or front (z = L)
+                    Tf[I(x, y, z)] = (Ti[I(x, y, zn)] + az * (Ti[I(x + 1, y, z)] + Ti[I(x - 1, y, z)]) +
+                                      bz * (Ti[I(x, y + 1, z)] + Ti[I(x, y - 1, z)]) + Biz * T_inf) /
+                                     ((R)1 + (R)2 * az + (R)2 * bz + Biz);
+                }
+            }
+}
+
 // heatEquationTopOpen (src/headers/Tfield.h:113-221): one Jacobi sweep Tf <- J(Ti, Q).
 // The y = 0 plane is never written (Dirichlet ovens).  NZ here is the heat grid depth.
+// With the surrounded-oven boundary the sweep is jacobi_sweep_oven.
 template <class R> void jacobi_sweep(const K<R> &k, int NZ, R dz, R *Tf, const R *Ti, R T_inf, const R *Q) {
+    if (k.bc == SHG_BC_OVEN_SURROUNDED) {
+        jacobi_sweep_oven<R>(k, NZ, dz, Tf, Ti, T_inf, Q);
+        return;
+    }
     const int NX = k.NX, NY = k.NY;
     const R dx = k.dx, dy = k.dy;
     auto I = [&](int x, int y, int z) { return (size_t)z * NX * NY + (size_t)y * NX + x; };
@@ -668,14 +700,18 @@ int oracle_thermal_run(const shg_params *p, int pump_from_T, const double *fh0, 
     const float hdz = HZ == NZ ? k.dz : k.LZ / (float)(unsigned)(HZ - 1);
     const float Temp = (float)p->temp, T_inf = (float)p->t_inf;
     st.Ti.assign(n3, Temp);                                      // initTemperature (Tfield.h:31-45)
-    st.Tf.assign(n3, Temp);
-    const float nzf = (float)(unsigned)HZ;                       // initBottomOvensTemperature (Tfield.h:48-68)
-    for (int z = 0; z < HZ; z++)
-        for (int x = 0; x < NX; x++) {
-            const float v = (z < nzf / 2.0) ? (float)p->t_peltier1 : (float)p->t_peltier2;
-            st.Ti[(size_t)z * n2 + x] = v;
-            st.Tf[(size_t)z * n2 + x] = v;
-        }
+    const float nzf = (float)(unsigned)HZ;
+    for (int z = 0; z < HZ; z++) {
+        const float v = (z < nzf / 2.0) ? (float)p->t_peltier1 : (float)p->t_peltier2;
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const bool oven = p->thermal_bc == SHG_BC_OVEN_SURROUNDED
+                                      ? (y == 0 || y == NY - 1 || x == 0 || x == NX - 1)   // initOvenSurrounded (:71-110)
+                                      : y == 0;                   // initBottomOvensTemperature (:48-68)
+                if (oven) st.Ti[(size_t)z * n2 + (size_t)y * NX + x] = v;
+            }
+    }
+    st.Tf = st.Ti;
     st.Q.assign(n3, 0.f);                                        // initQ0
     st.DK.assign(n2 * NZ, 0.f);
     st.DKint.assign(n2, 0.f);

@@ -74,6 +74,7 @@ def load_library(path=LIB_PATH):
         "shg_reset_fields": (ctypes.c_int, [vp]),
         "shg_march": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_get_fields": (ctypes.c_int, [vp, fp, fp]),
+        "shg_forward_snapshots": (ctypes.c_int, [vp, fp, fp]),
         "shg_powers": (ctypes.c_int, [vp, dp, dp, dp]),
         "shg_last_forward_ms": (ctypes.c_int, [vp, dp]),
         "shg_cell_steps": (ctypes.c_int64, [vp]),
@@ -180,6 +181,15 @@ class Context:
         fh = np.empty(shape, np.complex64)
         sh = np.empty(shape, np.complex64)
         _check(self.lib.shg_get_fields(self.h, _fptr(fh), _fptr(sh)))
+        return fh, sh
+
+    def forward_snapshots(self):
+        """The march of forward(), keeping the fields before every step: (fh, sh) of
+        shape (nz, ny, nx) complex64 (the reference's Pump_<s>, Signal_<s>)."""
+        shape = (self.params.nz, self.params.ny, self.params.nx)
+        fh = np.empty(shape, np.complex64)
+        sh = np.empty(shape, np.complex64)
+        _check(self.lib.shg_forward_snapshots(self.h, _fptr(fh), _fptr(sh)))
         return fh, sh
 
     def powers(self):

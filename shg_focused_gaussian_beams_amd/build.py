@@ -33,7 +33,7 @@ def build(verbose=True):
     _run([hipcc, *HIP_FLAGS, "-shared", "-o", LIB + ".tmp", *srcs])
     os.replace(LIB + ".tmp", LIB)
     for exe, src in ((CLI, "shg_cli.cpp"), (SWEEP, "shg_sweep.cpp")):
-        _run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(CSRC, src), "-L" + HERE, "-lshg",
+        _run(["g++", "-O2", "-std=c++17", "-pthread", "-o", exe, os.path.join(CSRC, src), "-L" + HERE, "-lshg",
               "-Wl,-rpath,$ORIGIN", "-I" + os.path.join(HERE, "..", "include")])
     return LIB
 

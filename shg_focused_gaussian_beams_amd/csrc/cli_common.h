@@ -100,6 +100,8 @@ bool parse_override(shg_params &p, const std::string &key, const std::string &va
     else if (key == "crystal") p.crystal = val == "ppln" ? SHG_CRYSTAL_PPLN : SHG_CRYSTAL_SPPLT;
     else if (key == "scheme") p.scheme = val == "strang" ? SHG_SCHEME_STRANG : SHG_SCHEME_REFERENCE;
     else if (key == "thermal") p.thermal = i;
+    else if (key == "thermal_bc")
+        p.thermal_bc = val == "top_open" ? SHG_BC_TOP_OPEN : val == "oven_surrounded" ? SHG_BC_OVEN_SURROUNDED : -1;
     else if (key == "heat_nz") p.heat_nz = i;
     else if (key == "jacobi_check") p.jacobi_check = i;
     else if (key == "max_outer") p.max_outer = i;

@@ -85,7 +85,6 @@ struct shg_ctx {
 
 // Shared helpers of the library's translation units (defined in shg_capi.cpp).
 int shg_internal_fail(int code, const std::string &msg);
-int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count);
 shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s);
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
 double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);

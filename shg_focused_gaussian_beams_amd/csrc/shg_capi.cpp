@@ -106,15 +106,18 @@ int upload_propagators(shg_ctx *c, float temp) {
 
 // Plain build: enqueues steps [first, first + count) on the context stream.  The field
 // is in real space before and after.
-int enqueue_march_plain(shg_ctx *c, int first, int count) {
+// With `snaps` (nz planes), the row pass after step s writes the state before step s+1
+// to snaps[s+1].
+int enqueue_march_plain(shg_ctx *c, int first, int count, float4 *snaps = nullptr) {
     if (count <= 0) return SHG_OK;
     const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+    const size_t n2 = (size_t)c->n * c->n;
     HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, first, shg::ROW_START, c->stream, rv));
     const int last = first + count - 1;
     for (int s = first; s <= last; s++) {
         HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
         HIP_TRY(shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, s, s < last ? shg::ROW_MID : shg::ROW_END, c->stream,
-                                     rv));
+                                     rv, snaps ? snaps + (size_t)(s + 1) * n2 : nullptr));
     }
     return SHG_OK;
 }
@@ -191,8 +194,9 @@ shg::ThermStep therm_step(const shg_ctx *c, int s) {
 // THERMAL build: all nz steps of Solver::CWES (src/headers/Solver.h:245-276) with the
 // heat-source update after every step; Uout receives the fields before the last step.
 // Reads the index table (launch_index_table of the current temperature first).
-int enqueue_march_thermal(shg_ctx *c) {
+int enqueue_march_thermal(shg_ctx *c, float4 *snaps = nullptr) {
     const int nz = c->p.nz;
+    const size_t n2 = (size_t)c->n * c->n;
     shg::ThermRowArgs a{};
     a.k = c->tc;
     a.ct = c->ctab;
@@ -204,14 +208,16 @@ int enqueue_march_thermal(shg_ctx *c) {
     for (int s = 0; s < nz; s++) {
         HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, c->col_v < 0 ? 0 : c->col_v));
         a.prev = therm_step(c, s);
+        float4 *snap = c->Uout;
         if (s < nz - 1) {
             a.cur = therm_step(c, s + 1);
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
-            if (s + 1 == nz - 1) a.flags |= shg::TROW_SNAP;
+            if (s + 1 == nz - 1 || snaps) a.flags |= shg::TROW_SNAP;
+            if (snaps) snap = snaps + (size_t)(s + 1) * n2;     // the state before step s+1
         } else {
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV;
         }
-        HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
+        HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, snap, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
     }
     return SHG_OK;
 }
@@ -298,10 +304,19 @@ int thermal_init_temperature(shg_ctx *c) {
     const int n = c->n, hz = c->hz;
     const size_t n2 = (size_t)n * n;
     std::vector<float> t(n2 * hz, (float)c->p.temp);          // initTemperature (Tfield.h:31-45)
-    const float nzf = (float)(unsigned)hz;                      // initBottomOvensTemperature (:48-68)
-    for (int z = 0; z < hz; z++)
-        for (int x = 0; x < n; x++)
-            t[(size_t)z * n2 + x] = (z < nzf / 2.0) ? (float)c->p.t_peltier1 : (float)c->p.t_peltier2;
+    const float nzf = (float)(unsigned)hz;
+    for (int z = 0; z < hz; z++) {
+        const float v = (z < nzf / 2.0) ? (float)c->p.t_peltier1 : (float)c->p.t_peltier2;
+        float *tz = t.data() + (size_t)z * n2;
+        if (c->p.thermal_bc == SHG_BC_OVEN_SURROUNDED) {
+            for (int i = 0; i < n; i++) {                       // initOvenSurrounded (:71-110)
+                tz[i] = tz[(size_t)(n - 1) * n + i] = v;        // y = 0, y = ny-1
+                tz[(size_t)i * n] = tz[(size_t)i * n + n - 1] = v;   // x = 0, x = nx-1
+            }
+        } else {
+            for (int x = 0; x < n; x++) tz[x] = v;              // initBottomOvensTemperature (:48-68)
+        }
+    }
     for (int b = 0; b < 2; b++)
         HIP_TRY(hipMemcpyAsync(c->T[b], t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemsetAsync(c->Q, 0, t.size() * sizeof(float), c->stream));   // initQ0 (:261-274)
@@ -341,7 +356,6 @@ int timed_march(shg_ctx *c) {
 
 }  // namespace
 
-int shg_internal_enqueue_march_plain(shg_ctx *c, int first, int count) { return enqueue_march_plain(c, first, count); }
 shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s) { return therm_step(c, s); }
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh) {
     return reduce_powers(c, U, sum_fh, sum_sh);
@@ -372,6 +386,7 @@ void shg_default_params(shg_params *p) {
     p->temp = 55.66; p->t_inf = 25.0;                          // src/SHG.sh:33, src/SHG.cpp:43
     p->t_peltier1 = 55.66; p->t_peltier2 = 55.66;
     p->heat_nz = 0; p->jacobi_check = 5000; p->max_outer = 5; p->jacobi_max = 0; p->jacobi_tol = 0.0;
+    p->thermal_bc = SHG_BC_TOP_OPEN;                           // src/headers/Solver.h:290-291
 }
 
 int shg_create(const shg_params *p, shg_ctx **out) {
@@ -390,6 +405,8 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         const int hz = p->heat_nz > 0 ? p->heat_nz : p->nz;
         if (hz < 2 || hz > p->nz) return fail(SHG_E_INVALID, "shg_create: heat_nz must be in [2, nz]");
         if (!(p->k_th > 0)) return fail(SHG_E_INVALID, "shg_create: k_th must be > 0");
+        if (p->thermal_bc != SHG_BC_TOP_OPEN && p->thermal_bc != SHG_BC_OVEN_SURROUNDED)
+            return fail(SHG_E_INVALID, "shg_create: unknown thermal_bc");
     }
 
     shg_ctx *c = new shg_ctx();
@@ -445,6 +462,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
         const float hdz = shg::phys::heat_dz(k, c->hz);
         c->jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p->k_th, (float)p->h_conv, (float)p->t_inf);
+        c->jc.bc = p->thermal_bc;
         shg::ThermConst &tc = c->tc;
         tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
         tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
@@ -516,6 +534,48 @@ int shg_march(shg_ctx *c, int first_step, int nsteps) {
     RC_TRY(enqueue_march_plain(c, first_step, nsteps));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SHG_OK;
+}
+
+int shg_forward_snapshots(shg_ctx *c, float *fh_steps, float *sh_steps) {
+    if (!c || !fh_steps || !sh_steps) return fail(SHG_E_INVALID, "shg_forward_snapshots: null argument");
+    if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward_snapshots: call shg_set_fields first");
+    const int nz = c->p.nz;
+    const size_t n2 = (size_t)c->n * c->n;
+    if (c->row_v < 0) {                                     // the layouts shg_forward would pick
+        RC_TRY(timed_march(c));
+    }
+    float4 *snaps = nullptr;
+    if (hipMalloc(&snaps, (size_t)nz * n2 * sizeof(float4)) != hipSuccess)
+        return fail(SHG_E_NOMEM, "shg_forward_snapshots: device allocation failed");
+    auto run = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(snaps, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        if (c->p.thermal) {
+            HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)n2 * c->hz, c->tc, c->stream));
+            RC_TRY(enqueue_march_thermal(c, snaps));
+            HIP_TRY(hipMemcpyAsync(c->Uout, snaps + (size_t)(nz - 1) * n2, n2 * sizeof(float4),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            RC_TRY(enqueue_march_plain(c, 0, c->steps, snaps));
+        }
+        std::vector<float> h(4 * n2);
+        for (int s = 0; s < nz; s++) {
+            HIP_TRY(hipMemcpyAsync(h.data(), snaps + (size_t)s * n2, n2 * sizeof(float4), hipMemcpyDeviceToHost,
+                                   c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            float *fh = fh_steps + 2 * (size_t)s * n2, *sh = sh_steps + 2 * (size_t)s * n2;
+            for (size_t i = 0; i < n2; i++) {
+                fh[2 * i] = h[4 * i];
+                fh[2 * i + 1] = h[4 * i + 1];
+                sh[2 * i] = h[4 * i + 2];
+                sh[2 * i + 1] = h[4 * i + 3];
+            }
+        }
+        return SHG_OK;
+    };
+    const int rc = run();
+    (void)hipFree(snaps);
+    return rc;
 }
 
 int shg_get_fields(shg_ctx *c, float *fh, float *sh) {

@@ -12,12 +12,47 @@
 // Overrides (defaults are the reference's constants): --nx --nz --lx --ly --lz --lambda
 // --deff --period --alpha_p --alpha_s --beta_s --rho_p --rho_s --k_th --h_conv --focus
 // --t_inf --crystal=spplt|ppln --scheme=reference|strang --thermal=0|1 --heat_nz
-// --jacobi_check --max_outer --jacobi_max --jacobi_tol --seed --noise --outdir
+// --thermal_bc=top_open|oven_surrounded --jacobi_check --max_outer --jacobi_max
+// --jacobi_tol --seed --noise --outdir
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
 #include "cli_common.h"
 
 using namespace shgcli;
 
 namespace {
+
+// save_only_last = 0 (src/headers/Solver.h:257-262): Pump_<s>, Signal_<s> before every
+// step s.  The march runs once on the GPU keeping every plane; the text files (20
+// significant digits, the bulk of the time) are formatted by a pool of writer threads.
+int save_every_step(shg_ctx *ctx, const shg_params &p, const std::string &outdir) {
+    const size_t n2 = (size_t)p.nx * p.ny, per = 2 * n2;
+    std::vector<float> fh(per * p.nz), sh(per * p.nz);
+    if (int rc = shg_forward_snapshots(ctx, fh.data(), sh.data())) return die("shg_forward_snapshots", rc);
+    const int workers = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    std::atomic<bool> ok{true};
+    auto work = [&] {
+        std::vector<float> a(per), b(per);
+        for (int s = next++; s < p.nz && ok; s = next++) {
+            std::copy(fh.begin() + per * s, fh.begin() + per * (s + 1), a.begin());
+            std::copy(sh.begin() + per * s, sh.begin() + per * (s + 1), b.begin());
+            if (!save_complex(outdir, "Pump_" + std::to_string(s), a, p.nx, p.ny) ||
+                !save_complex(outdir, "Signal_" + std::to_string(s), b, p.nx, p.ny))
+                ok = false;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; w++) pool.emplace_back(work);
+    for (auto &t : pool) t.join();
+    if (!ok) {
+        std::cerr << "SHG: cannot write output files\n";
+        return 1;
+    }
+    return 0;
+}
 
 void usage() {
     std::cerr << "usage: SHG <Power W> <waist um> <T1 C> <T2 C> <save_only_last 0|1> <save_temperature 0|1>"
@@ -74,10 +109,6 @@ int main(int argc, char *argv[]) {
     gaussian_noise(sh, noise, seed);
 
     shg_ctx *ctx = nullptr;
-    if (p.thermal && !save_only_last) {
-        std::cerr << "SHG: THERMAL runs write the output plane only (save_only_last=1)\n";
-        return 1;
-    }
     if ((rc = shg_create(&p, &ctx))) return die("shg_create", rc);
     if ((rc = shg_set_fields(ctx, fh.data(), sh.data()))) return die("shg_set_fields", rc);
     std::cout << "\n\nEvolution in the crystal...\n" << std::endl;
@@ -88,10 +119,17 @@ int main(int argc, char *argv[]) {
         if ((rc = shg_thermal_run(ctx, 1, &st))) return die("shg_thermal_run", rc);
         std::cout << "Global iterations: " << st.outer_iterations << ", Jacobi sweeps: " << st.sweeps_total
                   << ", last mean squared update: " << st.residual_last << std::endl;
-        if ((rc = shg_get_fields(ctx, fh.data(), sh.data()))) return die("shg_get_fields", rc);
-        if (!save_complex(outdir, "Pump_out", fh, p.nx, p.ny) || !save_complex(outdir, "Signal_out", sh, p.nx, p.ny)) {
-            std::cerr << "SHG: cannot write output files\n";
-            return 1;
+        if (save_only_last) {
+            if ((rc = shg_get_fields(ctx, fh.data(), sh.data()))) return die("shg_get_fields", rc);
+            if (!save_complex(outdir, "Pump_out", fh, p.nx, p.ny) ||
+                !save_complex(outdir, "Signal_out", sh, p.nx, p.ny)) {
+                std::cerr << "SHG: cannot write output files\n";
+                return 1;
+            }
+        } else {
+            // the files the last outer iteration's CWES writes: its march again at the
+            // final temperature (the march only reads T, and DKint restarts from zero)
+            if ((rc = save_every_step(ctx, p, outdir))) return rc;
         }
         if (save_temperature) {
             const int hz = p.heat_nz > 0 ? p.heat_nz : p.nz;
@@ -110,15 +148,7 @@ int main(int argc, char *argv[]) {
             return 1;
         }
     } else {
-        for (int s = 0; s < p.nz; s++) {
-            if ((rc = shg_get_fields(ctx, fh.data(), sh.data()))) return die("shg_get_fields", rc);
-            if (!save_complex(outdir, "Pump_" + std::to_string(s), fh, p.nx, p.ny) ||
-                !save_complex(outdir, "Signal_" + std::to_string(s), sh, p.nx, p.ny)) {
-                std::cerr << "SHG: cannot write output files\n";
-                return 1;
-            }
-            if (s < p.nz - 1 && (rc = shg_march(ctx, s, 1))) return die("shg_march", rc);
-        }
+        if ((rc = save_every_step(ctx, p, outdir))) return rc;
     }
     double pin = 0, pout = 0, eta = 0;
     if ((rc = shg_powers(ctx, &pin, &pout, &eta))) return die("shg_powers", rc);

@@ -17,6 +17,7 @@ struct JacobiConst {
     float av, bv, cv, ah, bh, ch, ad, bd, cd;     // edges
     float AV, BV, CV, GV, tinfBiD;                // vertices
     double V, H, D, sv, sh, sd;                   // double parts of the edge formulas
+    int bc;                                       // SHG_BC_*: which voxels a sweep writes
 };
 
 // Per-cell physics of the THERMAL nonlinear step (src/headers/Solver.h:44-63) and setQ

@@ -19,9 +19,13 @@
 namespace shg {
 
 // ---------------------------------------------------------------------------
-// One Jacobi sweep Tf <- J(Ti, Q) of heatEquationTopOpen (src/headers/Tfield.h:113-221).
-// Thread per voxel; the y = 0 plane (Dirichlet ovens) is never written.  With RESID the
-// FP64 sum of (Tf - Ti)^2 over the block goes to part[block].
+// One Jacobi sweep Tf <- J(Ti, Q) of heatEquationTopOpen (src/headers/Tfield.h:113-221):
+// thread per voxel; the y = 0 plane (Dirichlet ovens) is never written.  With
+// c.bc == SHG_BC_OVEN_SURROUNDED, kernelHeatEquationOvenSurrounded (:224-258): the four
+// lateral faces x = 0, nx-1, y = 0, ny-1 are Dirichlet ovens and are never written; the
+// voxels it updates (inner points and the z = 0, z = L faces) use the same formulas as
+// the top-open sweep's inner points and z faces.  With RESID the FP64 sum of
+// (Tf - Ti)^2 over the block goes to part[block].
 template <bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                 const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
@@ -31,7 +35,8 @@ __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const fl
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
     double r = 0.0;
-    if (x < nx && y >= 1 && y < ny) {
+    const bool oven = c.bc == SHG_BC_OVEN_SURROUNDED;
+    if (x < nx && y >= 1 && y < ny && (!oven || (x >= 1 && x < nx - 1 && y < ny - 1))) {
         const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * ny;
         const size_t i = (size_t)z * sz + (size_t)y * sy + x;
         const bool xi = x > 0 && x < nx - 1, zi = z > 0 && z < hz - 1, yi = y < ny - 1;

@@ -11,6 +11,8 @@ SHG_CRYSTAL_SPPLT = 0
 SHG_CRYSTAL_PPLN = 1
 SHG_SCHEME_REFERENCE = 0
 SHG_SCHEME_STRANG = 1
+SHG_BC_TOP_OPEN = 0
+SHG_BC_OVEN_SURROUNDED = 1
 
 
 class ShgParams(ctypes.Structure):
@@ -28,6 +30,7 @@ class ShgParams(ctypes.Structure):
         ("heat_nz", ctypes.c_int32), ("jacobi_check", ctypes.c_int32),
         ("max_outer", ctypes.c_int32), ("jacobi_max", ctypes.c_int32),
         ("jacobi_tol", ctypes.c_double),
+        ("thermal_bc", ctypes.c_int32),
     ]
 
     def copy(self, **kw):
@@ -60,6 +63,7 @@ def default_params(**overrides):
     p.t_peltier1, p.t_peltier2 = 55.66, 55.66
     p.heat_nz, p.jacobi_check, p.max_outer, p.jacobi_max = 0, 5000, 5, 0
     p.jacobi_tol = 0.0
+    p.thermal_bc = SHG_BC_TOP_OPEN
     for k, v in overrides.items():
         setattr(p, k, v)
     return p

@@ -76,6 +76,26 @@ def test_cli_thermal_matches_library(gpu, tmp_path):
     assert "Global iterations" in out
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("thermal", [0, 1])
+def test_cli_every_step_files(gpu, tmp_path, thermal):
+    # save_only_last = 0: Pump_<s>/Signal_<s> before every step s = 0 .. nz-1; the last
+    # pair is the Pump_out/Signal_out of a save_only_last = 1 run
+    extra = ["--nz=10"] + (["--thermal=1", "--heat_nz=5", "--max_outer=2", "--jacobi_check=50",
+                            "--jacobi_max=200", "--jacobi_tol=1e-8", "--thermal_bc=oven_surrounded"]
+                           if thermal else [])
+    every, last = tmp_path / "every", tmp_path / "last"
+    every.mkdir()
+    last.mkdir()
+    _run(every, ["30", "40", "27", "28", "0", "0"] + COMMON + extra)
+    _run(last, ["30", "40", "27", "28", "1", "0"] + COMMON + extra)
+    assert len(list(every.glob("Pump_*_r.dat"))) == 10 and len(list(every.glob("Signal_*_i.dat"))) == 10
+    assert np.array_equal(_read(every, "Pump_9"), _read(last, "Pump_out"))
+    assert np.array_equal(_read(every, "Signal_9"), _read(last, "Signal_out"))
+    p0 = _read(every, "Pump_0")
+    assert np.abs(p0).max() > 0 and not np.array_equal(p0, _read(every, "Pump_5"))
+
+
 SWEEP = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG_sweep")
 
 

@@ -43,6 +43,8 @@ CASES = {
     "cfg1_strang_n128_nz50": lambda: config1_params(nz=50, scheme=SHG_SCHEME_STRANG),
     "hot_n128_nz60": lambda: config1_params(nz=60, power=50.0, waist=20.0, lx=160.0, ly=160.0,
                                              alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7),
+    # walk-off (zero in the reference's constants, supported by its propagator formula)
+    "walkoff_n128_nz40": lambda: config1_params(nz=40, power=20.0, lx=400.0, ly=400.0, rho_p=2e-3, rho_s=1e-3),
 }
 
 
@@ -108,3 +110,25 @@ def test_every_layout_variant_matches_oracle32(gpu, n):
                 assert max(e) <= TOL_FIELD, (rv, cv, e)
         tuned = ctx.tune(reps=3)
         print(f"\n  n={n} tuned variants {tuned}")
+
+
+@pytest.mark.gpu
+def test_forward_snapshots_match_step_by_step_march(gpu):
+    p = config1_params(nx=128, ny=128, nz=12)
+    fh0, sh0 = _inputs(p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(fh0, sh0)
+        sfh, ssh = ctx.forward_snapshots()
+        out_fh, out_sh = ctx.get_fields()
+        ctx.reset_fields()
+        for s in range(p.nz):
+            fh, sh = ctx.get_fields()
+            scale = np.abs(sfh[s]).max()
+            assert np.abs(fh - sfh[s]).max() <= 1e-6 * scale, s
+            assert np.abs(sh - ssh[s]).max() <= 1e-6 * max(np.abs(ssh[s]).max(), 1e-30), s
+            if s < p.nz - 1:
+                ctx.march(s, 1)
+    assert np.array_equal(sfh[0], fh0)
+    assert np.array_equal(out_fh, sfh[-1]) and np.array_equal(out_sh, ssh[-1])
+    rfh, rsh = orc.cwes(p, fh0, sh0, prec=32)
+    assert orc.rel_l2(sfh[-1], rfh) <= TOL_FIELD and orc.rel_l2(ssh[-1], rsh) <= TOL_FIELD
