@@ -40,10 +40,10 @@ COL_256 = {0: "k_col_pass<256,16,8>", 1: "k_col_pass<256,8,8>", 2: "k_col_pass<2
 
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per THERMAL z-step (row + column pass, 256x256) from rocprofv3 --pmc FETCH_SIZE
-# (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy of known size
-# (profiles/thermal_step_256x256_pmc.md: 9066 KiB with the narrow column layout).
-TRAFFIC_THERMAL_STEP = 9066 * 1024
+# HBM bytes per THERMAL z-step (row + column pass, 256x256, the tuned layouts) from
+# rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy
+# of known size.
+TRAFFIC_THERMAL_STEP = 6977 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 
 
@@ -117,6 +117,8 @@ def bench_thermal(shg, args):
     fh0, sh0 = make_inputs(p)
     ctx = shg.Context(p)
     ctx.set_fields(fh0, sh0)
+    if args.variants:
+        ctx.set_variants(*(int(v) for v in args.variants.split(",")))
     log("thermal outer loop ...")
     t0 = time.perf_counter()
     st = ctx.thermal_run(pump_from_temperature=True)
@@ -300,6 +302,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--profile-reps", type=int, default=200)
+    ap.add_argument("--variants", default="",
+                    help="ROW,COL: pin the THERMAL march's layouts instead of tuning (profiler runs, "
+                         "whose serialised launches mislead the tuner)")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
     ap.add_argument("--inverse-instances", type=int, default=1)
