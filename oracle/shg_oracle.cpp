@@ -23,6 +23,9 @@
 //   diffraction chain           — closed-form spectral multiply H^m and Gaussian beam q-law;
 //   rk4 / cwes (plain)          — plane-wave geometric-sum KAT, conservation (alpha=beta=0);
 //   jacobi                      — fixed point, x-mirror symmetry, maximum principle;
+//   jacobi (surrounded ovens)   — fixed point, oven faces untouched, steady state equal to a
+//                                  sparse solve of the physics discretisation;
+//   walk-off                    — centroid shift -m dz tan(rho) of the diffraction chain;
 //   setQ                        — closed form of its three terms at alpha/beta toggles;
 //   powers                      — Gaussian closed form 2P/(eps0 c n).
 //   thermal outer loop          — parity unpinned beyond its parts (no reference run possible).
