@@ -68,15 +68,15 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_row_tape(const InstDesc 
     const int ls = step - tp.s0;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int c = tid; c < N; c += THREADS) {
-        const float4 v = d.U[base + c];
+        const float4 v = gp(d.U)[base + c];
         lds[0][lds_pad(c)] = make_float2(v.x, v.y);
         lds[1][lds_pad(c)] = make_float2(v.z, v.w);
     }
     __syncthreads();
     if (mode != ROW_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
     __syncthreads();
-    const float4 c0 = d.coef[step];
-    const float4 c1 = d.coef[mode == ROW_MID ? step + 1 : step];
+    const float4 c0 = gp(d.coef)[step];
+    const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
     for (int c = tid; c < N; c += THREADS) {
         float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
         if (mode == ROW_START) {
@@ -94,13 +94,13 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_row_tape(const InstDesc 
     __syncthreads();
     for (int c = tid; c < N; c += THREADS) {
         const float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
-        d.U[base + c] = make_float4(P.x, P.y, S.x, S.y);
+        gp(d.U)[base + c] = make_float4(P.x, P.y, S.x, S.y);
     }
 }
 
 // Columns x0 .. x0+CW-1 of U into LDS (field k/CW of column k%CW at lds[k]).
 template <int N, int CW, int THREADS>
-__device__ __forceinline__ void cols_to_lds(const float4 *__restrict__ U, int x0, float2 (*lds)[LdsLen<N>::value]) {
+__device__ __forceinline__ void cols_to_lds(gptr<const float4> U, int x0, float2 (*lds)[LdsLen<N>::value]) {
     for (int i = threadIdx.x; i < CW * N; i += THREADS) {
         const int y = i / CW, c = i % CW;
         const float4 v = U[(size_t)y * N + x0 + c];
@@ -109,7 +109,7 @@ __device__ __forceinline__ void cols_to_lds(const float4 *__restrict__ U, int x0
     }
 }
 template <int N, int CW, int THREADS>
-__device__ __forceinline__ void lds_to_cols(float4 *__restrict__ U, int x0, float2 (*lds)[LdsLen<N>::value]) {
+__device__ __forceinline__ void lds_to_cols(gptr<float4> U, int x0, float2 (*lds)[LdsLen<N>::value]) {
     for (int i = threadIdx.x; i < CW * N; i += THREADS) {
         const int y = i / CW, c = i % CW;
         const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
     const InstDesc d = descs[0];
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
-    cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
+    cols_to_lds<N, CW, THREADS>(gp(d.U), x0, lds);
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
     __syncthreads();
@@ -137,13 +137,13 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
         const int y = i / CW, c = i % CW, x = x0 + c;
         const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
         if (tp.spec) tp.spec[soff + (size_t)y * N + x] = make_float4(P.x, P.y, S.x, S.y);
-        lds[c][lds_pad(y)] = c_mul(c_mul(d.G[x], d.G[N + y]), P);
-        lds[CW + c][lds_pad(y)] = c_mul(c_mul(d.G[2 * N + x], d.G[3 * N + y]), S);
+        lds[c][lds_pad(y)] = c_mul(c_mul(gp(d.G)[x], gp(d.G)[N + y]), P);
+        lds[CW + c][lds_pad(y)] = c_mul(c_mul(gp(d.G)[2 * N + x], gp(d.G)[3 * N + y]), S);
     }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
     __syncthreads();
-    lds_to_cols<N, CW, THREADS>(d.U, x0, lds);
+    lds_to_cols<N, CW, THREADS>(gp(d.U), x0, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
     const float4 tape = mode == ADJ_END ? tp.a[idx] : tp.a2[(size_t)(mode == ADJ_START ? ls : ls - 1) * n2 + idx];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     if (cell) {
-        const float4 v = d.U[idx];
+        const float4 v = gp(d.U)[idx];
         lds[0][lds_pad(tid)] = make_float2(v.x, v.y);
         lds[1][lds_pad(tid)] = make_float2(v.z, v.w);
     }
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
     AdjAcc acc1, acc2;
     acc_zero(acc1);
     acc_zero(acc2);
-    const float4 c_s = d.coef[step];
-    const float4 c_p = d.coef[step > 0 ? step - 1 : 0];
+    const float4 c_s = gp(d.coef)[step];
+    const float4 c_p = gp(d.coef)[step > 0 ? step - 1 : 0];
     if (cell) {
         const int o = lds_pad(tid);
         float2 P = lds[0][o], S = lds[1][o];
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
     __syncthreads();
     if (cell) {
         const float2 P = lds[0][lds_pad(tid)], S = lds[1][lds_pad(tid)];
-        d.U[idx] = make_float4(P.x, P.y, S.x, S.y);
+        gp(d.U)[idx] = make_float4(P.x, P.y, S.x, S.y);
     }
 }
 
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     const InstDesc d = descs[0];
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
-    cols_to_lds<N, CW, THREADS>(d.U, x0, lds);
+    cols_to_lds<N, CW, THREADS>(gp(d.U), x0, lds);
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
     __syncthreads();
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     for (int i = tid; i < CW * N; i += THREADS) {
         const int y = i / CW, c = i % CW, x = x0 + c;
         const float q = qxy[x].x + qxy[y].y;
-        const float2 hp = c_mul(d.G[x], d.G[N + y]), hs = c_mul(d.G[2 * N + x], d.G[3 * N + y]);
+        const float2 hp = c_mul(gp(d.G)[x], gp(d.G)[N + y]), hs = c_mul(gp(d.G)[2 * N + x], gp(d.G)[3 * N + y]);
         const float4 xs = tp.spec[soff + (size_t)y * N + x];
         const float2 lp = lds[c][lds_pad(y)], lsg = lds[CW + c][lds_pad(y)];
         // Re(conj(l) * h * xs * i a q) = -a q Im(conj(l) h xs)
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
     __syncthreads();
-    lds_to_cols<N, CW, THREADS>(d.U, x0, lds);
+    lds_to_cols<N, CW, THREADS>(gp(d.U), x0, lds);
     for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = gsum;
     __syncthreads();

@@ -8,6 +8,18 @@
 
 namespace shg {
 
+// Global-memory view of a pointer.  Buffers reached through pointers stored in memory
+// (instance tables, argument structs) are generic (flat) addresses to the compiler, and
+// flat accesses count against the LDS wait counter too, so every LDS wait of the FFT
+// exchanges would also wait for the loads in flight.  The cast states the address space.
+// (The host compilation pass, which generates no kernel code, sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T> using gptr = T *;
+#endif
+template <class T> __device__ __forceinline__ gptr<T> gp(T *p) { return (gptr<T>)p; }
+
 // Workgroups are handed to the 8 XCDs round-robin (XCD = linear workgroup id mod 8), and
 // every XCD has its own L2.  Workgroups whose lines share 128-byte cache lines (adjacent
 // column groups) are given consecutive logical indices on the same XCD: logical group

@@ -41,9 +41,11 @@ __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict_
     const int lane = threadIdx.x;
     const int t = lane % T, rr = lane / T;
     const int row = blockIdx.x * C::ROWS + rr;
+    const float4 c0 = gp(d.coef)[step];
+    const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
     for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
     __syncthreads();
-    float4 *Ur = d.U + (size_t)row * N;
+    const auto Ur = gp(d.U) + (size_t)row * N;
     float2 fa[E], fb[E];
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -54,18 +56,16 @@ __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict_
     auto sync = [] { block_sync(); };
     if (mode != ROW_START) {
         fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
-        const float4 c = d.coef[step];
 #pragma unroll
-        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c0, d.nl);
         if (snap)
 #pragma unroll
             for (int j = 0; j < E; j++)
-                snap[(size_t)row * N + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
+                gp(snap)[(size_t)row * N + t + j * T] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
     }
     if (mode != ROW_END) {
-        const float4 c = d.coef[mode == ROW_START ? step : step + 1];
 #pragma unroll
-        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c, d.nl);
+        for (int j = 0; j < E; j++) rk4(fa[j], fb[j], c1, d.nl);
         fft2fe<N, E, +1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
     }
 #pragma unroll
@@ -83,8 +83,8 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
     __shared__ float2 lds[2][C::L * CW];
     __shared__ float2 tw[N];
     const InstDesc d = descs[blockIdx.y];
-    float4 *__restrict__ U = d.U;
-    const float2 *__restrict__ G = d.G;
+    const auto U = gp(d.U);
+    const auto G = gp(d.G);
     const int tid = threadIdx.x;
     const int c = tid % CW, t = tid / CW;
     const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
@@ -97,14 +97,20 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
         fa[j] = make_float2(v.x, v.y);
         fb[j] = make_float2(v.z, v.w);
     }
-    auto sync = [] { block_sync(); };
-    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+    // the propagator factors are loaded before the transform hides their latency
     const float2 gxp = G[x], gxs = G[2 * N + x];
+    float2 gyp[E], gys[E];
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        const int y = t + j * T;
-        fa[j] = c_mul(c_mul(gxp, G[N + y]), fa[j]);
-        fb[j] = c_mul(c_mul(gxs, G[3 * N + y]), fb[j]);
+        gyp[j] = G[N + t + j * T];
+        gys[j] = G[3 * N + t + j * T];
+    }
+    auto sync = [] { block_sync(); };
+    fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        fa[j] = c_mul(c_mul(gxp, gyp[j]), fa[j]);
+        fb[j] = c_mul(c_mul(gxs, gys[j]), fb[j]);
     }
     fft2fe<N, E, -1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
 #pragma unroll
@@ -131,7 +137,9 @@ __global__ __launch_bounds__(64 * W) void k_row_split(const InstDesc *__restrict
     const InstDesc d = descs[blockIdx.y];
     if (mode == ROW_AUTO) mode = step + 1 < d.steps ? ROW_MID : ROW_END;
     const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
-    float4 *Ur = d.U + (size_t)blockIdx.x * N;
+    const auto Ur = gp(d.U) + (size_t)blockIdx.x * N;
+    const float4 c0 = gp(d.coef)[step];
+    const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int c = tid; c < N; c += THREADS) {
         const float4 v = Ur[c];
@@ -141,13 +149,11 @@ __global__ __launch_bounds__(64 * W) void k_row_split(const InstDesc *__restrict
     __syncthreads();
     if (mode != ROW_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
     __syncthreads();
-    const float4 c0 = d.coef[step];
-    const float4 c1 = d.coef[mode == ROW_START ? step : (mode == ROW_MID ? step + 1 : step)];
     for (int c = tid; c < N; c += THREADS) {
         float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
         if (mode != ROW_START) {
             rk4(P, S, c0, d.nl);
-            if (snap) snap[(size_t)blockIdx.x * N + c] = make_float4(P.x, P.y, S.x, S.y);
+            if (snap) gp(snap)[(size_t)blockIdx.x * N + c] = make_float4(P.x, P.y, S.x, S.y);
         }
         if (mode != ROW_END) rk4(P, S, c1, d.nl);
         lds[0][lds_pad(c)] = P;
@@ -171,9 +177,10 @@ __global__ __launch_bounds__(128 * CW) void k_col_split(const InstDesc *__restri
     static_assert(T <= 64, "one wavefront per field transform");
     __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
+    constexpr int CPT = (CW * N + THREADS - 1) / THREADS;   // cells per thread outside the transforms
     const InstDesc d = descs[blockIdx.y];
-    float4 *__restrict__ U = d.U;
-    const float2 *__restrict__ G = d.G;
+    const auto U = gp(d.U);
+    const auto G = gp(d.G);
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63;
     const int x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
@@ -183,13 +190,28 @@ __global__ __launch_bounds__(128 * CW) void k_col_split(const InstDesc *__restri
         lds[c][lds_pad(y)] = make_float2(v.x, v.y);
         lds[CW + c][lds_pad(y)] = make_float2(v.z, v.w);
     }
+    // the propagator factors of the thread's cells, loaded before the transform
+    float2 gxp[CPT], gyp[CPT], gxs[CPT], gys[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int i = tid + q * THREADS, ii = i < CW * N ? i : 0;
+        const int y = ii / CW, x = x0 + ii % CW;
+        gxp[q] = G[x];
+        gyp[q] = G[N + y];
+        gxs[q] = G[2 * N + x];
+        gys[q] = G[3 * N + y];
+    }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
     __syncthreads();
-    for (int i = tid; i < CW * N; i += THREADS) {
-        const int y = i / CW, c = i % CW, x = x0 + c;
-        lds[c][lds_pad(y)] = c_mul(c_mul(G[x], G[N + y]), lds[c][lds_pad(y)]);
-        lds[CW + c][lds_pad(y)] = c_mul(c_mul(G[2 * N + x], G[3 * N + y]), lds[CW + c][lds_pad(y)]);
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int i = tid + q * THREADS;
+        if (i < CW * N) {
+            const int y = i / CW, c = i % CW;
+            lds[c][lds_pad(y)] = c_mul(c_mul(gxp[q], gyp[q]), lds[c][lds_pad(y)]);
+            lds[CW + c][lds_pad(y)] = c_mul(c_mul(gxs[q], gys[q]), lds[CW + c][lds_pad(y)]);
+        }
     }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
@@ -241,7 +263,7 @@ __global__ __launch_bounds__(256) void k_power_final(const double2 *__restrict__
 // One block per table entry: FP64 sums of |FH|^2 and |SH|^2 of that instance.
 __global__ __launch_bounds__(256) void k_power_batch(const InstDesc *__restrict__ descs, const int *__restrict__ ncell,
                                                      double2 *out) {
-    const float4 *U = descs[blockIdx.x].U;
+    const auto U = gp(descs[blockIdx.x].U);
     const int n = ncell[blockIdx.x];
     double sp = 0.0, ss = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) {

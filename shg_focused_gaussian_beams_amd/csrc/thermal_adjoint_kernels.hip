@@ -17,6 +17,7 @@
 #include "adjoint_cell.h"
 #include "adjoint_kernels.h"
 #include "fft_lds.h"
+#include "pass_cfg.h"
 #include "physics.h"
 #include "thermal_adjoint_kernels.h"
 #include "thermal_cell.h"
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
     static_assert(W >= 2 && T <= 64 && THREADS >= N, "one wavefront per field transform; one cell per thread");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
-    float4 *U = descs[0].U;
+    const auto U = gp(descs[0].U);
     const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
     const bool cell = tid < N;
     const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + (cell ? tid : 0);
