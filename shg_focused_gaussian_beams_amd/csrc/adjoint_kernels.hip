@@ -66,6 +66,8 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_row_tape(const InstDesc 
     const int tid = threadIdx.x, f = tid >> 6, lane = tid & 63;
     const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N;
     const int ls = step - tp.s0;
+    const float4 c0 = gp(d.coef)[step];
+    const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int c = tid; c < N; c += THREADS) {
         const float4 v = gp(d.U)[base + c];
@@ -75,8 +77,6 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_row_tape(const InstDesc 
     __syncthreads();
     if (mode != ROW_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
     __syncthreads();
-    const float4 c0 = gp(d.coef)[step];
-    const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
     for (int c = tid; c < N; c += THREADS) {
         float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
         if (mode == ROW_START) {
@@ -125,20 +125,34 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
     static_assert(T <= 64, "one wavefront per field transform");
     __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
+    constexpr int CPT = (CW * N + THREADS - 1) / THREADS;   // cells per thread (guarded: 64 x 2 < 256)
     const InstDesc d = descs[0];
+    const auto G = gp(d.G);
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     cols_to_lds<N, CW, THREADS>(gp(d.U), x0, lds);
+    // the propagator factors of the thread's cells, loaded before the transform
+    float2 gxp[CPT], gyp[CPT], gxs[CPT], gys[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int i = tid + q * THREADS, ii = i < CW * N ? i : 0, y = ii / CW, x = x0 + ii % CW;
+        gxp[q] = G[x];
+        gyp[q] = G[N + y];
+        gxs[q] = G[2 * N + x];
+        gys[q] = G[3 * N + y];
+    }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
     __syncthreads();
     const size_t soff = (size_t)(step - tp.s0) * N * N;
-    for (int i = tid; i < CW * N; i += THREADS) {
-        const int y = i / CW, c = i % CW, x = x0 + c;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int i = tid + q * THREADS, y = i / CW, c = i % CW, x = x0 + c;
+        if (i >= CW * N) break;
         const float2 P = lds[c][lds_pad(y)], S = lds[CW + c][lds_pad(y)];
-        if (tp.spec) tp.spec[soff + (size_t)y * N + x] = make_float4(P.x, P.y, S.x, S.y);
-        lds[c][lds_pad(y)] = c_mul(c_mul(gp(d.G)[x], gp(d.G)[N + y]), P);
-        lds[CW + c][lds_pad(y)] = c_mul(c_mul(gp(d.G)[2 * N + x], gp(d.G)[3 * N + y]), S);
+        if (tp.spec) gp(tp.spec)[soff + (size_t)y * N + x] = make_float4(P.x, P.y, S.x, S.y);
+        lds[c][lds_pad(y)] = c_mul(c_mul(gxp[q], gyp[q]), P);
+        lds[CW + c][lds_pad(y)] = c_mul(c_mul(gxs[q], gys[q]), S);
     }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
@@ -171,6 +185,8 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
     // the cell's taped forward state, loaded before the transform hides its latency:
     // START: A''_s;  MID: A''_{s-1};  END: A_s (the checkpoint)
     const float4 tape = mode == ADJ_END ? tp.a[idx] : tp.a2[(size_t)(mode == ADJ_START ? ls : ls - 1) * n2 + idx];
+    const float4 c_s = gp(d.coef)[step];
+    const float4 c_p = gp(d.coef)[step > 0 ? step - 1 : 0];
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     if (cell) {
         const float4 v = gp(d.U)[idx];
@@ -183,8 +199,6 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
     AdjAcc acc1, acc2;
     acc_zero(acc1);
     acc_zero(acc2);
-    const float4 c_s = gp(d.coef)[step];
-    const float4 c_p = gp(d.coef)[step > 0 ? step - 1 : 0];
     if (cell) {
         const int o = lds_pad(tid);
         float2 P = lds[0][o], S = lds[1][o];
@@ -230,20 +244,39 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     __shared__ float2 lds[2 * CW][L];
     __shared__ float2 tw[N];
     __shared__ double red[THREADS / 64];
+    constexpr int CPT = (CW * N + THREADS - 1) / THREADS;   // cells per thread (guarded: 64 x 2 < 256)
     const InstDesc d = descs[0];
+    const auto G = gp(d.G);
     const int tid = threadIdx.x, k = tid >> 6, lane = tid & 63, x0 = xcd_group(blockIdx.x, gridDim.x) * CW;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     cols_to_lds<N, CW, THREADS>(gp(d.U), x0, lds);
+    // the thread's cells' propagator factors, frequencies and taped spectra, loaded
+    // before the transform
+    const size_t soff = (size_t)(step - tp.s0) * N * N;
+    float2 gxp[CPT], gyp[CPT], gxs[CPT], gys[CPT];
+    float qc[CPT];
+    float4 xsp[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int i = tid + q * THREADS, ii = i < CW * N ? i : 0, y = ii / CW, x = x0 + ii % CW;
+        gxp[q] = G[x];
+        gyp[q] = G[N + y];
+        gxs[q] = G[2 * N + x];
+        gys[q] = G[3 * N + y];
+        qc[q] = qxy[x].x + qxy[y].y;
+        xsp[q] = gp(tp.spec)[soff + (size_t)y * N + x];
+    }
     __syncthreads();
     if (lane < T) fft_lds_field<N, E, +1>(lds[k], lane, tw);
     __syncthreads();
-    const size_t soff = (size_t)(step - tp.s0) * N * N;
     double gsum = 0.0;
-    for (int i = tid; i < CW * N; i += THREADS) {
-        const int y = i / CW, c = i % CW, x = x0 + c;
-        const float q = qxy[x].x + qxy[y].y;
-        const float2 hp = c_mul(gp(d.G)[x], gp(d.G)[N + y]), hs = c_mul(gp(d.G)[2 * N + x], gp(d.G)[3 * N + y]);
-        const float4 xs = tp.spec[soff + (size_t)y * N + x];
+#pragma unroll
+    for (int qq = 0; qq < CPT; qq++) {
+        const int i = tid + qq * THREADS, y = i / CW, c = i % CW;
+        if (i >= CW * N) break;
+        const float q = qc[qq];
+        const float2 hp = c_mul(gxp[qq], gyp[qq]), hs = c_mul(gxs[qq], gys[qq]);
+        const float4 xs = xsp[qq];
         const float2 lp = lds[c][lds_pad(y)], lsg = lds[CW + c][lds_pad(y)];
         // Re(conj(l) * h * xs * i a q) = -a q Im(conj(l) h xs)
         const float2 tp_ = c_mul(c_mul(cj(lp), hp), f2(xs.x, xs.y));
