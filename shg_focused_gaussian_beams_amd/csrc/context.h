@@ -26,8 +26,7 @@ struct AdjointBuffers {
     hipEvent_t ev_u = nullptr;                 // the forward state buffer is free for `ts`
     hipEvent_t ev_tape[2] = {nullptr, nullptr};  // tape slot k written
     hipEvent_t ev_free[2] = {nullptr, nullptr};  // tape slot k read by the adjoint
-    hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;  // captured forward / adjoint for interval k
-    int g_row_v = -1, g_col_v = -1;            // plain-pass layouts the forward graph was captured with
+    hipGraphExec_t g_adj = nullptr;            // captured adjoint march for interval k
     double2 *pump_df = nullptr, *pump_dn = nullptr;  // d(pump)/d(focus), d(pump)/d(n_p) per cell (device)
     double pump_T = -1e300;                    // temperature the pump derivative fields were built at
     float2 *grad = nullptr;                    // d eta / d FH(z = 0) per cell
