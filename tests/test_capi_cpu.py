@@ -45,7 +45,8 @@ def test_default_params_match_reference_constants():
 
 
 @pytest.mark.parametrize("bad", [dict(nx=64, ny=128), dict(nx=100, ny=100), dict(nz=1), dict(crystal=7),
-                                 dict(scheme=3), dict(lz=0.0), dict(thermal=1, heat_nz=500)])
+                                 dict(scheme=3), dict(lz=0.0), dict(thermal=1, heat_nz=500),
+                                 dict(thermal=1, thermal_bc=7)])
 def test_create_rejects_bad_parameters_without_device(bad):
     lib = shg.load_library()
     p = default_params(**bad)
@@ -110,3 +111,7 @@ def test_cli_rejects_bad_command_lines(tmp_path):
     r = subprocess.run([exe, "30", "29", "55", "55", "1", "0", "--nope=1"], capture_output=True, text=True,
                        cwd=tmp_path)
     assert r.returncode == 1 and "unknown option" in r.stderr
+    # parameter checks run before any device call
+    r = subprocess.run([exe, "30", "29", "55", "55", "1", "0", "--thermal=1", "--thermal_bc=sideways"],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1 and "thermal_bc" in r.stderr
