@@ -1,5 +1,6 @@
 // cli_common.h — pieces shared by the SHG and SHG_sweep programs: the seeded SH noise,
-// the reference's .dat writers and the --key=value overrides.
+// the reference's .dat writers (with an optional binary sidecar per file) and the
+// --key=value overrides.
 #pragma once
 #include <sys/time.h>
 
@@ -17,6 +18,19 @@
 #include "shg.h"
 
 namespace shgcli {
+
+// --sidecar=1: next to every <name>_r.dat / _i.dat pair also write <name>.c64 (the plane
+// as raw interleaved complex64, row-major, x fastest) and next to every Tfinal_<z>.dat
+// also Tfinal_<z>.f32 (raw float32): the exact floats, for programs and tests that
+// should not parse 20-digit text.
+inline bool g_sidecar = false;
+
+bool save_raw(const std::string &path, const float *a, size_t count) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.write(reinterpret_cast<const char *>(a), (std::streamsize)(count * sizeof(float)));
+    return (bool)f;
+}
 
 double seconds() {
     timeval tp;
@@ -52,6 +66,7 @@ void gaussian_noise(std::vector<float> &out, double sigma, uint64_t seed) {
 
 // saveMatrixComplex (src/headers/Files.h:36-57): NY rows of NX tab-terminated values.
 bool save_complex(const std::string &dir, const std::string &name, const std::vector<float> &a, int nx, int ny) {
+    if (g_sidecar && !save_raw(dir + name + ".c64", a.data(), 2 * (size_t)nx * ny)) return false;
     for (int part = 0; part < 2; part++) {
         std::ofstream f(dir + name + (part == 0 ? "_r.dat" : "_i.dat"));
         if (!f) return false;
@@ -66,6 +81,9 @@ bool save_complex(const std::string &dir, const std::string &name, const std::ve
 // saveTensorReal (src/headers/Files.h:60-79): one file per z slice, default precision.
 bool save_tensor(const std::string &dir, const std::string &name, const std::vector<float> &t, int nx, int ny, int nz) {
     for (int iz = 0; iz < nz; iz++) {
+        if (g_sidecar && !save_raw(dir + name + "_" + std::to_string(iz) + ".f32", t.data() + (size_t)iz * ny * nx,
+                                   (size_t)ny * nx))
+            return false;
         std::ofstream f(dir + name + "_" + std::to_string(iz) + ".dat");
         if (!f) return false;
         for (int iy = 0; iy < ny; iy++) {
@@ -109,6 +127,7 @@ bool parse_override(shg_params &p, const std::string &key, const std::string &va
     else if (key == "jacobi_tol") p.jacobi_tol = d;
     else if (key == "seed") seed = (uint64_t)std::strtoull(val.c_str(), nullptr, 10);
     else if (key == "noise") noise = d;
+    else if (key == "sidecar") g_sidecar = i != 0;
     else if (key == "outdir") { outdir = val; if (!outdir.empty() && outdir.back() != '/') outdir += "/"; }
     else return false;
     return true;

@@ -13,7 +13,7 @@
 // --deff --period --alpha_p --alpha_s --beta_s --rho_p --rho_s --k_th --h_conv --focus
 // --t_inf --crystal=spplt|ppln --scheme=reference|strang --thermal=0|1 --heat_nz
 // --thermal_bc=top_open|oven_surrounded --jacobi_check --max_outer --jacobi_max
-// --jacobi_tol --seed --noise --outdir
+// --jacobi_tol --seed --noise --outdir --sidecar=1 (raw .c64/.f32 copies of the outputs)
 #include <algorithm>
 #include <atomic>
 #include <thread>

@@ -54,7 +54,7 @@ def test_cli_plain_matches_library(gpu, tmp_path):
 def test_cli_thermal_matches_library(gpu, tmp_path):
     extra = ["--thermal=1", "--nz=12", "--heat_nz=6", "--alpha_p=1e-7", "--alpha_s=4e-6", "--beta_s=1e-7",
              "--max_outer=2", "--jacobi_check=50", "--jacobi_max=400", "--jacobi_tol=1e-8"]
-    out = _run(tmp_path, ["30", "40", "27", "27", "1", "1"] + COMMON + extra)
+    out = _run(tmp_path, ["30", "40", "27", "27", "1", "1", "--sidecar=1"] + COMMON + extra)
     fh, sh = _read(tmp_path, "Pump_out"), _read(tmp_path, "Signal_out")
     temps = sorted(tmp_path.glob("Tfinal_*.dat"))
     assert len(temps) == 6, temps
@@ -75,6 +75,11 @@ def test_cli_thermal_matches_library(gpu, tmp_path):
     assert max(e) < 1e-6, e
     assert np.abs(t5 - T[5]).max() <= 1e-4 * np.abs(T[5]).max()   # 6 significant digits in the file
     assert "Global iterations" in out
+    # the binary sidecars hold the same floats exactly (the .dat pair has 20 digits)
+    assert np.array_equal(np.fromfile(tmp_path / "Pump_out.c64", np.complex64).reshape(64, 64), fh)
+    assert np.array_equal(np.fromfile(tmp_path / "Signal_out.c64", np.complex64).reshape(64, 64), sh)
+    t5b = np.fromfile(tmp_path / "Tfinal_5.f32", np.float32).reshape(64, 64)
+    assert np.abs(t5b - t5).max() <= 1e-4 * np.abs(t5).max() and np.abs(t5b - T[5]).max() <= 1e-5 * np.abs(T[5]).max()
 
 
 @pytest.mark.gpu
