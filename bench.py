@@ -71,26 +71,43 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=15.0):
-    """oracle32 THERMAL march (per-cell n(), the reference's O(s) dk re-integration) on
-    one core, over the first k z-steps of the workload grid."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as orc
+def _thermal_sample(orc, p, T, fh0, sh0, budget_s, lib=None):
+    """Times oracle32's THERMAL march over the first k z-steps of the workload grid, k
+    sized so that the run takes about budget_s; returns (k, seconds)."""
     hz = p.heat_nz
     t0 = time.perf_counter()
     q = p.copy(nz=2, heat_nz=2)
-    orc.thermal_cwes(q, T[[0, -1]], fh0, sh0)
+    orc.thermal_cwes(q, T[[0, -1]], fh0, sh0, lib=lib)
     one = (time.perf_counter() - t0) / 2
     k = max(2, min(p.nz, int(budget_s / max(one, 1e-6))))
     q = p.copy(nz=k, heat_nz=max(2, min(k, hz)), lz=p.lz * (k - 1) / (p.nz - 1))
     idx = [min(hz - 1, int(i * hz / k)) for i in range(q.heat_nz)]
     t0 = time.perf_counter()
-    orc.thermal_cwes(q, T[idx], fh0, sh0)
-    dt = time.perf_counter() - t0
-    cells = p.nx * p.ny * k
-    return {"value": cells / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+    orc.thermal_cwes(q, T[idx], fh0, sh0, lib=lib)
+    return k, time.perf_counter() - t0
+
+
+def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=12.0, budget_o0_s=4.0):
+    """oracle32 THERMAL march (per-cell n(), the reference's O(s) dk re-integration) on
+    one core (this process pinned to one CPU for the measurement), over the first k
+    z-steps of the workload grid: the -O3 build (the baseline) and the -O0 build (the
+    optimisation level of the reference's own build line)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    affinity = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(affinity)})
+    try:
+        k, dt = _thermal_sample(orc, p, T, fh0, sh0, budget_s)
+        k0, dt0 = _thermal_sample(orc, p, T, fh0, sh0, budget_o0_s, lib=orc.load(orc.ORACLE_O0_SO))
+    finally:
+        os.sched_setaffinity(0, affinity)
+    cells = p.nx * p.ny
+    return {"value": cells * k / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"oracle32 -O3 THERMAL march (per-cell n(), own radix-2 FFT, no FFTW on the host) on "
-                      f"{p.nx}x{p.ny}, first {k} of {p.nz} z-steps, {dt:.1f} s; host CPU: {cpu_model()}"}
+                      f"{p.nx}x{p.ny}, first {k} of {p.nz} z-steps, {dt:.1f} s, pinned to one core; host CPU: "
+                      f"{cpu_model()}, {os.cpu_count()} CPUs visible",
+            "value_O0": cells * k0 / dt0,
+            "sample_O0": f"the same at -O0: first {k0} z-steps, {dt0:.1f} s"}
 
 
 def copy_peak_gbps(mib=1024, reps=5):

@@ -133,15 +133,19 @@ def set_q(p, Tslice, fh, sh):
     return q
 
 
-def thermal_cwes(p, T, fh0, sh0, Q=None, prec=32):
+ORACLE_O0_SO = os.path.join(ORACLE_DIR, "_build", "liboracle_O0.so")
+
+
+def thermal_cwes(p, T, fh0, sh0, Q=None, prec=32, lib=None):
     """One THERMAL z-march at temperature T (FP32 restatement, or FP64 with prec=64);
-    returns (fh_out, sh_out, Q)."""
+    returns (fh_out, sh_out, Q).  `lib`: another build of the oracle (load(path))."""
     hz = p.heat_nz if p.heat_nz > 0 else p.nz
     T = np.ascontiguousarray(T, np.float32).reshape(hz, p.ny, p.nx)
     Q = np.zeros_like(T) if Q is None else np.ascontiguousarray(Q, np.float32).copy()
     fh = _c128(fh0).copy()
     sh = _c128(sh0).copy()
-    fn = load().oracle_thermal_cwes if prec == 32 else load().oracle_thermal_cwes64
+    lib = load() if lib is None else lib
+    fn = lib.oracle_thermal_cwes if prec == 32 else lib.oracle_thermal_cwes64
     fn(ctypes.byref(p), _fp(T), _dp(fh), _dp(sh), _fp(Q))
     return fh, sh, Q
 
