@@ -6,6 +6,7 @@
 
 #include "../../include/shg.h"
 #include "nl_step.h"
+#include "sincos_phase.h"
 #include "thermal_kernels.h"
 
 namespace shg {
@@ -65,31 +66,11 @@ __device__ __forceinline__ CellPhys cell_from_table(float2 nn, float4 cf) {
 // The coefficients of a THERMAL RK4 stage group are (c1, c2) = (i kp e^{-i phi},
 // i ks e^{+i phi}) with phi = (float)(DKint * zfac / (s+1)); zr = zfac / (s+1) is
 // precomputed in FP64 (the float phase is the same but for rare one-ulp double roundings).
-// sin and cos of the three stage-group phases phi_g = (float)(DKint * zr_g), g = 0, 1, 2:
-// one sincosf of phi_0, then phi_1 and phi_2 by rotation through delta_g = phi_g - phi_0
-// (the groups differ by dz/4 and dz/2 of the step's z, so |delta| is the phase mismatch
-// over a fraction of one step) with the Taylor series of sin and cos, truncated where the
-// remainder is below 1e-8 for |delta| <= 0.5; larger deltas take sincosf.  The values
-// agree with sincosf(phi_g) to a few ulp.
+// sin and cos of the three stage-group phases phi_g = (float)(DKint * zr_g), g = 0, 1, 2,
+// each by sincos_phase (within 2 ulp of the correctly rounded values, branch-free).
 __device__ __forceinline__ void phase_sincos3(float dkint, const ThermStep &st, float sn[3], float cs[3]) {
-    float phi[3];
 #pragma unroll
-    for (int g = 0; g < 3; g++) phi[g] = (float)((double)dkint * st.zr[g]);
-    sincosf(phi[0], &sn[0], &cs[0]);
-#pragma unroll
-    for (int g = 1; g < 3; g++) {
-        const float d = phi[g] - phi[0];
-        if (fabsf(d) <= 0.5f) {
-            const float d2 = d * d;
-            const float sd = d * (1.0f - d2 * (1.0f / 6.0f) * (1.0f - d2 * (1.0f / 20.0f) * (1.0f - d2 * (1.0f / 42.0f))));
-            const float cd = 1.0f - d2 * 0.5f * (1.0f - d2 * (1.0f / 12.0f) *
-                                                           (1.0f - d2 * (1.0f / 30.0f) * (1.0f - d2 * (1.0f / 56.0f))));
-            sn[g] = sn[0] * cd + cs[0] * sd;
-            cs[g] = cs[0] * cd - sn[0] * sd;
-        } else {
-            sincosf(phi[g], &sn[g], &cs[g]);
-        }
-    }
+    for (int g = 0; g < 3; g++) sincos_phase((float)((double)dkint * st.zr[g]), sn[g], cs[g]);
 }
 
 // THERMAL solverRK4 (src/headers/Solver.h:197-213): stage increments 0, .25, .25, .5.

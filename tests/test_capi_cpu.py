@@ -115,3 +115,15 @@ def test_cli_rejects_bad_command_lines(tmp_path):
     r = subprocess.run([exe, "30", "29", "55", "55", "1", "0", "--thermal=1", "--thermal_bc=sideways"],
                        capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 1 and "thermal_bc" in r.stderr
+
+
+def test_phase_sincos_within_two_ulp(tmp_path):
+    # sincos_phase (the THERMAL stage-group phases): double-precision reduction by pi/2 and
+    # Cephes polynomials; against long-double sin/cos of the same float argument
+    exe = tmp_path / "sincos_emul"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", str(exe), os.path.join(ROOT, "tests", "cpu_emul", "sincos_emul.cpp")],
+                   check=True)
+    for xmax in (1.0, 30.0, 1e3, 1e5):
+        out = subprocess.run([str(exe), str(xmax), "300000", "7"], check=True, capture_output=True, text=True).stdout
+        max_abs, max_ulp = map(float, out.split())
+        assert max_ulp <= 2.0 and max_abs <= 1.5e-7, (xmax, max_abs, max_ulp)
