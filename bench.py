@@ -43,7 +43,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256, the tuned layouts) from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy
 # of known size.
-TRAFFIC_THERMAL_STEP = 6977 * 1024
+TRAFFIC_THERMAL_STEP = 6905 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 
 
