@@ -45,6 +45,10 @@ HBM_PEAK_GBPS = 8000.0
 # of known size.
 TRAFFIC_THERMAL_STEP = 6905 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
+# VALU lane-operations per plain cell-step of the batch's passes (rocprofv3 SQ_INSTS_VALU
+# x 64 per cell: 393 row pass + 141 column pass at 512^2) and the chip's FP32 issue rate
+# without packed FP32 (256 CU x 4 SIMD x 16 lanes x 2.4 GHz).
+VALU_OPS_PER_CELL_STEP, VALU_PEAK_LANE_OPS = 534, 39.3e12
 
 
 def log(*a):
@@ -407,6 +411,7 @@ def main():
             "instances": bt["count"], "s_per_batch": bt["s"], "cell_steps_per_s": bt["cells"] / bt["s"],
             "instances_per_s": bt["count"] / bt["s"], "algorithmic_GBps": bt["cells"] * B_PLAIN / bt["s"] / 1e9,
             "hbm_frac": bt["cells"] * B_PLAIN / bt["s"] / 1e9 / HBM_PEAK_GBPS,
+            "valu_frac_est": bt["cells"] * VALU_OPS_PER_CELL_STEP / bt["s"] / VALU_PEAK_LANE_OPS,
             "one_at_a_time_s": bt["single_s"], "speedup_vs_one_at_a_time": bt["single_s"] / bt["s"],
             "one_at_a_time_sampled_instances": bt["sampled"], "mean_eta": bt["eta_mean"]}
         log("forward+adjoint config 4 ...")
