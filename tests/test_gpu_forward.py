@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
-from shg_focused_gaussian_beams_amd.params import default_params, config1_params, SHG_SCHEME_STRANG
+from shg_focused_gaussian_beams_amd.params import default_params, config1_params, SHG_SCHEME_STRANG, SHG_CRYSTAL_PPLN
 
 TOL_FIELD = 1e-4
 TOL_ETA = 1e-4
@@ -45,6 +45,8 @@ CASES = {
                                              alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7),
     # walk-off (zero in the reference's constants, supported by its propagator formula)
     "walkoff_n128_nz40": lambda: config1_params(nz=40, power=20.0, lx=400.0, ly=400.0, rho_p=2e-3, rho_s=1e-3),
+    # MgO:PPLN (the crystal model the reference's README names but does not implement)
+    "ppln_n128_nz40": lambda: config1_params(nz=40, crystal=SHG_CRYSTAL_PPLN, qpm_period=6.92, temp=40.0),
 }
 
 
