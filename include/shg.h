@@ -18,7 +18,10 @@
  *     std::complex<float> / fftwf_complex), indexed IDX(x,y) = y*nx + x, x fastest
  *     (src/headers/Common.h:27-29).  Host buffers are owned by the caller and copied.
  *   - A context owns all of its device memory and one HIP stream.  Calls on one
- *     context are not reentrant; use one context per host thread.
+ *     context are not reentrant; use one context per host thread.  Contexts on
+ *     different threads run concurrently: the library copies only through its own
+ *     non-blocking streams, so one thread's copies never break another's graph
+ *     capture (SHG_sweep runs THERMAL points this way).
  *   - Units are the reference's: um, ps, V, W, degC.  Fields are in V/um.
  */
 #ifndef SHG_H

@@ -87,6 +87,11 @@ int shg_internal_fail(int code, const std::string &msg);
 shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s);
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
 double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);
+// A copy ordered on `st` that has finished on return.  The library copies through its own
+// (non-blocking) streams only, never the legacy null stream: a legacy-stream copy issued by
+// one host thread invalidates a graph capture running in another, so contexts driven from
+// several threads at once (one per point of a sweep) must not use it.
+hipError_t shg_internal_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st);
 // Captures the work `enqueue` issues on `st` (and on streams forked from it through
 // events) into an executable graph.
 int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out);

@@ -101,11 +101,11 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
             q[2 * i] = (float)(2 * PI * PI * fx * fx);
             q[2 * i + 1] = (float)(2 * PI * PI * fy * fy);
         }
-        AHIP_TRY(hipMemcpy(b.qxy, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice));
+        AHIP_TRY(shg_internal_copy(b.qxy, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
         shg::InstDesc d{};
-        AHIP_TRY(hipMemcpy(&d, c->d_desc, sizeof(d), hipMemcpyDeviceToHost));
+        AHIP_TRY(shg_internal_copy(&d, c->d_desc, sizeof(d), hipMemcpyDeviceToHost, c->stream));
         d.U = b.lam;
-        AHIP_TRY(hipMemcpy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice));
+        AHIP_TRY(shg_internal_copy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
     }
     if (b.k != K || b.nseg != nseg) {
         if (b.g_adj) { (void)hipGraphExecDestroy(b.g_adj); b.g_adj = nullptr; }
@@ -169,8 +169,8 @@ int ensure_pump_fields(shg_ctx *c, double np, double T) {
             pdf[(size_t)iy * n + ix] = dA_dM * cd(0.0, -1.0 / zr);
             pdn[(size_t)iy * n + ix] = -A / (2.0 * np) + dA_dM * cd(0.0, f / (zr * np));
         }
-    AHIP_TRY(hipMemcpy(b.pump_df, pdf.data(), n2 * sizeof(double2), hipMemcpyHostToDevice));
-    AHIP_TRY(hipMemcpy(b.pump_dn, pdn.data(), n2 * sizeof(double2), hipMemcpyHostToDevice));
+    AHIP_TRY(shg_internal_copy(b.pump_df, pdf.data(), n2 * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    AHIP_TRY(shg_internal_copy(b.pump_dn, pdn.data(), n2 * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     b.pump_T = T;
     return SHG_OK;
 }
@@ -561,7 +561,7 @@ int shg_adjoint_temperature_map(shg_ctx *c, double *map) {
     if (!c->p.thermal || !c->adj.map_valid)
         return shg_internal_fail(SHG_E_STATE, "shg_adjoint_temperature_map: run shg_adjoint on a THERMAL context first");
     const size_t n2 = (size_t)c->n * c->n;
-    AHIP_TRY(hipMemcpy(map, c->adj.Ap, n2 * c->hz * sizeof(double), hipMemcpyDeviceToHost));
+    AHIP_TRY(shg_internal_copy(map, c->adj.Ap, n2 * c->hz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     map[(size_t)(c->n / 2) * c->n + c->n / 2] += c->adj.map_centre_extra;   // slice 0, grid centre
     return SHG_OK;
 }

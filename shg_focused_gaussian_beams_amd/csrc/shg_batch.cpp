@@ -24,6 +24,7 @@
 
 // Error reporting shared with shg_capi.cpp.
 int shg_internal_fail(int code, const std::string &msg);
+hipError_t shg_internal_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st);
 int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out);
 int shg_internal_time_graph(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps, const std::function<int()> &one,
                             float *ms);
@@ -300,7 +301,7 @@ int shg_batch_create(const shg_params *params, int count, shg_batch **out) {
         if (!b->cls_cnt[c]) continue;
         const auto tw = shg::phys::twiddle_table(kSizes[c]);
         if (hipMalloc(&b->tw[c], tw.size() * sizeof(float)) != hipSuccess ||
-            hipMemcpy(b->tw[c], tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            shg_internal_copy(b->tw[c], tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) != hipSuccess)
             return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: twiddle upload failed"));
     }
     std::vector<float> gh(2 * g_total), ch(4 * coef_total);
@@ -326,10 +327,12 @@ int shg_batch_create(const shg_params *params, int count, shg_batch **out) {
         d.steps = b->steps[s];
         nc[s] = p.nx * p.nx;
     }
-    if (hipMemcpy(b->G, gh.data(), gh.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->coef, ch.data(), ch.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->desc, dh.data(), dh.size() * sizeof(shg::InstDesc), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->ncell, nc.data(), nc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    hipStream_t st = b->stream;
+    if (shg_internal_copy(b->G, gh.data(), gh.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
+        shg_internal_copy(b->coef, ch.data(), ch.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
+        shg_internal_copy(b->desc, dh.data(), dh.size() * sizeof(shg::InstDesc), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        shg_internal_copy(b->ncell, nc.data(), nc.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
         return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: table upload failed"));
     *out = b;
     return SHG_OK;
@@ -369,7 +372,7 @@ int shg_batch_init_fields(shg_batch *b, double noise_rel, uint64_t seed) {
     }
     shg::PumpArgs *d = nullptr;
     BHIP_TRY(hipMalloc(&d, a.size() * sizeof(shg::PumpArgs)));
-    hipError_t e = hipMemcpy(d, a.data(), a.size() * sizeof(shg::PumpArgs), hipMemcpyHostToDevice);
+    hipError_t e = shg_internal_copy(d, a.data(), a.size() * sizeof(shg::PumpArgs), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = shg::launch_pump_init(d, b->count, max_cells, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(d);
@@ -387,7 +390,7 @@ int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *
         h[4 * i] = fh[2 * i]; h[4 * i + 1] = fh[2 * i + 1];
         h[4 * i + 2] = sh[2 * i]; h[4 * i + 3] = sh[2 * i + 1];
     }
-    BHIP_TRY(hipMemcpy(b->U0 + b->cell_off[s], h.data(), nc * sizeof(float4), hipMemcpyHostToDevice));
+    BHIP_TRY(shg_internal_copy(b->U0 + b->cell_off[s], h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, b->stream));
     return after_fields_set(b);
 }
 
@@ -440,7 +443,8 @@ static int batch_copy_out(shg_batch *b, const float4 *src_base, int index, float
     const int s = b->slot_of[index];
     const size_t nc = (size_t)b->n[s] * b->n[s];
     std::vector<float> h(4 * nc);
-    BHIP_TRY(hipMemcpy(h.data(), src_base + b->cell_off[s], nc * sizeof(float4), hipMemcpyDeviceToHost));
+    BHIP_TRY(shg_internal_copy(h.data(), src_base + b->cell_off[s], nc * sizeof(float4), hipMemcpyDeviceToHost,
+                               b->stream));
     for (size_t i = 0; i < nc; i++) {
         fh[2 * i] = h[4 * i]; fh[2 * i + 1] = h[4 * i + 1];
         sh[2 * i] = h[4 * i + 2]; sh[2 * i + 1] = h[4 * i + 3];

@@ -46,6 +46,11 @@ bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512;
 
 int shg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
 
+hipError_t shg_internal_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+}
+
 int shg_internal_capture(hipStream_t st, const std::function<int()> &enqueue, hipGraphExec_t *out) {
     *out = nullptr;
     hipGraph_t graph = nullptr;
@@ -432,7 +437,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     const shg::phys::Consts k(*p);
     const float T = (float)p->temp;
     const auto tw = shg::phys::twiddle_table(c->n);
-    if (hipMemcpy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    if (shg_internal_copy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     const float h = p->scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
     c->nl.lossP = (float)(0.5 * (double)k.alpha_p);
@@ -446,7 +451,8 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         c->coef = shg::phys::step_coefficients(k, T, c->steps);
         if (hipMalloc(&c->d_coef, c->coef.size() * sizeof(float)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-        if (hipMemcpy(c->d_coef, c->coef.data(), c->coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        if (shg_internal_copy(c->d_coef, c->coef.data(), c->coef.size() * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream) != hipSuccess)
             return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     } else {
         const size_t n3 = ncell * c->hz;
@@ -479,7 +485,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
     hd.coef = c->d_coef;
     hd.nl = c->nl;
     hd.steps = c->steps;
-    if (hipMemcpy(c->d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice) != hipSuccess)
+    if (shg_internal_copy(c->d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
     if (int rc = upload_propagators(c, T)) return bad(rc);
     *out = c;
@@ -724,8 +730,8 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
         s.sweeps_total += counter;
         // fields and propagators at the temperature of the grid centre, slice 0 (Solver.h:329-331)
         float tcen = 0.f;
-        HIP_TRY(hipMemcpy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
-                          hipMemcpyDeviceToHost));
+        HIP_TRY(shg_internal_copy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
+                                  hipMemcpyDeviceToHost, c->stream));
         if (pump_from_temperature)
             shg::phys::make_pump(k, (float)c->p.power, (float)c->p.focus, (float)c->p.waist, tcen, fh0.data());
         RC_TRY(set_u_from_host(c, c->U, fh0.data(), sh0.data()));
@@ -759,28 +765,29 @@ int shg_jacobi(shg_ctx *c, int64_t nsweeps, double *mean_sq_update) {
 
 int shg_get_temperature(shg_ctx *c, float *T) {
     if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_temperature: bad argument");
-    HIP_TRY(hipMemcpy(T, c->T[c->tcur], (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(shg_internal_copy(T, c->T[c->tcur], (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
     return SHG_OK;
 }
 
 int shg_set_temperature(shg_ctx *c, const float *T) {
     if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_temperature: bad argument");
     const size_t bytes = (size_t)c->n * c->n * c->hz * sizeof(float);
-    HIP_TRY(hipMemcpy(c->T[0], T, bytes, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->T[1], T, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(shg_internal_copy(c->T[0], T, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(shg_internal_copy(c->T[1], T, bytes, hipMemcpyHostToDevice, c->stream));
     c->tcur = 0;
     return upload_propagators(c, T[(size_t)(c->n / 2) * c->n + c->n / 2]);
 }
 
 int shg_get_heat_source(shg_ctx *c, float *Q) {
     if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_heat_source: bad argument");
-    HIP_TRY(hipMemcpy(Q, c->Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(shg_internal_copy(Q, c->Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     return SHG_OK;
 }
 
 int shg_set_heat_source(shg_ctx *c, const float *Q) {
     if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_heat_source: bad argument");
-    HIP_TRY(hipMemcpy(c->Q, Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(shg_internal_copy(c->Q, Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyHostToDevice, c->stream));
     return SHG_OK;
 }
 

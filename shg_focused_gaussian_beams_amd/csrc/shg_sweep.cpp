@@ -3,7 +3,7 @@
 // point (THERMAL), with the same per-point folder and file names.
 //
 //   SHG_sweep --power=LIST --waist=LIST --t1=LIST [--t2=LIST] [--save_temperature=0|1]
-//             [--folder=Simulations] [--key=value ...]
+//             [--folder=Simulations] [--workers=N] [--key=value ...]
 //
 // LIST is `v1,v2,...` or `start:stop:step` (inclusive, as the awk loops of SHG.sh).  The
 // points are the product power x waist x t1 (x t2; by default t2 = t1, as SHG.sh sets
@@ -12,8 +12,14 @@
 // THERMAL temperature Tfinal_<z>.dat when asked, and <same name>.txt with the point's
 // powers and efficiency (src/SHG.sh:65-80).  Every SHG override (--nx, --nz, --thermal,
 // --seed, --noise, ...) applies to all points; the SH noise of point k uses seed + k.
+// THERMAL points run as one context each, N at a time from N host threads (default 8):
+// every context has its own stream, and one point's latency-bound passes leave most of
+// the GPU free for the others.
+#include <atomic>
 #include <filesystem>
 #include <sstream>
+#include <thread>
+#include <utility>
 
 #include "cli_common.h"
 
@@ -48,7 +54,7 @@ std::string num(double v) {
 
 void usage() {
     std::cerr << "usage: SHG_sweep --power=LIST --waist=LIST --t1=LIST [--t2=LIST] [--save_temperature=0|1]"
-                 " [--folder=DIR] [--key=value ...]   (LIST: v1,v2,... or start:stop:step)\n";
+                 " [--folder=DIR] [--workers=N] [--key=value ...]   (LIST: v1,v2,... or start:stop:step)\n";
 }
 
 struct Point {
@@ -69,6 +75,7 @@ int main(int argc, char *argv[]) {
     double noise = 1.0e-20;
     std::string unused_outdir;
     bool focus_set = false;
+    int workers = 8;
     for (int a = 1; a < argc; a++) {
         const std::string s = argv[a];
         const size_t eq = s.find('=');
@@ -82,6 +89,7 @@ int main(int argc, char *argv[]) {
         else if (key == "save_temperature") save_temperature = std::atoi(val.c_str()) != 0;
         else if (key == "save_only_last") ok = std::atoi(val.c_str()) != 0;   // sweeps save the last slice
         else if (key == "folder") { folder = val; if (!folder.empty() && folder.back() != '/') folder += "/"; }
+        else if (key == "workers") ok = (workers = std::atoi(val.c_str())) >= 1 && workers <= 64;
         else {
             if (key == "focus") focus_set = true;
             ok = parse_override(base, key, val, seed, noise, unused_outdir);
@@ -135,26 +143,67 @@ int main(int argc, char *argv[]) {
         }
         shg_batch_destroy(b);
     } else {
-        for (int k = 0; k < count; k++) {
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        std::vector<std::string> errors(count);
+        // context of point k with its z = 0 fields (SH noise of seed + k)
+        auto prepare = [&](int k, shg_ctx **out) -> bool {
             std::vector<float> fh(2 * n2), sh(2 * n2);
-            if ((rc = shg_make_pump(&ps[k], ps[k].temp, fh.data()))) return die("shg_make_pump", rc, "SHG_sweep");
-            gaussian_noise(sh, noise, seed + (uint64_t)k);
             shg_ctx *ctx = nullptr;
-            if ((rc = shg_create(&ps[k], &ctx))) return die("shg_create", rc, "SHG_sweep");
-            shg_thermal_stats st{};
-            if ((rc = shg_set_fields(ctx, fh.data(), sh.data())) || (rc = shg_thermal_run(ctx, 1, &st)) ||
-                (rc = shg_get_fields(ctx, fh.data(), sh.data())) || (rc = shg_powers(ctx, &pin[k], &pout[k], &eta[k]))) {
-                const int r = die("THERMAL point", rc, "SHG_sweep");
+            if (shg_make_pump(&ps[k], ps[k].temp, fh.data())) {
+                errors[k] = std::string("shg_make_pump: ") + shg_last_error();
+                return false;
+            }
+            gaussian_noise(sh, noise, seed + (uint64_t)k);
+            if (shg_create(&ps[k], &ctx) || shg_set_fields(ctx, fh.data(), sh.data())) {
+                errors[k] = std::string("shg_create: ") + shg_last_error();
                 shg_destroy(ctx);
-                return r;
+                return false;
             }
-            if (save_temperature) {
+            *out = ctx;
+            return true;
+        };
+        // One tuning for the whole sweep, on point 0 before the workers start: every point
+        // has the same grid and so the same fastest pass layouts.
+        shg_ctx *first = nullptr;
+        int rv = -1, cv = -1;
+        const char *nt = std::getenv("SHG_NO_TUNE");
+        if (!(nt && nt[0] == '1')) {
+            if (!prepare(0, &first)) return die("shg_create", SHG_E_DEVICE, "SHG_sweep");
+            if ((rc = shg_tune(first, 20, &rv, &cv))) {
+                shg_destroy(first);
+                return die("shg_tune", rc, "SHG_sweep");
+            }
+        }
+        auto point = [&](int k) -> bool {
+            shg_ctx *ctx = k == 0 ? std::exchange(first, nullptr) : nullptr;
+            if (!ctx && !prepare(k, &ctx)) return false;
+            std::vector<float> fh(2 * n2), sh(2 * n2);
+            shg_thermal_stats st{};
+            bool ok = !((rv >= 0 && shg_set_variants(ctx, rv, cv)) || shg_thermal_run(ctx, 1, &st) ||
+                        shg_get_fields(ctx, fh.data(), sh.data()) || shg_powers(ctx, &pin[k], &pout[k], &eta[k]));
+            if (ok && save_temperature) {
                 temps[k].resize(n2 * (size_t)(ps[k].heat_nz > 0 ? ps[k].heat_nz : ps[k].nz));
-                if ((rc = shg_get_temperature(ctx, temps[k].data()))) return die("shg_get_temperature", rc, "SHG_sweep");
+                ok = shg_get_temperature(ctx, temps[k].data()) == SHG_OK;
             }
+            if (!ok) errors[k] = std::string("THERMAL point: ") + shg_last_error();
             outs_fh[k] = fh;
             outs_sh[k] = sh;
             shg_destroy(ctx);
+            return ok;
+        };
+        auto work = [&] {
+            for (int k = next++; k < count && !failed; k = next++)
+                if (!point(k)) failed = true;
+        };
+        std::vector<std::thread> pool;
+        for (int w = 0; w < std::min(workers, count); w++) pool.emplace_back(work);
+        for (auto &t : pool) t.join();
+        shg_destroy(first);                            // set only if point 0 never ran
+        if (failed) {
+            for (int k = 0; k < count; k++)
+                if (!errors[k].empty()) std::cerr << "SHG_sweep: " << pts[k].name << ": " << errors[k] << "\n";
+            return 1;
         }
     }
     const double elapsed = seconds() - t_start;
