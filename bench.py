@@ -179,6 +179,8 @@ def bench_plain(shg, args, p):
     fh0, sh0 = make_inputs(p)
     ctx = shg.Context(p)
     ctx.set_fields(fh0, sh0)
+    if args.plain_variants:
+        ctx.set_variants(*(int(v) for v in args.plain_variants.split(",")))
     for _ in range(args.warmup):
         ctx.forward()
     torch.cuda.synchronize()
@@ -326,6 +328,7 @@ def main():
     ap.add_argument("--variants", default="",
                     help="ROW,COL: pin the THERMAL march's layouts instead of tuning (profiler runs, "
                          "whose serialised launches mislead the tuner)")
+    ap.add_argument("--plain-variants", default="", help="ROW,COL: the same for the plain 256x256 march")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
     ap.add_argument("--inverse-instances", type=int, default=1)
