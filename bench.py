@@ -276,45 +276,59 @@ def bench_inverse(shg, args):
     """Config 5 sample: THERMAL 512x512, nz = 1000, heat 512x512x200, P = 50 W; the
     temperature field from a bounded THERMAL outer loop, then adjoint ascent of a seeded
     +-0.1 rad pump phase mask with the field held fixed, judged by a plain forward.
-    `inverse_instances` x `inverse_iterations` are timed and projected to 32 x 100."""
+    `inverse_instances` instances ascend at once (one context and host thread each) for
+    `inverse_iterations` iterations; instance 0 is also timed alone.  Both are projected
+    to 32 instances x 100 iterations."""
     import numpy as np
     import torch
-    from shg_focused_gaussian_beams_amd.inverse import ascend_phase_mask
-    out = []
-    for i in range(args.inverse_instances):
+    from shg_focused_gaussian_beams_amd.inverse import ascend_phase_mask, ascend_phase_masks
+    n = max(1, args.inverse_instances)
+    ctxs, pumps, masks, th = [], [], [], []
+    for i in range(n):
         p = shg.config5_params(max_outer=1, jacobi_check=500, jacobi_max=1000, jacobi_tol=1e-6)
         rng = np.random.default_rng(5 + i)
-        mask0 = rng.uniform(-0.1, 0.1, (p.ny, p.nx))
+        masks.append(rng.uniform(-0.1, 0.1, (p.ny, p.nx)))
         ctx = shg.Context(p)
         pump = shg.make_pump(p)
         ctx.set_fields(pump, np.zeros_like(pump))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = ctx.thermal_run(pump_from_temperature=True)
-        thermal_s = time.perf_counter() - t0
+        th.append(dict(thermal_s=time.perf_counter() - t0, jacobi_sweeps=st["sweeps_total"]))
         T = ctx.get_temperature()
-        pump = shg.make_pump(p, float(T[0, p.ny // 2, p.nx // 2]))
-        ascend_phase_mask(ctx, pump, mask0, 1)                      # warm-up: graphs, tuning, buffers
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        mask, etas = ascend_phase_mask(ctx, pump, mask0, args.inverse_iterations)
-        torch.cuda.synchronize()
-        it_s = (time.perf_counter() - t0) / args.inverse_iterations
-        ctx.close()
+        pumps.append(shg.make_pump(p, float(T[0, p.ny // 2, p.nx // 2])))
+        ctxs.append(ctx)
+    it = args.inverse_iterations
+    ascend_phase_mask(ctxs[0], pumps[0], masks[0], 1)                # warm-up: graphs, tuning, buffers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ascend_phase_mask(ctxs[0], pumps[0], masks[0], it)
+    torch.cuda.synchronize()
+    alone_s = (time.perf_counter() - t0) / it
+    ascend_phase_masks(ctxs, pumps, masks, 1)                          # warm-up of the others
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = ascend_phase_masks(ctxs, pumps, masks, it)
+    torch.cuda.synchronize()
+    conc_s = (time.perf_counter() - t0) / it / n
+    out = []
+    for i, (mask, etas) in enumerate(res):
+        ctxs[i].close()
+        p = ctxs[i].params
         judge = shg.Context(p.copy(thermal=0))
-        fld = (pump * np.exp(1j * mask)).astype(np.complex64)
+        fld = (pumps[i] * np.exp(1j * mask)).astype(np.complex64)
         judge.set_fields(fld, np.zeros_like(fld))
         judge.forward()
-        eta_plain = judge.powers()[2]
+        out.append(dict(**th[i], eta_first=etas[0], eta_last=etas[-1], eta_plain_judge=judge.powers()[2]))
         judge.close()
-        out.append(dict(thermal_s=thermal_s, jacobi_sweeps=st["sweeps_total"], s_per_iteration=it_s,
-                        eta_first=etas[0], eta_last=etas[-1], eta_plain_judge=eta_plain))
-    it_s = float(np.mean([o["s_per_iteration"] for o in out]))
     th_s = float(np.mean([o["thermal_s"] for o in out]))
     cells = 512 * 512 * 999
-    return dict(instances_timed=len(out), iterations_timed=args.inverse_iterations, per_instance=out,
-                s_per_iteration=it_s, fwd_adj_cell_steps_per_s=cells / it_s,
-                projected_s_32x100=32 * (th_s + 100 * it_s))
+    return dict(instances_concurrent=n, iterations_timed=it, per_instance=out,
+                s_per_iteration_alone=alone_s, s_per_instance_iteration_concurrent=conc_s,
+                fwd_adj_cell_steps_per_s_alone=cells / alone_s,
+                fwd_adj_cell_steps_per_s_concurrent=cells / conc_s,
+                projected_s_32x100_alone=32 * (th_s + 100 * alone_s),
+                projected_s_32x100_concurrent=32 * (th_s + 100 * conc_s))
 
 
 def main():
@@ -331,7 +345,7 @@ def main():
     ap.add_argument("--plain-variants", default="", help="ROW,COL: the same for the plain 256x256 march")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
-    ap.add_argument("--inverse-instances", type=int, default=1)
+    ap.add_argument("--inverse-instances", type=int, default=4)
     ap.add_argument("--inverse-iterations", type=int, default=2)
     ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse"], default="all",
                     help="run one extra workload alone and print its figures (for profiler runs)")

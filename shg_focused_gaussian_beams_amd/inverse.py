@@ -36,3 +36,18 @@ def ascend_phase_mask(ctx, pump0, mask0, iterations, step=0.02, signal=None, che
     ctx.forward()
     etas.append(ctx.powers()[2])
     return phi, etas
+
+
+def ascend_phase_masks(ctxs, pumps, masks, iterations, step=0.02, workers=None, checkpoint_interval=0):
+    """The batched form: instance k ascends its own mask on its own context, all of them
+    at once from a pool of host threads (the library calls release the GIL, and each
+    context owns its stream, so the instances overlap on the GPU and with each other's
+    host work).  Returns [(mask, etas)] in instance order, each as ascend_phase_mask."""
+    from concurrent.futures import ThreadPoolExecutor
+    if not (len(ctxs) == len(pumps) == len(masks)):
+        raise ValueError("ctxs, pumps and masks must have one entry per instance")
+    workers = len(ctxs) if workers is None else max(1, int(workers))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        futs = [pool.submit(ascend_phase_mask, c, p, m, iterations, step, None, checkpoint_interval)
+                for c, p, m in zip(ctxs, pumps, masks)]
+        return [f.result() for f in futs]

@@ -148,3 +148,27 @@ def test_phase_mask_ascent_increases_efficiency(gpu):
         mask, hist = ascend_phase_mask(ctx, pump, mask0, 4, step=0.05)
     print("  eta history", ["%.6e" % x for x in hist])
     assert hist[-1] > hist[0]
+
+
+@pytest.mark.gpu
+def test_concurrent_phase_mask_ascents_match_one_at_a_time(gpu):
+    # the batched ascent (one context and host thread per instance, all at once) follows
+    # the same iterates as each instance ascending alone
+    from shg_focused_gaussian_beams_amd.inverse import ascend_phase_mask, ascend_phase_masks
+    p = _params(nz=20)
+    T = _smooth_T(p)
+    pump = gpu.make_pump(p, float(T[0, p.ny // 2, p.nx // 2]))
+    masks = [np.random.default_rng(10 + k).uniform(-0.1, 0.1, pump.shape) for k in range(3)]
+    ctxs = [gpu.Context(p) for _ in range(3)]
+    try:
+        for c in ctxs:
+            c.set_temperature(T)
+        together = ascend_phase_masks(ctxs, [pump] * 3, masks, 3, step=0.05)
+        for k, c in enumerate(ctxs):
+            mask, etas = ascend_phase_mask(c, pump, masks[k], 3, step=0.05)
+            print(f"\n  instance {k}: eta {etas[0]:.6e} -> {etas[-1]:.6e}", flush=True)
+            assert np.allclose(together[k][0], mask, rtol=0, atol=1e-5), k
+            assert np.allclose(together[k][1], etas, rtol=1e-5, atol=0), k
+    finally:
+        for c in ctxs:
+            c.close()
