@@ -1,6 +1,6 @@
 """Throughput of the split-step (SSFM) z-march on one MI355X.
 
-Headline workload (BASELINE config 2): THERMAL SSFM, 1064 -> 532 nm in MgO:sPPLT,
+Headline workload: THERMAL SSFM, 1064 -> 532 nm in MgO:sPPLT,
 P = 30 W, w0 = 40 um, 256x256 grid, nz = 400, heat grid 256x256x100, Jacobi to a mean
 squared update < 1e-6, dk(T) fed back.  A "step" of this benchmark is one THERMAL
 z-march of the field pair through the crystal at the converged temperature (nz steps:
@@ -12,9 +12,10 @@ the last one feeds the heat source), inputs resident in HBM.
            the timed marches (HIP events on the library stream); 8 TB/s peak, and the
            copy bandwidth this GPU reaches (a device copy timed in the same run).
 Also reported (same run): the THERMAL outer loop, Jacobi voxel-sweeps/s, the batched
-sweep of config 3 (256 uneven instances in one hipGraph) against one-at-a-time solves,
-forward+adjoint of config 4 against one forward, the plain config-2-shape march, and a
-timed sample of the config-5 inverse design (THERMAL 512x512, nz 1000, adjoint ascent
+sweep of 256 uneven instances (64^2..512^2, nz 100..400, in one hipGraph) against
+one-at-a-time solves, forward+adjoint at 256x256, nz 400 against one forward, the plain
+256x256, nz 400 march, and a timed sample of the inverse design (THERMAL 512x512,
+nz 1000, adjoint ascent
 of a pump phase mask), projected to its 32 instances x 100 iterations.
 cpu_baseline: the FP32 CPU restatement of the reference's THERMAL march (oracle/, -O3,
 per-cell n() as the reference evaluates it) on one host core over a bounded number of
@@ -134,7 +135,7 @@ def copy_peak_gbps(mib=1024, reps=5):
 
 def bench_thermal(shg, args):
     import numpy as np
-    p = shg.config2_params(heat_nz=100, jacobi_tol=1e-6)
+    p = shg.thermal256_params(heat_nz=100, jacobi_tol=1e-6)
     fh0, sh0 = make_inputs(p)
     ctx = shg.Context(p)
     ctx.set_fields(fh0, sh0)
@@ -199,7 +200,7 @@ def bench_plain(shg, args, p):
 def bench_batch(shg, args):
     import numpy as np
     import torch
-    ps = shg.config3_params(count=args.batch, seed=3)
+    ps = shg.uneven_sweep_params(count=args.batch, seed=3)
     b = shg.Batch(ps)
     b.init_fields(noise_rel=1e-6, seed=3)
     log("batch: first run (graph capture) ...")
@@ -243,7 +244,7 @@ def bench_batch(shg, args):
 
 def bench_adjoint(shg, args):
     import torch
-    p = shg.config1_params(nx=256, ny=256, nz=400, lx=8 * 35.0, ly=8 * 35.0, power=10.0, waist=35.0)
+    p = shg.plain128_params(nx=256, ny=256, nz=400, lx=8 * 35.0, ly=8 * 35.0, power=10.0, waist=35.0)
     fh0, sh0 = make_inputs(p)
     ctx = shg.Context(p)
     ctx.set_fields(fh0, sh0)
@@ -273,7 +274,7 @@ def bench_adjoint(shg, args):
 
 
 def bench_inverse(shg, args):
-    """Config 5 sample: THERMAL 512x512, nz = 1000, heat 512x512x200, P = 50 W; the
+    """Inverse-design sample: THERMAL 512x512, nz = 1000, heat 512x512x200, P = 50 W; the
     temperature field from a bounded THERMAL outer loop, then adjoint ascent of a seeded
     +-0.1 rad pump phase mask with the field held fixed, judged by a plain forward.
     `inverse_instances` instances ascend at once (one context and host thread each) for
@@ -285,7 +286,7 @@ def bench_inverse(shg, args):
     n = max(1, args.inverse_instances)
     ctxs, pumps, masks, th = [], [], [], []
     for i in range(n):
-        p = shg.config5_params(max_outer=1, jacobi_check=500, jacobi_max=1000, jacobi_tol=1e-6)
+        p = shg.inverse512_params(max_outer=1, jacobi_check=500, jacobi_max=1000, jacobi_tol=1e-6)
         rng = np.random.default_rng(5 + i)
         masks.append(rng.uniform(-0.1, 0.1, (p.ny, p.nx)))
         ctx = shg.Context(p)
@@ -373,7 +374,7 @@ def main():
 
     if args.section != "all":
         if args.section == "plain":
-            r = bench_plain(shg, args, shg.config2_params(thermal=0))
+            r = bench_plain(shg, args, shg.thermal256_params(thermal=0))
         elif args.section == "batch":
             r = bench_batch(shg, args)
         elif args.section == "inverse":
@@ -397,7 +398,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian pump + 1e-6 relative complex noise, SH = 0)",
-        "config": {"workload": "config2 THERMAL SSFM march: MgO:sPPLT 30 mm, P=30 W, w0=40 um, 256x256, nz=400, "
+        "config": {"workload": "THERMAL SSFM march: MgO:sPPLT 30 mm, P=30 W, w0=40 um, 256x256, nz=400, "
                                "heat grid 256x256x100, Jacobi to mean squared update < 1e-6",
                    "nx": p.nx, "ny": p.ny, "nz": p.nz, "heat_nz": p.heat_nz,
                    "cell_steps_per_march": th["cells"], "eta": th["eta"],
@@ -416,32 +417,32 @@ def main():
                      "measured_copy_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps},
     }
     if not args.no_extra:
-        log("plain config-2 shape ...")
-        pl = bench_plain(shg, args, shg.config2_params(thermal=0))
+        log("plain 256x256, nz 400 ...")
+        pl = bench_plain(shg, args, shg.thermal256_params(thermal=0))
         out["config"]["plain_256x256_nz400"] = {
             "cell_steps_per_s": pl["cells"] / pl["s_per_solve"], "ms_per_solve": pl["s_per_solve"] * 1e3,
             "row_pass_us": pl["row_ms"] * 1e3, "col_pass_us": pl["col_ms"] * 1e3, "variants": pl["variants"],
             "step_GBps_algorithmic": B_PLAIN * 65536 / ((pl["row_ms"] + pl["col_ms"]) * 1e-3) / 1e9}
-        log("batched config 3 ...")
+        log("batched sweep of 256 uneven instances ...")
         bt = bench_batch(shg, args)
-        out["config"]["batched_config3"] = {
+        out["config"]["batched_sweep_256_uneven"] = {
             "instances": bt["count"], "s_per_batch": bt["s"], "cell_steps_per_s": bt["cells"] / bt["s"],
             "instances_per_s": bt["count"] / bt["s"], "algorithmic_GBps": bt["cells"] * B_PLAIN / bt["s"] / 1e9,
             "hbm_frac": bt["cells"] * B_PLAIN / bt["s"] / 1e9 / HBM_PEAK_GBPS,
             "valu_frac_est": bt["cells"] * VALU_OPS_PER_CELL_STEP / bt["s"] / VALU_PEAK_LANE_OPS,
             "one_at_a_time_s": bt["single_s"], "speedup_vs_one_at_a_time": bt["single_s"] / bt["s"],
             "one_at_a_time_sampled_instances": bt["sampled"], "mean_eta": bt["eta_mean"]}
-        log("forward+adjoint config 4 ...")
+        log("forward+adjoint 256x256, nz 400 ...")
         ad = bench_adjoint(shg, args)
-        out["config"]["fwd_adjoint_config4"] = {
+        out["config"]["fwd_adjoint_256x256_nz400"] = {
             "forward_ms": ad["fwd_s"] * 1e3, "fwd_adj_ms": ad["adj_s"] * 1e3,
             "fwd_adj_over_fwd": ad["adj_s"] / ad["fwd_s"],
             "fwd_adj_full_tape_ms": ad["adj_full_s"] * 1e3, "fwd_adj_full_tape_over_fwd": ad["adj_full_s"] / ad["fwd_s"],
             "cell_steps_per_s": ad["cells"] / ad["adj_s"],
             "checkpoint_interval": ad["res"]["checkpoint_interval"], "eta": ad["res"]["eta"],
             "d_eta_d_temp": ad["res"]["d_eta_d_temp"], "d_eta_d_focus": ad["res"]["d_eta_d_focus"]}
-        log("inverse design config 5 (sample) ...")
-        out["config"]["inverse_design_config5_sample"] = bench_inverse(shg, args)
+        log("inverse design 512x512, nz 1000 (sample) ...")
+        out["config"]["inverse_design_512x512_nz1000_sample"] = bench_inverse(shg, args)
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline_thermal(p, th["T"], th["fh0"], th["sh0"])

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .params import (ShgParams, default_params, config1_params, config2_params, config3_params, config5_params,  # noqa: F401
+from .params import (ShgParams, default_params, plain128_params, thermal256_params, uneven_sweep_params, inverse512_params,  # noqa: F401
                      SHG_CRYSTAL_SPPLT, SHG_CRYSTAL_PPLN, SHG_SCHEME_REFERENCE, SHG_SCHEME_STRANG)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -69,7 +69,7 @@ def default_params(**overrides):
     return p
 
 
-def config1_params(**overrides):
+def plain128_params(**overrides):
     """Plain SSFM workload, 1064->532 nm CW SHG in MgO:sPPLT: L = 30 mm, 128x128 grid over
     an 8*w0 window, nz = 200, P = 5 W, w0 = 31 um focused at the crystal centre,
     deff = 10 pm/V, T = 27 C with the grating period that phase-matches at 27 C."""
@@ -82,10 +82,10 @@ def config1_params(**overrides):
     return p
 
 
-def config3_params(count=256, seed=3, sizes=(64, 128, 256, 512), nz_range=(100, 400), **overrides):
+def uneven_sweep_params(count=256, seed=3, sizes=(64, 128, 256, 512), nz_range=(100, 400), **overrides):
     """Batched sweep: per instance, grid size uniform over `sizes`, w0 ~ U(20, 80) um,
     P ~ U(1, 50) W, T ~ 27 C (QPM) + U(-2, 2), focus ~ L/2 + U(-L/4, L/4),
-    nz ~ U{nz_range}, window 8*w0; otherwise as config1 (seeded)."""
+    nz ~ U{nz_range}, window 8*w0; otherwise as plain128_params (seeded)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     out = []
@@ -93,7 +93,7 @@ def config3_params(count=256, seed=3, sizes=(64, 128, 256, 512), nz_range=(100, 
     for _ in range(count):
         n = int(rng.choice(sizes))
         w0 = float(rng.uniform(20.0, 80.0))
-        p = config1_params(nx=n, ny=n, nz=int(rng.integers(nz_range[0], nz_range[1] + 1)),
+        p = plain128_params(nx=n, ny=n, nz=int(rng.integers(nz_range[0], nz_range[1] + 1)),
                            lx=8 * w0, ly=8 * w0, waist=w0, power=float(rng.uniform(1.0, 50.0)),
                            temp=27.0 + float(rng.uniform(-2.0, 2.0)),
                            focus=L / 2 + float(rng.uniform(-L / 4, L / 4)))
@@ -104,8 +104,8 @@ def config3_params(count=256, seed=3, sizes=(64, 128, 256, 512), nz_range=(100, 
     return out
 
 
-def config2_params(**overrides):
-    """THERMAL workload: as config1 but P = 30 W, w0 = 40 um, 256x256 grid, nz = 400,
+def thermal256_params(**overrides):
+    """THERMAL workload: as plain128_params but P = 30 W, w0 = 40 um, 256x256 grid, nz = 400,
     alpha_p = 0.1/m, alpha_s = 4/m, beta_s = 1e-13 m/W, 25 C ambient."""
     w0 = 40.0
     p = default_params(nx=256, ny=256, nz=400, lx=8 * w0, ly=8 * w0, deff=10e-6,
@@ -117,10 +117,10 @@ def config2_params(**overrides):
     return p
 
 
-def config5_params(**overrides):
-    """Large inverse-design instance: as config2 (THERMAL, w0 = 40 um, 8*w0 window) but
+def inverse512_params(**overrides):
+    """Large inverse-design instance: as thermal256_params (THERMAL, w0 = 40 um, 8*w0 window) but
     512x512 grid, nz = 1000, heat grid 512x512x200, P = 50 W."""
-    p = config2_params(nx=512, ny=512, nz=1000, heat_nz=200, power=50.0)
+    p = thermal256_params(nx=512, ny=512, nz=1000, heat_nz=200, power=50.0)
     for k, v in overrides.items():
         setattr(p, k, v)
     return p

@@ -12,7 +12,7 @@ import pytest
 
 import oracle_lib as orc
 import shg_focused_gaussian_beams_amd as shg
-from shg_focused_gaussian_beams_amd.params import default_params, config1_params
+from shg_focused_gaussian_beams_amd.params import default_params, plain128_params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -56,7 +56,7 @@ def test_create_rejects_bad_parameters_without_device(bad):
     assert lib.shg_last_error().decode().startswith("shg_create")
 
 
-@pytest.mark.parametrize("mk", [default_params, config1_params])
+@pytest.mark.parametrize("mk", [default_params, plain128_params])
 def test_host_pump_is_bit_exact_with_oracle32(mk):
     p = mk()
     a = shg.make_pump(p)

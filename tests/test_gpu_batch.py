@@ -5,11 +5,11 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
-from shg_focused_gaussian_beams_amd.params import config3_params
+from shg_focused_gaussian_beams_amd.params import uneven_sweep_params
 
 
 def _mixed(count, seed, nz_range=(8, 30)):
-    return config3_params(count=count, seed=seed, nz_range=nz_range)
+    return uneven_sweep_params(count=count, seed=seed, nz_range=nz_range)
 
 
 @pytest.mark.gpu

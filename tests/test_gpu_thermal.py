@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
-from shg_focused_gaussian_beams_amd.params import (config2_params, SHG_BC_TOP_OPEN, SHG_BC_OVEN_SURROUNDED,
+from shg_focused_gaussian_beams_amd.params import (thermal256_params, SHG_BC_TOP_OPEN, SHG_BC_OVEN_SURROUNDED,
                                                    SHG_CRYSTAL_SPPLT, SHG_CRYSTAL_PPLN)
 
 TOL = 1e-4
@@ -20,7 +20,7 @@ TOL = 1e-4
 def _thermal_params(**kw):
     base = dict(nx=64, ny=64, nz=16, lx=320.0, ly=320.0)
     base.update(kw)
-    return config2_params(**base)
+    return thermal256_params(**base)
 
 
 def _rand_T(p, seed, lo=26.0, hi=29.0):

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
-from shg_focused_gaussian_beams_amd.params import (default_params, config1_params, SHG_CRYSTAL_PPLN,
+from shg_focused_gaussian_beams_amd.params import (default_params, plain128_params, SHG_CRYSTAL_PPLN,
                                                    SHG_SCHEME_STRANG, SHG_BC_OVEN_SURROUNDED)
 
 
@@ -82,7 +82,7 @@ def test_ppln_index_against_literature():
 # ---- pump and power ---------------------------------------------------------------
 
 def test_pump_power_closed_form():
-    p = config1_params(nx=256, ny=256, lx=400.0, ly=400.0)
+    p = plain128_params(nx=256, ny=256, lx=400.0, ly=400.0)
     a = orc.pump(p, 64)
     # a Gaussian of power P: sum |A|^2 dx dy = 2P / (eps0 c n)
     pw = orc.power(p, 0, a)
@@ -94,7 +94,7 @@ def test_pump_power_closed_form():
 
 def test_pump_focus_phase_is_a_converging_wave():
     # focus > 0: the z = 0 plane has a converging spherical phase (phase decreases with r^2)
-    p = config1_params()
+    p = plain128_params()
     a = orc.pump(p, 64)
     c = p.nx // 2
     ph = np.unwrap(np.angle(a[c, c:c + 10]))
@@ -122,7 +122,7 @@ def _linear(p):
 
 
 def test_propagator_shape():
-    p = config1_params(nx=64, ny=64)
+    p = plain128_params(nx=64, ny=64)
     hp, hs = orc.propagators(p, 64)
     assert np.allclose(np.abs(hp), 1, atol=1e-14) and np.allclose(np.abs(hs), 1, atol=1e-14)
     assert hp[0, 0] == 1 and hs[0, 0] == 1                      # DC at index 0 after fftshift
@@ -132,7 +132,7 @@ def test_propagator_shape():
 
 def test_diffraction_chain_is_spectral_multiply():
     # deff = 0: m steps == multiply the FFTW_BACKWARD spectrum by H^m (numpy, FP64)
-    p = _linear(config1_params(nx=64, ny=64, nz=12))
+    p = _linear(plain128_params(nx=64, ny=64, nz=12))
     a = orc.pump(p, 64)
     hp, _ = orc.propagators(p, 64)
     m = p.nz - 1
@@ -146,7 +146,7 @@ def test_diffraction_chain_is_spectral_multiply():
 def test_gaussian_beam_waist_law():
     # free propagation of the reference pump (focused at L/2) through the whole crystal:
     # at z = L the beam radius is w0 sqrt(1 + ((L - f)/zR)^2), zR = pi n w0^2 / lambda.
-    p = _linear(config1_params(nx=256, ny=256, nz=60, lx=600.0, ly=600.0, waist=31.0))
+    p = _linear(plain128_params(nx=256, ny=256, nz=60, lx=600.0, ly=600.0, waist=31.0))
     a = orc.pump(p, 64)
     out, _ = orc.cwes(p, a, np.zeros_like(a), prec=64)
     n = orc.n_index(0, 64, p.lambda_p, p.temp)
@@ -172,7 +172,7 @@ def test_walkoff_shifts_the_beam_by_dz_tan_rho():
     # FFTW_BACKWARD spectrum, then FFTW_FORWARD / N) that factor maps A(x) to A(x + d),
     # d = dz tan(rho): after m diffraction steps the FH centroid sits m d further toward
     # -x, whatever the focusing does to the width.
-    p = _linear(config1_params(nx=128, ny=128, nz=21, lx=600.0, ly=600.0))
+    p = _linear(plain128_params(nx=128, ny=128, nz=21, lx=600.0, ly=600.0))
     p.rho_p = 2e-3
     a = orc.pump(p, 64)
     out, _ = orc.cwes(p, a, np.zeros_like(a), prec=64)
@@ -205,7 +205,7 @@ def test_plane_wave_geometric_sum(scheme, factor):
     #   S_out = factor * i kappa_s P0^2 dz * sum_{s<M} e^{i dk s dz},  M = nz - 1,
     # factor 2 for the reference scheme (two RK4 over dz per step, src/headers/Solver.h:236-242),
     # 1 for the Strang scheme.
-    p = config1_params(nx=64, ny=64, nz=41, alpha_p=0.0, alpha_s=0.0, beta_s=0.0, scheme=scheme)
+    p = plain128_params(nx=64, ny=64, nz=41, alpha_p=0.0, alpha_s=0.0, beta_s=0.0, scheme=scheme)
     P0, S0 = _plane_wave(p)
     _, s64 = orc.cwes(p, P0, S0, prec=64)
     _, s32 = orc.cwes(p, P0, S0, prec=32)
@@ -232,7 +232,7 @@ def test_energy_defect_is_first_order_in_dz():
     # 203-224), which makes it a first-order scheme: the power defect halves with dz.
     defects = []
     for nz in (41, 81):
-        p = config1_params(nx=64, ny=64, nz=nz, power=50.0, waist=20.0, lx=160.0, ly=160.0,
+        p = plain128_params(nx=64, ny=64, nz=nz, power=50.0, waist=20.0, lx=160.0, ly=160.0,
                            alpha_p=0.0, alpha_s=0.0, beta_s=0.0)
         a = orc.pump(p, 64)
         fh, sh = orc.cwes(p, a, np.zeros_like(a), prec=64)
@@ -246,7 +246,7 @@ def test_energy_defect_is_first_order_in_dz():
 # ---- heat equation -----------------------------------------------------------------
 
 def _heat_params(**kw):
-    return config1_params(nx=64, ny=64, nz=8, **kw)
+    return plain128_params(nx=64, ny=64, nz=8, **kw)
 
 
 def test_jacobi_uniform_fixed_point():
@@ -381,7 +381,7 @@ def test_thermal_fp64_restatement_matches_fp32_where_dk_agrees():
     # At 26.913 C the float and double phase mismatch agree to 7e-7 rad over the crystal,
     # so the FP32 and FP64 THERMAL marches must agree to float rounding (2e-6 here; at
     # 27 C the float dk offset alone separates them by 5e-3 on the SH).
-    p = config1_params(nx=64, ny=64, nz=12, thermal=1, heat_nz=4, temp=26.913, power=40.0, waist=30.0,
+    p = plain128_params(nx=64, ny=64, nz=12, thermal=1, heat_nz=4, temp=26.913, power=40.0, waist=30.0,
                        lx=240.0, ly=240.0, alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7)
     T = np.full((4, 64, 64), 26.913, np.float32)
     fh0 = orc.pump(p, 64).astype(np.complex128)
