@@ -36,11 +36,14 @@ def _run_gpu(shg, p, fh0, sh0):
 
 CASES = {
     "ref64_nz20": lambda: default_params(nz=20),
-    "cfg1_n64_nz40": lambda: plain128_params(nx=64, ny=64, nz=40),
-    "cfg1_n128_nz200": lambda: plain128_params(),
-    "cfg1_n256_nz30": lambda: plain128_params(nx=256, ny=256, nz=30, lx=400.0, ly=400.0),
-    "cfg1_n512_nz8": lambda: plain128_params(nx=512, ny=512, nz=8, lx=500.0, ly=500.0),
-    "cfg1_strang_n128_nz50": lambda: plain128_params(nz=50, scheme=SHG_SCHEME_STRANG),
+    "plain_n64_nz40": lambda: plain128_params(nx=64, ny=64, nz=40),
+    "plain_n128_nz200": lambda: plain128_params(),
+    "plain_n256_nz30": lambda: plain128_params(nx=256, ny=256, nz=30, lx=400.0, ly=400.0),
+    "plain_n512_nz8": lambda: plain128_params(nx=512, ny=512, nz=8, lx=500.0, ly=500.0),
+    # the gradient workload's forward at full size: 256x256, nz 400, P = 10 W, w0 = 35 um
+    "plain_n256_nz400": lambda: plain128_params(nx=256, ny=256, nz=400, lx=280.0, ly=280.0, power=10.0,
+                                                waist=35.0),
+    "strang_n128_nz50": lambda: plain128_params(nz=50, scheme=SHG_SCHEME_STRANG),
     "hot_n128_nz60": lambda: plain128_params(nz=60, power=50.0, waist=20.0, lx=160.0, ly=160.0,
                                              alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7),
     # walk-off (zero in the reference's constants, supported by its propagator formula)

@@ -176,3 +176,32 @@ def test_thermal_forward_snapshots(gpu):
     assert np.array_equal(again[0], out_fh) and np.array_equal(Q1, Q2)
     rfh, rsh, _ = orc.thermal_cwes(p, T, fh0, sh0)
     assert orc.rel_l2(sfh[-1], rfh) <= TOL and orc.rel_l2(ssh[-1], rsh) <= TOL
+
+
+@pytest.mark.gpu
+def test_thermal_march_full_size_matches_oracle(gpu):
+    # the headline workload at its full size (256x256, nz 400, heat grid 256x256x100) on a
+    # temperature field with a hot spot on the beam axis that cools along z, so dk(T)
+    # varies over every heat slice; the oracle march takes ~20 s on one core
+    p = thermal256_params(heat_nz=100, jacobi_tol=1e-6)
+    z = np.linspace(0, 1, p.heat_nz)[:, None, None]
+    y = np.linspace(0, 1, p.ny)[None, :, None]
+    x = np.linspace(0, 1, p.nx)[None, None, :]
+    T = (27.0 + 0.5 * np.exp(-((x - .5) ** 2 + (y - .5) ** 2) / 0.02) * (1 - 0.5 * z)).astype(np.float32)
+    fh0 = gpu.make_pump(p, float(T[0, p.ny // 2, p.nx // 2]))
+    sh0 = np.zeros_like(fh0)
+    print("\nthermal march 256x256, nz 400, heat 256x256x100", flush=True)
+    with gpu.Context(p) as ctx:
+        ctx.set_temperature(T)
+        ctx.set_fields(fh0, sh0)
+        ctx.forward()
+        fh, sh = ctx.get_fields()
+        Q = ctx.get_heat_source()
+        _, _, eta = ctx.powers()
+    rfh, rsh, rQ = orc.thermal_cwes(p, T, fh0, sh0)
+    t_c = float(T[0, p.ny // 2, p.nx // 2])
+    r_eta = orc.power(p, 1, rsh, t_c) / orc.power(p, 0, fh0, t_c)
+    e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh), float(np.linalg.norm(Q - rQ) / np.linalg.norm(rQ)))
+    print(f"  relL2 FH {e[0]:.2e} SH {e[1]:.2e} Q {e[2]:.2e}; eta {eta:.8f} oracle {r_eta:.8f}")
+    assert max(e) <= TOL, e
+    assert abs(eta - r_eta) <= 1e-4 * r_eta, (eta, r_eta)
