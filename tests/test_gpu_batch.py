@@ -86,3 +86,34 @@ def test_batch_pump_matches_host_pump(gpu):
             ref = gpu.make_pump(p)
             assert np.abs(fh0 - ref).max() <= 2e-6 * np.abs(ref).max()
             assert not np.any(sh0)
+
+
+@pytest.mark.gpu
+def test_full_sweep_of_256_instances(gpu, monkeypatch):
+    # the sweep workload at its full size (256 instances, 64^2..512^2, nz 100..400): a
+    # sample of instances of every grid size is bit-identical to its own solve (fixed
+    # layouts), and the 64^2 instances of the sample meet the oracle at full nz
+    monkeypatch.setenv("SHG_NO_TUNE", "1")
+    ps = uneven_sweep_params(count=256, seed=3)
+    sample = []
+    for n in (64, 128, 256, 512):
+        sample += [i for i, p in enumerate(ps) if p.nx == n][:3]
+    with gpu.Batch(ps) as b:
+        b.init_fields(noise_rel=1e-6, seed=3)
+        b.run()
+        outs = {i: b.get_fields(i) for i in sample}
+        ins = {i: b.get_input(i) for i in sample}
+        _, _, eta = b.powers()
+    assert np.isfinite(eta).all() and (eta > 0).all()
+    for i in sample:
+        p = ps[i]
+        with gpu.Context(p) as ctx:
+            ctx.set_fields(*ins[i])
+            ctx.forward()
+            fh, sh = ctx.get_fields()
+        assert np.array_equal(fh, outs[i][0]) and np.array_equal(sh, outs[i][1]), (i, p.nx)
+        if p.nx == 64:
+            rfh, rsh = orc.cwes(p, ins[i][0], ins[i][1], prec=32)
+            e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
+            print(f"\n  instance {i}: 64^2 nz={p.nz} relL2 FH {e[0]:.2e} SH {e[1]:.2e}", flush=True)
+            assert max(e) <= 1e-4, (i, e)
