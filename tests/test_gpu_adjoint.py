@@ -5,6 +5,9 @@ central finite differences.
   relative to |A0| (as specified for the adjoint check).  FP32 rounding of eta puts a
   noise of ~1e-4 absolute on these FD slopes, so they are judged as a 64-vector (relative
   L2 and correlation); with a 1e-2 step the per-direction agreement is checked tightly.
+  At the gradient workload's full size (256x256, nz 400) the directions are smooth
+  modulations of the pump, along which eta moves at O(eta) and the FD is clean: there
+  the 64-vector must agree to 5e-3.
 - field and focus derivatives against central FD of the FP32 restatement (oracle32, the
   same arithmetic as the GPU forward, so the same operating point), step 1e-2 relative.
 - field and temperature derivatives against central FD of the FP64 restatement
@@ -59,10 +62,39 @@ def _directions(a0, count, seed):
         yield v * (np.linalg.norm(a0) / np.linalg.norm(v))
 
 
+def _smooth_directions(a0, count, seed):
+    # the pump times a random low-order modulation (2D Fourier modes |k| <= 2 with complex
+    # normal weights): directions along which eta changes at O(eta), so that the FP32
+    # rounding of eta (~1e-6 relative after 400 steps) stays far below a step-1e-3 FD
+    # difference; white-noise directions mostly miss the beam and their FD is noise
+    rng = np.random.default_rng(seed)
+    ny, nx = a0.shape
+    y, x = np.meshgrid(np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    for _ in range(count):
+        m = np.zeros(a0.shape, np.complex128)
+        for ky in range(-2, 3):
+            for kx in range(-2, 3):
+                c = rng.standard_normal() + 1j * rng.standard_normal()
+                m += c * np.exp(2j * np.pi * (kx * x + ky * y))
+        v = a0 * m
+        yield v * (np.linalg.norm(a0) / np.linalg.norm(v))
+
+
+FD_CASES = {
+    "n64_nz24": (lambda: _params(nx=64, ny=64, nz=24), _directions, 5e-2),
+    "n128_nz40": (lambda: _params(nx=128, ny=128, nz=40), _directions, 5e-2),
+    # the gradient workload at full size: 256x256, nz 400, P = 10 W, w0 = 35 um
+    "n256_nz400_p10_w35": (lambda: plain128_params(nx=256, ny=256, nz=400, lx=280.0, ly=280.0, power=10.0,
+                                                   waist=35.0), _smooth_directions, 5e-3),
+}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,nz", [(64, 24), (128, 40)])
-def test_field_gradient_vs_gpu_fd_64_directions(gpu, n, nz):
-    p = _params(nx=n, ny=n, nz=nz)
+@pytest.mark.parametrize("case", list(FD_CASES))
+def test_field_gradient_vs_gpu_fd_64_directions(gpu, case):
+    make, directions, tol = FD_CASES[case]
+    p = make()
+    n, nz = p.nx, p.nz
     pump, noise, sh = _inputs(gpu, p)
     a0 = pump + noise
     with gpu.Context(p) as ctx:
@@ -70,7 +102,7 @@ def test_field_gradient_vs_gpu_fd_64_directions(gpu, n, nz):
         res, g = ctx.adjoint()
         g = g.astype(np.complex128)
         fd3, fd2, ad = [], [], []
-        for k, v in enumerate(_directions(a0, 64, 4)):
+        for k, v in enumerate(directions(a0, 64, 4)):
             ad.append(float(np.real(np.sum(np.conj(g) * v))))
             for eps, out in ((1e-3, fd3), (1e-2, fd2)):
                 if eps == 1e-2 and k >= 8:
@@ -84,7 +116,7 @@ def test_field_gradient_vs_gpu_fd_64_directions(gpu, n, nz):
     rel2 = np.abs(fd2 - ad[:8]) / np.sqrt(np.mean(ad ** 2))
     print(f"\nn={n} nz={nz} eta={res['eta']:.6e}: 64 dirs step 1e-3 relL2 {rel3:.2e} corr {corr:.6f}; "
           f"8 dirs step 1e-2 max rel {rel2.max():.2e}", flush=True)
-    assert rel3 < 5e-2 and corr > 0.998
+    assert rel3 < tol and corr > 0.998
     assert rel2.max() < 5e-3
 
 
