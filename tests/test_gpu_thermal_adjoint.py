@@ -9,6 +9,8 @@ fixed (shg_adjoint on a THERMAL context) against central finite differences.
   compared with d_eta_d_temp_crystal, a shift of one heat slice with the sum of that
   slice of the per-voxel map.
 - the checkpoint interval changes nothing; the temperature and heat source are untouched.
+- full size (256x256, nz 400, heat 256x256x100): the field gradient against FD of the
+  GPU THERMAL forward along smooth modulations of the pump.
 """
 import numpy as np
 import pytest
@@ -172,3 +174,42 @@ def test_concurrent_phase_mask_ascents_match_one_at_a_time(gpu):
     finally:
         for c in ctxs:
             c.close()
+
+
+@pytest.mark.gpu
+def test_thermal_field_gradient_full_size_vs_gpu_fd(gpu):
+    # THERMAL at its full size (256x256, nz 400, heat grid 256x256x100) on a temperature
+    # field with an on-axis hot spot: the frozen-T adjoint against central FD of the GPU
+    # THERMAL forward, step 1e-3 relative, along 32 smooth modulations of the pump (the
+    # pump times random 2D Fourier modes |k| <= 2), along which eta moves at O(eta)
+    p = thermal256_params(heat_nz=100)
+    z = np.linspace(0, 1, p.heat_nz)[:, None, None]
+    y = np.linspace(0, 1, p.ny)[None, :, None]
+    x = np.linspace(0, 1, p.nx)[None, None, :]
+    T = (27.0 + 0.5 * np.exp(-((x - .5) ** 2 + (y - .5) ** 2) / 0.02) * (1 - 0.5 * z)).astype(np.float32)
+    fh, sh = _inputs(gpu, p, T)
+    rng = np.random.default_rng(8)
+    yy, xx = np.meshgrid(np.arange(p.ny) / p.ny, np.arange(p.nx) / p.nx, indexing="ij")
+    with gpu.Context(p) as ctx:
+        ctx.set_temperature(T)
+        ctx.set_fields(fh, sh)
+        res, g = ctx.adjoint()
+        g = g.astype(np.complex128)
+        fd, ad = [], []
+        for _ in range(32):
+            m = sum((rng.standard_normal() + 1j * rng.standard_normal()) * np.exp(2j * np.pi * (kx * xx + ky * yy))
+                    for ky in range(-2, 3) for kx in range(-2, 3))
+            v = fh * m
+            v *= np.linalg.norm(fh) / np.linalg.norm(v)
+            ad.append(float(np.real(np.sum(np.conj(g) * v))))
+            e = 1e-3
+            etas = []
+            for sgn in (1, -1):
+                ctx.set_fields((fh + sgn * e * v).astype(np.complex64), sh)
+                ctx.forward()
+                etas.append(ctx.powers()[2])
+            fd.append((etas[0] - etas[1]) / (2 * e))
+    fd, ad = np.array(fd), np.array(ad)
+    rel = np.linalg.norm(fd - ad) / np.linalg.norm(ad)
+    print(f"\neta {res['eta']:.6e}: 32 smooth directions, FD step 1e-3 vs adjoint relL2 {rel:.2e}", flush=True)
+    assert rel < 5e-3
