@@ -23,7 +23,7 @@ SHG_HD inline void sincos_phase(float x, float &s, float &c) {
     const double xd = (double)x;
     const double q = rint(xd * 0.63661977236758134308);          // 2/pi
     const float r = (float)fma(-q, 1.5707963267948966192, xd);    // x - q pi/2
-    const int quad = (int)((int64_t)q & 3);
+    const int quad = (int)q & 3;                                  // q exact, |q| < 2^31 for |x| < 3e9
     const float z = r * r;
     const float ps = fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f);
     const float pc = fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z, 4.166664568298827e-2f);
