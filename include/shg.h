@@ -36,10 +36,11 @@ extern "C" {
 #define SHG_OK              0
 #define SHG_E_INVALID     (-1)   /* bad argument or unsupported parameter */
 #define SHG_E_DEVICE      (-2)   /* HIP runtime error */
-#define SHG_E_NOMEM       (-3)   /* device allocation failed */
+#define SHG_E_NOMEM       (-3)   /* device or host allocation failed */
 #define SHG_E_STATE       (-4)   /* call out of order (e.g. forward before set_fields) */
 #define SHG_E_NONFINITE   (-5)   /* NaN/Inf found in a field */
 #define SHG_E_NOCONVERGE  (-6)   /* Jacobi hit its sweep cap before the tolerance */
+#define SHG_E_INTERNAL    (-7)   /* unexpected host-side failure (message says which) */
 
 #define SHG_CRYSTAL_SPPLT   0    /* MgO:sPPLT, src/headers/Crystal.h:55-71 */
 #define SHG_CRYSTAL_PPLN    1    /* MgO:PPLN (Gayer et al. 2008); absent from the reference */

@@ -21,7 +21,7 @@ _lib = None
 
 SHG_OK = 0
 ERRORS = {-1: "SHG_E_INVALID", -2: "SHG_E_DEVICE", -3: "SHG_E_NOMEM", -4: "SHG_E_STATE",
-          -5: "SHG_E_NONFINITE", -6: "SHG_E_NOCONVERGE"}
+          -5: "SHG_E_NONFINITE", -6: "SHG_E_NOCONVERGE", -7: "SHG_E_INTERNAL"}
 
 
 class ThermalStats(ctypes.Structure):

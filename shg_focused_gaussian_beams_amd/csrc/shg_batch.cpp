@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/shg.h"
+#include "guard.h"
 #include "batch_kernels.h"
 #include "physics.h"
 #include "ssfm_kernels.h"
@@ -138,11 +139,9 @@ int after_fields_set(shg_batch *b) {
     return SHG_OK;
 }
 
-// Per size class, times `reps` launches of every layout variant over all the class's
-// instances and keeps the fastest (clobbers U; shg_batch_run restores it from U0).
 // Per size class, times every (row layout, column layout) pair as the schedule runs them
 // (a row pass then a column pass over a group of the class, replayed from a graph) and
-// keeps the fastest pair.
+// keeps the fastest pair (clobbers U; shg_batch_run restores it from U0).
 int tune_batch(shg_batch *b, int reps) {
     for (int c = 0; c < 4; c++) {
         if (!b->cls_cnt[c]) continue;
@@ -232,211 +231,221 @@ int enqueue_batch(shg_batch *b) {
 extern "C" {
 
 int shg_batch_create(const shg_params *params, int count, shg_batch **out) {
-    if (!params || !out || count <= 0) return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: bad argument");
-    *out = nullptr;
-    for (int i = 0; i < count; i++) {
-        const shg_params &p = params[i];
-        if (p.nx != p.ny || size_class(p.nx) < 0)
-            return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: instance " + std::to_string(i) +
-                                                        ": nx == ny in {64, 128, 256, 512} required");
-        if (p.nz < 2 || !(p.lx > 0 && p.ly > 0 && p.lz > 0) || p.thermal ||
-            (p.crystal != SHG_CRYSTAL_SPPLT && p.crystal != SHG_CRYSTAL_PPLN) ||
-            (p.scheme != SHG_SCHEME_REFERENCE && p.scheme != SHG_SCHEME_STRANG))
-            return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: instance " + std::to_string(i) +
-                                                        ": unsupported parameters (batches are plain SSFM)");
-    }
-    if (count > 65535) return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: at most 65535 instances");
-    shg_batch *b = new shg_batch();
-    auto bad = [&](int rc) { batch_release(b); return rc; };
-    b->count = count;
-    b->p.assign(params, params + count);
-    b->order.resize(count);
-    std::iota(b->order.begin(), b->order.end(), 0);
-    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
-        const int cx = size_class(params[x].nx), cy = size_class(params[y].nx);
-        if (cx != cy) return cx < cy;
-        return params[x].nz > params[y].nz;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!params || !out || count <= 0) return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: bad argument");
+        *out = nullptr;
+        for (int i = 0; i < count; i++) {
+            const shg_params &p = params[i];
+            if (p.nx != p.ny || size_class(p.nx) < 0)
+                return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: instance " + std::to_string(i) +
+                                                            ": nx == ny in {64, 128, 256, 512} required");
+            if (p.nz < 2 || !(p.lx > 0 && p.ly > 0 && p.lz > 0) || p.thermal ||
+                (p.crystal != SHG_CRYSTAL_SPPLT && p.crystal != SHG_CRYSTAL_PPLN) ||
+                (p.scheme != SHG_SCHEME_REFERENCE && p.scheme != SHG_SCHEME_STRANG))
+                return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: instance " + std::to_string(i) +
+                                                            ": unsupported parameters (batches are plain SSFM)");
+        }
+        if (count > 65535) return shg_internal_fail(SHG_E_INVALID, "shg_batch_create: at most 65535 instances");
+        shg_batch *b = new shg_batch();
+        auto bad = [&](int rc) { batch_release(b); return rc; };
+        b->count = count;
+        b->p.assign(params, params + count);
+        b->order.resize(count);
+        std::iota(b->order.begin(), b->order.end(), 0);
+        std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+            const int cx = size_class(params[x].nx), cy = size_class(params[y].nx);
+            if (cx != cy) return cx < cy;
+            return params[x].nz > params[y].nz;
+        });
+        b->slot_of.resize(count);
+        b->n.resize(count);
+        b->steps.resize(count);
+        b->cell_off.resize(count);
+        b->t_slot.resize(count);
+        size_t coef_total = 0, g_total = 0;
+        std::vector<size_t> coef_off(count), g_off(count);
+        for (int s = 0; s < count; s++) {
+            const shg_params &p = params[b->order[s]];
+            b->slot_of[b->order[s]] = s;
+            b->n[s] = p.nx;
+            b->steps[s] = p.nz - 1;
+            b->cell_off[s] = b->total_cells;
+            b->total_cells += (size_t)p.nx * p.nx;
+            coef_off[s] = coef_total;
+            coef_total += b->steps[s];
+            g_off[s] = g_total;
+            g_total += 4 * (size_t)p.nx;
+            b->max_steps = std::max(b->max_steps, b->steps[s]);
+            const int c = size_class(p.nx);
+            if (b->cls_cnt[c] == 0) b->cls_off[c] = s;
+            b->cls_cnt[c]++;
+            b->cls_steps[c].push_back(b->steps[s]);
+            b->t_slot[s] = (float)p.temp;
+        }
+        bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+        for (int k = 0; ok && k < kMaxStreams - 1; k++)
+            ok = hipStreamCreateWithFlags(&b->xs[k], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) == hipSuccess;
+        if (!ok) return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: stream/event creation failed"));
+        build_groups(b);
+        if (hipMalloc(&b->U, b->total_cells * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b->U0, b->total_cells * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b->G, g_total * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&b->coef, coef_total * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&b->desc, count * sizeof(shg::InstDesc)) != hipSuccess ||
+            hipMalloc(&b->ncell, count * sizeof(int)) != hipSuccess || hipMalloc(&b->pow, count * sizeof(double2)) != hipSuccess)
+            return bad(shg_internal_fail(SHG_E_NOMEM, "shg_batch_create: device allocation failed"));
+        for (int c = 0; c < 4; c++) {
+            if (!b->cls_cnt[c]) continue;
+            const auto tw = shg::phys::twiddle_table(kSizes[c]);
+            if (hipMalloc(&b->tw[c], tw.size() * sizeof(float)) != hipSuccess ||
+                shg_internal_copy(b->tw[c], tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+                return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: twiddle upload failed"));
+        }
+        std::vector<float> gh(2 * g_total), ch(4 * coef_total);
+        std::vector<shg::InstDesc> dh(count);
+        std::vector<int> nc(count);
+        for (int s = 0; s < count; s++) {
+            const shg_params &p = params[b->order[s]];
+            const shg::phys::Consts k(p);
+            const float T = (float)p.temp;
+            const auto g = shg::phys::propagator_factors(k, T, (double)k.dz);
+            std::copy(g.begin(), g.end(), gh.begin() + 2 * g_off[s]);
+            const auto cf = shg::phys::step_coefficients(k, T, b->steps[s]);
+            std::copy(cf.begin(), cf.begin() + 4 * (size_t)b->steps[s], ch.begin() + 4 * coef_off[s]);
+            shg::InstDesc &d = dh[s];
+            d.U = b->U + b->cell_off[s];
+            d.G = b->G + g_off[s];
+            d.coef = b->coef + coef_off[s];
+            const float h = p.scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
+            d.nl.lossP = (float)(0.5 * (double)k.alpha_p);
+            d.nl.lossS = (float)(0.5 * (double)k.alpha_s);
+            d.nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
+            d.nl.h6 = h / 6.0f;
+            d.steps = b->steps[s];
+            nc[s] = p.nx * p.nx;
+        }
+        hipStream_t st = b->stream;
+        if (shg_internal_copy(b->G, gh.data(), gh.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
+            shg_internal_copy(b->coef, ch.data(), ch.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
+            shg_internal_copy(b->desc, dh.data(), dh.size() * sizeof(shg::InstDesc), hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            shg_internal_copy(b->ncell, nc.data(), nc.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+            return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: table upload failed"));
+        *out = b;
+        return SHG_OK;
     });
-    b->slot_of.resize(count);
-    b->n.resize(count);
-    b->steps.resize(count);
-    b->cell_off.resize(count);
-    b->t_slot.resize(count);
-    size_t coef_total = 0, g_total = 0;
-    std::vector<size_t> coef_off(count), g_off(count);
-    for (int s = 0; s < count; s++) {
-        const shg_params &p = params[b->order[s]];
-        b->slot_of[b->order[s]] = s;
-        b->n[s] = p.nx;
-        b->steps[s] = p.nz - 1;
-        b->cell_off[s] = b->total_cells;
-        b->total_cells += (size_t)p.nx * p.nx;
-        coef_off[s] = coef_total;
-        coef_total += b->steps[s];
-        g_off[s] = g_total;
-        g_total += 4 * (size_t)p.nx;
-        b->max_steps = std::max(b->max_steps, b->steps[s]);
-        const int c = size_class(p.nx);
-        if (b->cls_cnt[c] == 0) b->cls_off[c] = s;
-        b->cls_cnt[c]++;
-        b->cls_steps[c].push_back(b->steps[s]);
-        b->t_slot[s] = (float)p.temp;
-    }
-    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreate(&b->e0) == hipSuccess && hipEventCreate(&b->e1) == hipSuccess &&
-              hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
-    for (int k = 0; ok && k < kMaxStreams - 1; k++)
-        ok = hipStreamCreateWithFlags(&b->xs[k], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming) == hipSuccess;
-    if (!ok) return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: stream/event creation failed"));
-    build_groups(b);
-    if (hipMalloc(&b->U, b->total_cells * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&b->U0, b->total_cells * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&b->G, g_total * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&b->coef, coef_total * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&b->desc, count * sizeof(shg::InstDesc)) != hipSuccess ||
-        hipMalloc(&b->ncell, count * sizeof(int)) != hipSuccess || hipMalloc(&b->pow, count * sizeof(double2)) != hipSuccess)
-        return bad(shg_internal_fail(SHG_E_NOMEM, "shg_batch_create: device allocation failed"));
-    for (int c = 0; c < 4; c++) {
-        if (!b->cls_cnt[c]) continue;
-        const auto tw = shg::phys::twiddle_table(kSizes[c]);
-        if (hipMalloc(&b->tw[c], tw.size() * sizeof(float)) != hipSuccess ||
-            shg_internal_copy(b->tw[c], tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) != hipSuccess)
-            return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: twiddle upload failed"));
-    }
-    std::vector<float> gh(2 * g_total), ch(4 * coef_total);
-    std::vector<shg::InstDesc> dh(count);
-    std::vector<int> nc(count);
-    for (int s = 0; s < count; s++) {
-        const shg_params &p = params[b->order[s]];
-        const shg::phys::Consts k(p);
-        const float T = (float)p.temp;
-        const auto g = shg::phys::propagator_factors(k, T, (double)k.dz);
-        std::copy(g.begin(), g.end(), gh.begin() + 2 * g_off[s]);
-        const auto cf = shg::phys::step_coefficients(k, T, b->steps[s]);
-        std::copy(cf.begin(), cf.begin() + 4 * (size_t)b->steps[s], ch.begin() + 4 * coef_off[s]);
-        shg::InstDesc &d = dh[s];
-        d.U = b->U + b->cell_off[s];
-        d.G = b->G + g_off[s];
-        d.coef = b->coef + coef_off[s];
-        const float h = p.scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
-        d.nl.lossP = (float)(0.5 * (double)k.alpha_p);
-        d.nl.lossS = (float)(0.5 * (double)k.alpha_s);
-        d.nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
-        d.nl.h6 = h / 6.0f;
-        d.steps = b->steps[s];
-        nc[s] = p.nx * p.nx;
-    }
-    hipStream_t st = b->stream;
-    if (shg_internal_copy(b->G, gh.data(), gh.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
-        shg_internal_copy(b->coef, ch.data(), ch.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess ||
-        shg_internal_copy(b->desc, dh.data(), dh.size() * sizeof(shg::InstDesc), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        shg_internal_copy(b->ncell, nc.data(), nc.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
-        return bad(shg_internal_fail(SHG_E_DEVICE, "shg_batch_create: table upload failed"));
-    *out = b;
-    return SHG_OK;
 }
 
 void shg_batch_destroy(shg_batch *b) { batch_release(b); }
 
 int shg_batch_init_fields(shg_batch *b, double noise_rel, uint64_t seed) {
-    if (!b || !(noise_rel >= 0)) return shg_internal_fail(SHG_E_INVALID, "shg_batch_init_fields: bad argument");
-    std::vector<shg::PumpArgs> a(b->count);
-    int max_cells = 0;
-    for (int s = 0; s < b->count; s++) {
-        const int i = b->order[s];
-        const shg_params &p = b->p[i];
-        const shg::phys::Consts k(p);
-        // the prologue of setPump (src/headers/Efields.h:77-92) in float
-        const float T = (float)p.temp, w2 = (float)p.waist * (float)p.waist;
-        const float np = k.n_p(T);
-        const float zr = k.pi * np * w2 / k.lp;
-        const float ap0 = sqrtf(4.0f * (float)p.power / (k.eps0 * k.c * k.pi * np * w2));
-        const float eta = (float)p.focus / zr;
-        const float mr = 1.0f, mi = -eta;
-        const float mden = mr * mr + mi * mi;
-        shg::PumpArgs &q = a[s];
-        q.U = b->U0 + b->cell_off[s];
-        q.ar = (ap0 * mr) / mden;
-        q.ai = (-ap0 * mi) / mden;
-        q.qr = w2 * mr;
-        q.qi = w2 * mi;
-        q.qden = q.qr * q.qr + q.qi * q.qi;
-        q.dx = k.dx; q.dy = k.dy;
-        q.ux = (float)(0.5 * p.nx); q.uy = (float)(0.5 * p.ny);
-        q.sigma = (float)(noise_rel * std::sqrt((double)q.ar * q.ar + (double)q.ai * q.ai));
-        q.n = p.nx;
-        q.seed = seed * 0x100000001B3ull + (uint64_t)i;
-        max_cells = std::max(max_cells, p.nx * p.nx);
-    }
-    shg::PumpArgs *d = nullptr;
-    BHIP_TRY(hipMalloc(&d, a.size() * sizeof(shg::PumpArgs)));
-    hipError_t e = shg_internal_copy(d, a.data(), a.size() * sizeof(shg::PumpArgs), hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = shg::launch_pump_init(d, b->count, max_cells, b->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    (void)hipFree(d);
-    if (e != hipSuccess) return shg_internal_fail(SHG_E_DEVICE, std::string("shg_batch_init_fields: ") + hipGetErrorString(e));
-    return after_fields_set(b);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !(noise_rel >= 0)) return shg_internal_fail(SHG_E_INVALID, "shg_batch_init_fields: bad argument");
+        std::vector<shg::PumpArgs> a(b->count);
+        int max_cells = 0;
+        for (int s = 0; s < b->count; s++) {
+            const int i = b->order[s];
+            const shg_params &p = b->p[i];
+            const shg::phys::Consts k(p);
+            // the prologue of setPump (src/headers/Efields.h:77-92) in float
+            const float T = (float)p.temp, w2 = (float)p.waist * (float)p.waist;
+            const float np = k.n_p(T);
+            const float zr = k.pi * np * w2 / k.lp;
+            const float ap0 = sqrtf(4.0f * (float)p.power / (k.eps0 * k.c * k.pi * np * w2));
+            const float eta = (float)p.focus / zr;
+            const float mr = 1.0f, mi = -eta;
+            const float mden = mr * mr + mi * mi;
+            shg::PumpArgs &q = a[s];
+            q.U = b->U0 + b->cell_off[s];
+            q.ar = (ap0 * mr) / mden;
+            q.ai = (-ap0 * mi) / mden;
+            q.qr = w2 * mr;
+            q.qi = w2 * mi;
+            q.qden = q.qr * q.qr + q.qi * q.qi;
+            q.dx = k.dx; q.dy = k.dy;
+            q.ux = (float)(0.5 * p.nx); q.uy = (float)(0.5 * p.ny);
+            q.sigma = (float)(noise_rel * std::sqrt((double)q.ar * q.ar + (double)q.ai * q.ai));
+            q.n = p.nx;
+            q.seed = seed * 0x100000001B3ull + (uint64_t)i;
+            max_cells = std::max(max_cells, p.nx * p.nx);
+        }
+        shg::PumpArgs *d = nullptr;
+        BHIP_TRY(hipMalloc(&d, a.size() * sizeof(shg::PumpArgs)));
+        hipError_t e = shg_internal_copy(d, a.data(), a.size() * sizeof(shg::PumpArgs), hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = shg::launch_pump_init(d, b->count, max_cells, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        (void)hipFree(d);
+        if (e != hipSuccess) return shg_internal_fail(SHG_E_DEVICE, std::string("shg_batch_init_fields: ") + hipGetErrorString(e));
+        return after_fields_set(b);
+    });
 }
 
 int shg_batch_set_fields(shg_batch *b, int index, const float *fh, const float *sh) {
-    if (!b || !fh || !sh || index < 0 || index >= b->count)
-        return shg_internal_fail(SHG_E_INVALID, "shg_batch_set_fields: bad argument");
-    const int s = b->slot_of[index];
-    const size_t nc = (size_t)b->n[s] * b->n[s];
-    std::vector<float> h(4 * nc);
-    for (size_t i = 0; i < nc; i++) {
-        h[4 * i] = fh[2 * i]; h[4 * i + 1] = fh[2 * i + 1];
-        h[4 * i + 2] = sh[2 * i]; h[4 * i + 3] = sh[2 * i + 1];
-    }
-    BHIP_TRY(shg_internal_copy(b->U0 + b->cell_off[s], h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, b->stream));
-    return after_fields_set(b);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !fh || !sh || index < 0 || index >= b->count)
+            return shg_internal_fail(SHG_E_INVALID, "shg_batch_set_fields: bad argument");
+        const int s = b->slot_of[index];
+        const size_t nc = (size_t)b->n[s] * b->n[s];
+        std::vector<float> h(4 * nc);
+        for (size_t i = 0; i < nc; i++) {
+            h[4 * i] = fh[2 * i]; h[4 * i + 1] = fh[2 * i + 1];
+            h[4 * i + 2] = sh[2 * i]; h[4 * i + 3] = sh[2 * i + 1];
+        }
+        BHIP_TRY(shg_internal_copy(b->U0 + b->cell_off[s], h.data(), nc * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+        return after_fields_set(b);
+    });
 }
 
 int shg_batch_set_schedule(shg_batch *b, int group_mib, int streams) {
-    if (!b || group_mib < 1 || streams < 1 || streams > kMaxStreams)
-        return shg_internal_fail(SHG_E_INVALID, "shg_batch_set_schedule: group_mib >= 1, 1 <= streams <= " +
-                                                    std::to_string(kMaxStreams));
-    b->group_mb = group_mib;
-    b->nstreams = streams;
-    build_groups(b);
-    if (b->gexec) {
-        (void)hipGraphExecDestroy(b->gexec);
-        b->gexec = nullptr;
-    }
-    for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = -1;   // group sizes changed: tune again
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || group_mib < 1 || streams < 1 || streams > kMaxStreams)
+            return shg_internal_fail(SHG_E_INVALID, "shg_batch_set_schedule: group_mib >= 1, 1 <= streams <= " +
+                                                        std::to_string(kMaxStreams));
+        b->group_mb = group_mib;
+        b->nstreams = streams;
+        build_groups(b);
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+        }
+        for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = -1;   // group sizes changed: tune again
+        return SHG_OK;
+    });
 }
 
 int shg_batch_run(shg_batch *b) {
-    if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_run: null batch");
-    if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_run: set or init the fields first");
-    if (b->row_v[0] < 0 && b->row_v[1] < 0 && b->row_v[2] < 0 && b->row_v[3] < 0) {
-        const char *nt = std::getenv("SHG_NO_TUNE");
-        if (nt && nt[0] == '1') {
-            for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = 0;
-        } else if (int rc = tune_batch(b, 5)) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_run: null batch");
+        if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_run: set or init the fields first");
+        if (b->row_v[0] < 0 && b->row_v[1] < 0 && b->row_v[2] < 0 && b->row_v[3] < 0) {
+            const char *nt = std::getenv("SHG_NO_TUNE");
+            if (nt && nt[0] == '1') {
+                for (int c = 0; c < 4; c++) b->row_v[c] = b->col_v[c] = 0;
+            } else if (int rc = tune_batch(b, 5)) {
+                return rc;
+            }
+        }
+        BHIP_TRY(hipMemcpyAsync(b->U, b->U0, b->total_cells * sizeof(float4), hipMemcpyDeviceToDevice, b->stream));
+        const char *ng = std::getenv("SHG_NO_GRAPH");
+        const bool graph = !(ng && ng[0] == '1');
+        if (graph && !b->gexec)
+            if (int rc = shg_internal_capture(b->stream, [b] { return enqueue_batch(b); }, &b->gexec)) return rc;
+        BHIP_TRY(hipEventRecord(b->e0, b->stream));
+        if (graph) {
+            BHIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+        } else if (int rc = enqueue_batch(b)) {
             return rc;
         }
-    }
-    BHIP_TRY(hipMemcpyAsync(b->U, b->U0, b->total_cells * sizeof(float4), hipMemcpyDeviceToDevice, b->stream));
-    const char *ng = std::getenv("SHG_NO_GRAPH");
-    const bool graph = !(ng && ng[0] == '1');
-    if (graph && !b->gexec)
-        if (int rc = shg_internal_capture(b->stream, [b] { return enqueue_batch(b); }, &b->gexec)) return rc;
-    BHIP_TRY(hipEventRecord(b->e0, b->stream));
-    if (graph) {
-        BHIP_TRY(hipGraphLaunch(b->gexec, b->stream));
-    } else if (int rc = enqueue_batch(b)) {
-        return rc;
-    }
-    BHIP_TRY(hipEventRecord(b->e1, b->stream));
-    BHIP_TRY(hipEventSynchronize(b->e1));
-    float ms = 0.f;
-    BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
-    b->last_ms = ms;
-    return SHG_OK;
+        BHIP_TRY(hipEventRecord(b->e1, b->stream));
+        BHIP_TRY(hipEventSynchronize(b->e1));
+        float ms = 0.f;
+        BHIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+        b->last_ms = ms;
+        return SHG_OK;
+    });
 }
 
 static int batch_copy_out(shg_batch *b, const float4 *src_base, int index, float *fh, float *sh) {
@@ -453,31 +462,37 @@ static int batch_copy_out(shg_batch *b, const float4 *src_base, int index, float
 }
 
 int shg_batch_get_fields(shg_batch *b, int index, float *fh, float *sh) {
-    if (!b || !fh || !sh || index < 0 || index >= b->count)
-        return shg_internal_fail(SHG_E_INVALID, "shg_batch_get_fields: bad argument");
-    return batch_copy_out(b, b->U, index, fh, sh);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !fh || !sh || index < 0 || index >= b->count)
+            return shg_internal_fail(SHG_E_INVALID, "shg_batch_get_fields: bad argument");
+        return batch_copy_out(b, b->U, index, fh, sh);
+    });
 }
 
 int shg_batch_get_input(shg_batch *b, int index, float *fh, float *sh) {
-    if (!b || !fh || !sh || index < 0 || index >= b->count)
-        return shg_internal_fail(SHG_E_INVALID, "shg_batch_get_input: bad argument");
-    return batch_copy_out(b, b->U0, index, fh, sh);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !fh || !sh || index < 0 || index >= b->count)
+            return shg_internal_fail(SHG_E_INVALID, "shg_batch_get_input: bad argument");
+        return batch_copy_out(b, b->U0, index, fh, sh);
+    });
 }
 
 int shg_batch_powers(shg_batch *b, double *p_fh_in, double *p_sh_out, double *eta) {
-    if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_powers: null batch");
-    if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_powers: no fields");
-    std::vector<double2> sums;
-    if (int rc = batch_sums(b, sums)) return rc;
-    for (int i = 0; i < b->count; i++) {
-        const int s = b->slot_of[i];
-        if (!std::isfinite(sums[s].y)) return shg_internal_fail(SHG_E_NONFINITE, "shg_batch_powers: non-finite field");
-        const double po = slot_power(b, s, 1, sums[s].y);
-        if (p_fh_in) p_fh_in[i] = b->p_in[s];
-        if (p_sh_out) p_sh_out[i] = po;
-        if (eta) eta[i] = b->p_in[s] > 0 ? po / b->p_in[s] : 0.0;
-    }
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_batch_powers: null batch");
+        if (!b->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_batch_powers: no fields");
+        std::vector<double2> sums;
+        if (int rc = batch_sums(b, sums)) return rc;
+        for (int i = 0; i < b->count; i++) {
+            const int s = b->slot_of[i];
+            if (!std::isfinite(sums[s].y)) return shg_internal_fail(SHG_E_NONFINITE, "shg_batch_powers: non-finite field");
+            const double po = slot_power(b, s, 1, sums[s].y);
+            if (p_fh_in) p_fh_in[i] = b->p_in[s];
+            if (p_sh_out) p_sh_out[i] = po;
+            if (eta) eta[i] = b->p_in[s] > 0 ? po / b->p_in[s] : 0.0;
+        }
+        return SHG_OK;
+    });
 }
 
 int64_t shg_batch_cell_steps(const shg_batch *b) {
@@ -488,9 +503,11 @@ int64_t shg_batch_cell_steps(const shg_batch *b) {
 }
 
 int shg_batch_last_ms(shg_batch *b, double *ms) {
-    if (!b || !ms) return shg_internal_fail(SHG_E_INVALID, "shg_batch_last_ms: bad argument");
-    *ms = b->last_ms;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !ms) return shg_internal_fail(SHG_E_INVALID, "shg_batch_last_ms: bad argument");
+        *ms = b->last_ms;
+        return SHG_OK;
+    });
 }
 
 }  // extern "C"

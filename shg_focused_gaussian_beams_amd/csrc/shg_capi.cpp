@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/shg.h"
+#include "guard.h"
 #include "physics.h"
 #include "ssfm_kernels.h"
 #include "thermal_kernels.h"
@@ -394,401 +395,441 @@ void shg_default_params(shg_params *p) {
 }
 
 int shg_create(const shg_params *p, shg_ctx **out) {
-    if (!p || !out) return fail(SHG_E_INVALID, "shg_create: null argument");
-    *out = nullptr;
-    if (p->nx != p->ny) return fail(SHG_E_INVALID, "shg_create: nx must equal ny");
-    if (!size_supported(p->nx)) return fail(SHG_E_INVALID, "shg_create: nx must be one of 64, 128, 256, 512");
-    if (p->nz < 2) return fail(SHG_E_INVALID, "shg_create: nz must be >= 2");
-    if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return fail(SHG_E_INVALID, "shg_create: box lengths must be > 0");
-    if (!(p->lambda_p > 0)) return fail(SHG_E_INVALID, "shg_create: lambda_p must be > 0");
-    if (p->crystal != SHG_CRYSTAL_SPPLT && p->crystal != SHG_CRYSTAL_PPLN)
-        return fail(SHG_E_INVALID, "shg_create: unknown crystal");
-    if (p->scheme != SHG_SCHEME_REFERENCE && p->scheme != SHG_SCHEME_STRANG)
-        return fail(SHG_E_INVALID, "shg_create: unknown scheme");
-    if (p->thermal) {
-        const int hz = p->heat_nz > 0 ? p->heat_nz : p->nz;
-        if (hz < 2 || hz > p->nz) return fail(SHG_E_INVALID, "shg_create: heat_nz must be in [2, nz]");
-        if (!(p->k_th > 0)) return fail(SHG_E_INVALID, "shg_create: k_th must be > 0");
-        if (p->thermal_bc != SHG_BC_TOP_OPEN && p->thermal_bc != SHG_BC_OVEN_SURROUNDED)
-            return fail(SHG_E_INVALID, "shg_create: unknown thermal_bc");
-    }
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!p || !out) return fail(SHG_E_INVALID, "shg_create: null argument");
+        *out = nullptr;
+        if (p->nx != p->ny) return fail(SHG_E_INVALID, "shg_create: nx must equal ny");
+        if (!size_supported(p->nx)) return fail(SHG_E_INVALID, "shg_create: nx must be one of 64, 128, 256, 512");
+        if (p->nz < 2) return fail(SHG_E_INVALID, "shg_create: nz must be >= 2");
+        if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return fail(SHG_E_INVALID, "shg_create: box lengths must be > 0");
+        if (!(p->lambda_p > 0)) return fail(SHG_E_INVALID, "shg_create: lambda_p must be > 0");
+        if (p->crystal != SHG_CRYSTAL_SPPLT && p->crystal != SHG_CRYSTAL_PPLN)
+            return fail(SHG_E_INVALID, "shg_create: unknown crystal");
+        if (p->scheme != SHG_SCHEME_REFERENCE && p->scheme != SHG_SCHEME_STRANG)
+            return fail(SHG_E_INVALID, "shg_create: unknown scheme");
+        if (p->thermal) {
+            const int hz = p->heat_nz > 0 ? p->heat_nz : p->nz;
+            if (hz < 2 || hz > p->nz) return fail(SHG_E_INVALID, "shg_create: heat_nz must be in [2, nz]");
+            if (!(p->k_th > 0)) return fail(SHG_E_INVALID, "shg_create: k_th must be > 0");
+            if (p->thermal_bc != SHG_BC_TOP_OPEN && p->thermal_bc != SHG_BC_OVEN_SURROUNDED)
+                return fail(SHG_E_INVALID, "shg_create: unknown thermal_bc");
+        }
 
-    shg_ctx *c = new shg_ctx();
-    c->p = *p;
-    c->n = p->nx;
-    c->steps = p->thermal ? p->nz : p->nz - 1;
-    c->hz = p->thermal ? (p->heat_nz > 0 ? p->heat_nz : p->nz) : 0;
-    const char *g = std::getenv("SHG_NO_GRAPH");
-    c->use_graph = (g && g[0] == '1') ? 0 : 1;
-    const size_t ncell = (size_t)c->n * c->n;
-    auto bad = [&](int code) { release(c); return code; };
-    hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
-        return bad(fail(SHG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
-    c->nparts = 256;
-    if (hipMalloc(&c->U, ncell * sizeof(float4)) != hipSuccess || hipMalloc(&c->U0, ncell * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&c->tw, (size_t)c->n * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&c->G, 4 * (size_t)c->n * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&c->part, c->nparts * sizeof(double2)) != hipSuccess || hipMalloc(&c->red, sizeof(double2)) != hipSuccess)
-        return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-    if (hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
-        return bad(fail(SHG_E_DEVICE, "shg_create: hipEventCreate failed"));
-
-    const shg::phys::Consts k(*p);
-    const float T = (float)p->temp;
-    const auto tw = shg::phys::twiddle_table(c->n);
-    if (shg_internal_copy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
-        return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
-    const float h = p->scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
-    c->nl.lossP = (float)(0.5 * (double)k.alpha_p);
-    c->nl.lossS = (float)(0.5 * (double)k.alpha_s);
-    c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
-    c->nl.h6 = h / 6.0f;
-    c->host_stage.resize(4 * ncell);
-    if (hipMalloc(&c->d_desc, sizeof(shg::InstDesc)) != hipSuccess)
-        return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-    if (!p->thermal) {
-        c->coef = shg::phys::step_coefficients(k, T, c->steps);
-        if (hipMalloc(&c->d_coef, c->coef.size() * sizeof(float)) != hipSuccess)
+        shg_ctx *c = new shg_ctx();
+        c->p = *p;
+        c->n = p->nx;
+        c->steps = p->thermal ? p->nz : p->nz - 1;
+        c->hz = p->thermal ? (p->heat_nz > 0 ? p->heat_nz : p->nz) : 0;
+        const char *g = std::getenv("SHG_NO_GRAPH");
+        c->use_graph = (g && g[0] == '1') ? 0 : 1;
+        const size_t ncell = (size_t)c->n * c->n;
+        auto bad = [&](int code) { release(c); return code; };
+        hipError_t e;
+        if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+            return bad(fail(SHG_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+        c->nparts = 256;
+        if (hipMalloc(&c->U, ncell * sizeof(float4)) != hipSuccess || hipMalloc(&c->U0, ncell * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&c->tw, (size_t)c->n * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&c->G, 4 * (size_t)c->n * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&c->part, c->nparts * sizeof(double2)) != hipSuccess || hipMalloc(&c->red, sizeof(double2)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-        if (shg_internal_copy(c->d_coef, c->coef.data(), c->coef.size() * sizeof(float), hipMemcpyHostToDevice,
-                              c->stream) != hipSuccess)
+        if (hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess)
+            return bad(fail(SHG_E_DEVICE, "shg_create: hipEventCreate failed"));
+
+        const shg::phys::Consts k(*p);
+        const float T = (float)p->temp;
+        const auto tw = shg::phys::twiddle_table(c->n);
+        if (shg_internal_copy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
-    } else {
-        const size_t n3 = ncell * c->hz;
-        c->njpart = shg::jacobi_blocks(c->n, c->n, c->hz);
-        if (hipMalloc(&c->Uout, ncell * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
-            hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
-            hipMalloc(&c->ntab, n3 * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&c->ctab, n3 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
-            hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
+        const float h = p->scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
+        c->nl.lossP = (float)(0.5 * (double)k.alpha_p);
+        c->nl.lossS = (float)(0.5 * (double)k.alpha_s);
+        c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
+        c->nl.h6 = h / 6.0f;
+        c->host_stage.resize(4 * ncell);
+        if (hipMalloc(&c->d_desc, sizeof(shg::InstDesc)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-        const float hdz = shg::phys::heat_dz(k, c->hz);
-        c->jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p->k_th, (float)p->h_conv, (float)p->t_inf);
-        c->jc.bc = p->thermal_bc;
-        shg::ThermConst &tc = c->tc;
-        tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
-        tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
-        tc.tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
-        tc.q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
-        tc.q_s = (double)k.alpha_s * 0.5 * (double)k.c * (double)k.eps0;
-        tc.q_i = 0.5 * (double)k.c * (double)k.eps0;
-        tc.beta_s = k.beta_s;
-        tc.crystal = p->crystal;
-        if (int rc = thermal_init_temperature(c)) return bad(rc);
-    }
-    shg::InstDesc hd{};
-    hd.U = c->U;
-    hd.G = c->G;
-    hd.coef = c->d_coef;
-    hd.nl = c->nl;
-    hd.steps = c->steps;
-    if (shg_internal_copy(c->d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice, c->stream) != hipSuccess)
-        return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
-    if (int rc = upload_propagators(c, T)) return bad(rc);
-    *out = c;
-    return SHG_OK;
+        if (!p->thermal) {
+            c->coef = shg::phys::step_coefficients(k, T, c->steps);
+            if (hipMalloc(&c->d_coef, c->coef.size() * sizeof(float)) != hipSuccess)
+                return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
+            if (shg_internal_copy(c->d_coef, c->coef.data(), c->coef.size() * sizeof(float), hipMemcpyHostToDevice,
+                                  c->stream) != hipSuccess)
+                return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
+        } else {
+            const size_t n3 = ncell * c->hz;
+            c->njpart = shg::jacobi_blocks(c->n, c->n, c->hz);
+            if (hipMalloc(&c->Uout, ncell * sizeof(float4)) != hipSuccess ||
+                hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
+                hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
+                hipMalloc(&c->ntab, n3 * sizeof(float2)) != hipSuccess ||
+                hipMalloc(&c->ctab, n3 * sizeof(float4)) != hipSuccess ||
+                hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
+                hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
+                return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
+            const float hdz = shg::phys::heat_dz(k, c->hz);
+            c->jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p->k_th, (float)p->h_conv, (float)p->t_inf);
+            c->jc.bc = p->thermal_bc;
+            shg::ThermConst &tc = c->tc;
+            tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
+            tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
+            tc.tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
+            tc.q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
+            tc.q_s = (double)k.alpha_s * 0.5 * (double)k.c * (double)k.eps0;
+            tc.q_i = 0.5 * (double)k.c * (double)k.eps0;
+            tc.beta_s = k.beta_s;
+            tc.crystal = p->crystal;
+            if (int rc = thermal_init_temperature(c)) return bad(rc);
+        }
+        shg::InstDesc hd{};
+        hd.U = c->U;
+        hd.G = c->G;
+        hd.coef = c->d_coef;
+        hd.nl = c->nl;
+        hd.steps = c->steps;
+        if (shg_internal_copy(c->d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
+        if (int rc = upload_propagators(c, T)) return bad(rc);
+        *out = c;
+        return SHG_OK;
+    });
 }
 
 void shg_destroy(shg_ctx *ctx) { release(ctx); }
 
 int shg_make_pump(const shg_params *p, double t_centre, float *fh) {
-    if (!p || !fh) return fail(SHG_E_INVALID, "shg_make_pump: null argument");
-    if (p->nx < 2 || p->ny < 2) return fail(SHG_E_INVALID, "shg_make_pump: bad grid");
-    const shg::phys::Consts k(*p);
-    shg::phys::make_pump(k, (float)p->power, (float)p->focus, (float)p->waist, (float)t_centre, fh);
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!p || !fh) return fail(SHG_E_INVALID, "shg_make_pump: null argument");
+        if (p->nx < 2 || p->ny < 2) return fail(SHG_E_INVALID, "shg_make_pump: bad grid");
+        const shg::phys::Consts k(*p);
+        shg::phys::make_pump(k, (float)p->power, (float)p->focus, (float)p->waist, (float)t_centre, fh);
+        return SHG_OK;
+    });
 }
 
 int shg_set_fields(shg_ctx *c, const float *fh, const float *sh) {
-    if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_set_fields: null argument");
-    const size_t ncell = (size_t)c->n * c->n;
-    RC_TRY(set_u_from_host(c, c->U0, fh, sh));
-    HIP_TRY(hipMemcpyAsync(c->U, c->U0, ncell * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    double sf, ss;
-    RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
-    if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_set_fields: non-finite input field");
-    c->p_fh_in = power_of(c, 0, sf, c->t_centre);
-    c->have_fields = true;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_set_fields: null argument");
+        const size_t ncell = (size_t)c->n * c->n;
+        RC_TRY(set_u_from_host(c, c->U0, fh, sh));
+        HIP_TRY(hipMemcpyAsync(c->U, c->U0, ncell * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        double sf, ss;
+        RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
+        if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_set_fields: non-finite input field");
+        c->p_fh_in = power_of(c, 0, sf, c->t_centre);
+        c->have_fields = true;
+        return SHG_OK;
+    });
 }
 
 int shg_forward(shg_ctx *c) {
-    if (!c) return fail(SHG_E_INVALID, "shg_forward: null context");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward: call shg_set_fields first");
-    HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    return timed_march(c);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_forward: null context");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward: call shg_set_fields first");
+        HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        return timed_march(c);
+    });
 }
 
 int shg_reset_fields(shg_ctx *c) {
-    if (!c) return fail(SHG_E_INVALID, "shg_reset_fields: null context");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_reset_fields: call shg_set_fields first");
-    HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_reset_fields: null context");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_reset_fields: call shg_set_fields first");
+        HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return SHG_OK;
+    });
 }
 
 int shg_march(shg_ctx *c, int first_step, int nsteps) {
-    if (!c) return fail(SHG_E_INVALID, "shg_march: null context");
-    if (c->p.thermal) return fail(SHG_E_INVALID, "shg_march: THERMAL contexts march whole solves (shg_forward)");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_march: call shg_set_fields first");
-    if (first_step < 0 || nsteps < 0 || first_step + nsteps > c->steps)
-        return fail(SHG_E_INVALID, "shg_march: step range outside [0, nz-1)");
-    RC_TRY(enqueue_march_plain(c, first_step, nsteps));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_march: null context");
+        if (c->p.thermal) return fail(SHG_E_INVALID, "shg_march: THERMAL contexts march whole solves (shg_forward)");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_march: call shg_set_fields first");
+        if (first_step < 0 || nsteps < 0 || first_step + nsteps > c->steps)
+            return fail(SHG_E_INVALID, "shg_march: step range outside [0, nz-1)");
+        RC_TRY(enqueue_march_plain(c, first_step, nsteps));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return SHG_OK;
+    });
 }
 
 int shg_forward_snapshots(shg_ctx *c, float *fh_steps, float *sh_steps) {
-    if (!c || !fh_steps || !sh_steps) return fail(SHG_E_INVALID, "shg_forward_snapshots: null argument");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward_snapshots: call shg_set_fields first");
-    const int nz = c->p.nz;
-    const size_t n2 = (size_t)c->n * c->n;
-    if (c->row_v < 0) {                                     // the layouts shg_forward would pick
-        RC_TRY(timed_march(c));
-    }
-    float4 *snaps = nullptr;
-    if (hipMalloc(&snaps, (size_t)nz * n2 * sizeof(float4)) != hipSuccess)
-        return fail(SHG_E_NOMEM, "shg_forward_snapshots: device allocation failed");
-    auto run = [&]() -> int {
-        HIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(snaps, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-        if (c->p.thermal) {
-            HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)n2 * c->hz, c->tc, c->stream));
-            RC_TRY(enqueue_march_thermal(c, snaps));
-            HIP_TRY(hipMemcpyAsync(c->Uout, snaps + (size_t)(nz - 1) * n2, n2 * sizeof(float4),
-                                   hipMemcpyDeviceToDevice, c->stream));
-        } else {
-            RC_TRY(enqueue_march_plain(c, 0, c->steps, snaps));
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !fh_steps || !sh_steps) return fail(SHG_E_INVALID, "shg_forward_snapshots: null argument");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_forward_snapshots: call shg_set_fields first");
+        const int nz = c->p.nz;
+        const size_t n2 = (size_t)c->n * c->n;
+        if (c->row_v < 0) {                                     // the layouts shg_forward would pick
+            RC_TRY(timed_march(c));
         }
-        std::vector<float> h(4 * n2);
-        for (int s = 0; s < nz; s++) {
-            HIP_TRY(hipMemcpyAsync(h.data(), snaps + (size_t)s * n2, n2 * sizeof(float4), hipMemcpyDeviceToHost,
-                                   c->stream));
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            float *fh = fh_steps + 2 * (size_t)s * n2, *sh = sh_steps + 2 * (size_t)s * n2;
-            for (size_t i = 0; i < n2; i++) {
-                fh[2 * i] = h[4 * i];
-                fh[2 * i + 1] = h[4 * i + 1];
-                sh[2 * i] = h[4 * i + 2];
-                sh[2 * i + 1] = h[4 * i + 3];
+        float4 *snaps = nullptr;
+        if (hipMalloc(&snaps, (size_t)nz * n2 * sizeof(float4)) != hipSuccess)
+            return fail(SHG_E_NOMEM, "shg_forward_snapshots: device allocation failed");
+        auto run = [&]() -> int {
+            HIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(snaps, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+            if (c->p.thermal) {
+                HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)n2 * c->hz, c->tc, c->stream));
+                RC_TRY(enqueue_march_thermal(c, snaps));
+                HIP_TRY(hipMemcpyAsync(c->Uout, snaps + (size_t)(nz - 1) * n2, n2 * sizeof(float4),
+                                       hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                RC_TRY(enqueue_march_plain(c, 0, c->steps, snaps));
             }
-        }
-        return SHG_OK;
-    };
-    const int rc = run();
-    (void)hipFree(snaps);
-    return rc;
+            std::vector<float> h(4 * n2);
+            for (int s = 0; s < nz; s++) {
+                HIP_TRY(hipMemcpyAsync(h.data(), snaps + (size_t)s * n2, n2 * sizeof(float4), hipMemcpyDeviceToHost,
+                                       c->stream));
+                HIP_TRY(hipStreamSynchronize(c->stream));
+                float *fh = fh_steps + 2 * (size_t)s * n2, *sh = sh_steps + 2 * (size_t)s * n2;
+                for (size_t i = 0; i < n2; i++) {
+                    fh[2 * i] = h[4 * i];
+                    fh[2 * i + 1] = h[4 * i + 1];
+                    sh[2 * i] = h[4 * i + 2];
+                    sh[2 * i + 1] = h[4 * i + 3];
+                }
+            }
+            return SHG_OK;
+        };
+        const int rc = run();
+        (void)hipFree(snaps);
+        return rc;
+    });
 }
 
 int shg_get_fields(shg_ctx *c, float *fh, float *sh) {
-    if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_get_fields: null argument");
-    const size_t ncell = (size_t)c->n * c->n;
-    float *h = c->host_stage.data();
-    const float4 *src = c->p.thermal ? c->Uout : c->U;
-    HIP_TRY(hipMemcpyAsync(h, src, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < ncell; i++) {
-        fh[2 * i] = h[4 * i];
-        fh[2 * i + 1] = h[4 * i + 1];
-        sh[2 * i] = h[4 * i + 2];
-        sh[2 * i + 1] = h[4 * i + 3];
-    }
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !fh || !sh) return fail(SHG_E_INVALID, "shg_get_fields: null argument");
+        const size_t ncell = (size_t)c->n * c->n;
+        float *h = c->host_stage.data();
+        const float4 *src = c->p.thermal ? c->Uout : c->U;
+        HIP_TRY(hipMemcpyAsync(h, src, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < ncell; i++) {
+            fh[2 * i] = h[4 * i];
+            fh[2 * i + 1] = h[4 * i + 1];
+            sh[2 * i] = h[4 * i + 2];
+            sh[2 * i + 1] = h[4 * i + 3];
+        }
+        return SHG_OK;
+    });
 }
 
 int shg_powers(shg_ctx *c, double *p_fh_in, double *p_sh_out, double *eta) {
-    if (!c) return fail(SHG_E_INVALID, "shg_powers: null context");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_powers: no fields");
-    double sf, ss;
-    RC_TRY(reduce_powers(c, c->p.thermal ? c->Uout : c->U, &sf, &ss));
-    if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_powers: non-finite field");
-    const double psh = power_of(c, 1, ss, c->t_centre);
-    if (p_fh_in) *p_fh_in = c->p_fh_in;
-    if (p_sh_out) *p_sh_out = psh;
-    if (eta) *eta = c->p_fh_in > 0 ? psh / c->p_fh_in : 0.0;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_powers: null context");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_powers: no fields");
+        double sf, ss;
+        RC_TRY(reduce_powers(c, c->p.thermal ? c->Uout : c->U, &sf, &ss));
+        if (!std::isfinite(sf) || !std::isfinite(ss)) return fail(SHG_E_NONFINITE, "shg_powers: non-finite field");
+        const double psh = power_of(c, 1, ss, c->t_centre);
+        if (p_fh_in) *p_fh_in = c->p_fh_in;
+        if (p_sh_out) *p_sh_out = psh;
+        if (eta) *eta = c->p_fh_in > 0 ? psh / c->p_fh_in : 0.0;
+        return SHG_OK;
+    });
 }
 
 int shg_last_forward_ms(shg_ctx *c, double *ms) {
-    if (!c || !ms) return fail(SHG_E_INVALID, "shg_last_forward_ms: null argument");
-    *ms = c->last_ms;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !ms) return fail(SHG_E_INVALID, "shg_last_forward_ms: null argument");
+        *ms = c->last_ms;
+        return SHG_OK;
+    });
 }
 
 int64_t shg_cell_steps(const shg_ctx *c) { return c ? (int64_t)c->n * c->n * c->steps : 0; }
 
 int shg_tune(shg_ctx *c, int reps, int *row_variant, int *col_variant) {
-    if (!c || reps <= 0) return fail(SHG_E_INVALID, "shg_tune: bad argument");
-    RC_TRY(tune_variants(c, reps));
-    if (c->have_fields)
-        HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (row_variant) *row_variant = c->row_v;
-    if (col_variant) *col_variant = c->col_v;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || reps <= 0) return fail(SHG_E_INVALID, "shg_tune: bad argument");
+        RC_TRY(tune_variants(c, reps));
+        if (c->have_fields)
+            HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (row_variant) *row_variant = c->row_v;
+        if (col_variant) *col_variant = c->col_v;
+        return SHG_OK;
+    });
 }
 
 int shg_set_variants(shg_ctx *c, int row_variant, int col_variant) {
-    const int nrow = c && c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
-    if (!c || row_variant < 0 || row_variant >= nrow || col_variant < 0 || col_variant >= shg::kColVariantCount)
-        return fail(SHG_E_INVALID, "shg_set_variants: variant out of range");
-    if (c->gexec && (row_variant != c->row_v || col_variant != c->col_v)) {
-        (void)hipGraphExecDestroy(c->gexec);
-        c->gexec = nullptr;
-    }
-    c->row_v = row_variant;
-    c->col_v = col_variant;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        const int nrow = c && c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
+        if (!c || row_variant < 0 || row_variant >= nrow || col_variant < 0 || col_variant >= shg::kColVariantCount)
+            return fail(SHG_E_INVALID, "shg_set_variants: variant out of range");
+        if (c->gexec && (row_variant != c->row_v || col_variant != c->col_v)) {
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+        c->row_v = row_variant;
+        c->col_v = col_variant;
+        return SHG_OK;
+    });
 }
 
 int shg_get_variants(const shg_ctx *c, int *row_variant, int *col_variant) {
-    if (!c || !row_variant || !col_variant) return fail(SHG_E_INVALID, "shg_get_variants: bad argument");
-    *row_variant = c->row_v;
-    *col_variant = c->col_v;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !row_variant || !col_variant) return fail(SHG_E_INVALID, "shg_get_variants: bad argument");
+        *row_variant = c->row_v;
+        *col_variant = c->col_v;
+        return SHG_OK;
+    });
 }
 
 int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
-    if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
-    if (c->p.thermal)
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
-    const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
-    const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
-    const shg::ThermRowArgs a = c->p.thermal ? therm_mid_args(c) : shg::ThermRowArgs{};
-    auto check = [](hipError_t e) {
-        return e == hipSuccess ? (int)SHG_OK : fail(SHG_E_DEVICE, std::string("profiled pass: ") + hipGetErrorString(e));
-    };
-    float ms = 0.f;
-    RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
-        return check(c->p.thermal ? shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint,
-                                                                 a, c->stream, rv)
-                                  : shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv));
-    }, &ms));
-    *row_ms = ms;
-    RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
-        return check(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
-    }, &ms));
-    *col_ms = ms;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
+        if (c->p.thermal)
+            HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
+        const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+        const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
+        const shg::ThermRowArgs a = c->p.thermal ? therm_mid_args(c) : shg::ThermRowArgs{};
+        auto check = [](hipError_t e) {
+            return e == hipSuccess ? (int)SHG_OK : fail(SHG_E_DEVICE, std::string("profiled pass: ") + hipGetErrorString(e));
+        };
+        float ms = 0.f;
+        RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
+            return check(c->p.thermal ? shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, c->ntab, c->Q, c->DKint,
+                                                                     a, c->stream, rv)
+                                      : shg::launch_row_pass(c->n, c->d_desc, 1, c->tw, 0, mode, c->stream, rv));
+        }, &ms));
+        *row_ms = ms;
+        RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, reps, [&] {
+            return check(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, cv));
+        }, &ms));
+        *col_ms = ms;
+        return SHG_OK;
+    });
 }
 
 // ----- THERMAL --------------------------------------------------------------
 
 int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st) {
-    if (!c) return fail(SHG_E_INVALID, "shg_thermal_run: null context");
-    if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_thermal_run: context was created with thermal = 0");
-    if (!c->have_fields) return fail(SHG_E_STATE, "shg_thermal_run: call shg_set_fields first (SH input)");
-    const shg::phys::Consts k(c->p);
-    const size_t ncell = (size_t)c->n * c->n, n3 = ncell * c->hz;
-    const int check = c->p.jacobi_check > 0 ? c->p.jacobi_check : 5000;
-    const int num_of_iter = c->p.max_outer > 0 ? c->p.max_outer : 5;
-    const double tol = c->p.jacobi_tol > 0 ? (double)(float)c->p.jacobi_tol : (double)(float)(5e-4 / (double)n3);
-    RC_TRY(thermal_init_temperature(c));
-    // the caller's z = 0 fields (SH input; FH used when pump_from_temperature == 0)
-    std::vector<float> fh0(2 * ncell), sh0(2 * ncell), stage(4 * ncell);
-    HIP_TRY(hipMemcpyAsync(stage.data(), c->U0, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < ncell; i++) {
-        fh0[2 * i] = stage[4 * i]; fh0[2 * i + 1] = stage[4 * i + 1];
-        sh0[2 * i] = stage[4 * i + 2]; sh0[2 * i + 1] = stage[4 * i + 3];
-    }
-    shg_thermal_stats s{};
-    double jac_ms = 0.0, march_ms = 0.0;
-    int global_count = 0;
-    while (global_count < num_of_iter) {
-        // Jacobi to steady state, residual checked every `check` sweeps (src/headers/Solver.h:302-325)
-        double reduced = 1.0, aux = 0.0;
-        int64_t counter = 0;
-        float ms = 0.f;
-        HIP_TRY(hipEventRecord(c->e0, c->stream));
-        while (reduced >= tol) {
-            if (counter % check == 0) {
-                double r = 0.0;
-                RC_TRY(jacobi_run(c, 1, true, &r));
-                if (counter > 0) aux = reduced;
-                reduced = r;
-                if (counter > 0 && reduced == aux) reduced = 0.5 * tol;
-                s.residual_last = r;
-                counter++;
-            } else {
-                const int64_t next_check = (counter / check + 1) * (int64_t)check;
-                int64_t burst = next_check - counter;
-                if (c->p.jacobi_max > 0) burst = std::min<int64_t>(burst, c->p.jacobi_max - counter);
-                RC_TRY(jacobi_run(c, burst, false, nullptr));
-                counter += burst;
-            }
-            if (c->p.jacobi_max > 0 && counter >= c->p.jacobi_max) break;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_thermal_run: null context");
+        if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_thermal_run: context was created with thermal = 0");
+        if (!c->have_fields) return fail(SHG_E_STATE, "shg_thermal_run: call shg_set_fields first (SH input)");
+        const shg::phys::Consts k(c->p);
+        const size_t ncell = (size_t)c->n * c->n, n3 = ncell * c->hz;
+        const int check = c->p.jacobi_check > 0 ? c->p.jacobi_check : 5000;
+        const int num_of_iter = c->p.max_outer > 0 ? c->p.max_outer : 5;
+        const double tol = c->p.jacobi_tol > 0 ? (double)(float)c->p.jacobi_tol : (double)(float)(5e-4 / (double)n3);
+        RC_TRY(thermal_init_temperature(c));
+        // the caller's z = 0 fields (SH input; FH used when pump_from_temperature == 0)
+        std::vector<float> fh0(2 * ncell), sh0(2 * ncell), stage(4 * ncell);
+        HIP_TRY(hipMemcpyAsync(stage.data(), c->U0, ncell * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < ncell; i++) {
+            fh0[2 * i] = stage[4 * i]; fh0[2 * i + 1] = stage[4 * i + 1];
+            sh0[2 * i] = stage[4 * i + 2]; sh0[2 * i + 1] = stage[4 * i + 3];
         }
-        HIP_TRY(hipEventRecord(c->e1, c->stream));
-        HIP_TRY(hipEventSynchronize(c->e1));
-        HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
-        jac_ms += ms;
-        s.sweeps_last = counter;
-        s.sweeps_total += counter;
-        // fields and propagators at the temperature of the grid centre, slice 0 (Solver.h:329-331)
-        float tcen = 0.f;
-        HIP_TRY(shg_internal_copy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
-                                  hipMemcpyDeviceToHost, c->stream));
-        if (pump_from_temperature)
-            shg::phys::make_pump(k, (float)c->p.power, (float)c->p.focus, (float)c->p.waist, tcen, fh0.data());
-        RC_TRY(set_u_from_host(c, c->U, fh0.data(), sh0.data()));
-        RC_TRY(upload_propagators(c, tcen));
-        RC_TRY(timed_march(c));
-        march_ms += c->last_ms;
-        s.outer_iterations++;
-        if (counter == 1 && global_count != 0) global_count = num_of_iter;   // (Solver.h:336)
-        global_count++;
-    }
-    // input power of the last outer iteration's pump
-    RC_TRY(set_u_from_host(c, c->U0, fh0.data(), sh0.data()));
-    double sf, ss;
-    RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
-    c->p_fh_in = power_of(c, 0, sf, c->t_centre);
-    s.jacobi_ms = jac_ms;
-    s.march_ms = march_ms;
-    if (st) *st = s;
-    return SHG_OK;
+        shg_thermal_stats s{};
+        double jac_ms = 0.0, march_ms = 0.0;
+        int global_count = 0;
+        while (global_count < num_of_iter) {
+            // Jacobi to steady state, residual checked every `check` sweeps (src/headers/Solver.h:302-325)
+            double reduced = 1.0, aux = 0.0;
+            int64_t counter = 0;
+            float ms = 0.f;
+            HIP_TRY(hipEventRecord(c->e0, c->stream));
+            while (reduced >= tol) {
+                if (counter % check == 0) {
+                    double r = 0.0;
+                    RC_TRY(jacobi_run(c, 1, true, &r));
+                    if (counter > 0) aux = reduced;
+                    reduced = r;
+                    if (counter > 0 && reduced == aux) reduced = 0.5 * tol;
+                    s.residual_last = r;
+                    counter++;
+                } else {
+                    const int64_t next_check = (counter / check + 1) * (int64_t)check;
+                    int64_t burst = next_check - counter;
+                    if (c->p.jacobi_max > 0) burst = std::min<int64_t>(burst, c->p.jacobi_max - counter);
+                    RC_TRY(jacobi_run(c, burst, false, nullptr));
+                    counter += burst;
+                }
+                if (c->p.jacobi_max > 0 && counter >= c->p.jacobi_max) break;
+            }
+            HIP_TRY(hipEventRecord(c->e1, c->stream));
+            HIP_TRY(hipEventSynchronize(c->e1));
+            HIP_TRY(hipEventElapsedTime(&ms, c->e0, c->e1));
+            jac_ms += ms;
+            s.sweeps_last = counter;
+            s.sweeps_total += counter;
+            // fields and propagators at the temperature of the grid centre, slice 0 (Solver.h:329-331)
+            float tcen = 0.f;
+            HIP_TRY(shg_internal_copy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
+                                      hipMemcpyDeviceToHost, c->stream));
+            if (pump_from_temperature)
+                shg::phys::make_pump(k, (float)c->p.power, (float)c->p.focus, (float)c->p.waist, tcen, fh0.data());
+            RC_TRY(set_u_from_host(c, c->U, fh0.data(), sh0.data()));
+            RC_TRY(upload_propagators(c, tcen));
+            RC_TRY(timed_march(c));
+            march_ms += c->last_ms;
+            s.outer_iterations++;
+            if (counter == 1 && global_count != 0) global_count = num_of_iter;   // (Solver.h:336)
+            global_count++;
+        }
+        // input power of the last outer iteration's pump
+        RC_TRY(set_u_from_host(c, c->U0, fh0.data(), sh0.data()));
+        double sf, ss;
+        RC_TRY(reduce_powers(c, c->U0, &sf, &ss));
+        c->p_fh_in = power_of(c, 0, sf, c->t_centre);
+        s.jacobi_ms = jac_ms;
+        s.march_ms = march_ms;
+        if (st) *st = s;
+        return SHG_OK;
+    });
 }
 
 int shg_jacobi(shg_ctx *c, int64_t nsweeps, double *mean_sq_update) {
-    if (!c || nsweeps < 0) return fail(SHG_E_INVALID, "shg_jacobi: bad argument");
-    if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_jacobi: context was created with thermal = 0");
-    double r = 0.0;
-    RC_TRY(jacobi_run(c, nsweeps, mean_sq_update != nullptr, &r));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (mean_sq_update) *mean_sq_update = r;
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || nsweeps < 0) return fail(SHG_E_INVALID, "shg_jacobi: bad argument");
+        if (!c->p.thermal) return fail(SHG_E_INVALID, "shg_jacobi: context was created with thermal = 0");
+        double r = 0.0;
+        RC_TRY(jacobi_run(c, nsweeps, mean_sq_update != nullptr, &r));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (mean_sq_update) *mean_sq_update = r;
+        return SHG_OK;
+    });
 }
 
 int shg_get_temperature(shg_ctx *c, float *T) {
-    if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_temperature: bad argument");
-    HIP_TRY(shg_internal_copy(T, c->T[c->tcur], (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost,
-                              c->stream));
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_temperature: bad argument");
+        HIP_TRY(shg_internal_copy(T, c->T[c->tcur], (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->stream));
+        return SHG_OK;
+    });
 }
 
 int shg_set_temperature(shg_ctx *c, const float *T) {
-    if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_temperature: bad argument");
-    const size_t bytes = (size_t)c->n * c->n * c->hz * sizeof(float);
-    HIP_TRY(shg_internal_copy(c->T[0], T, bytes, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(shg_internal_copy(c->T[1], T, bytes, hipMemcpyHostToDevice, c->stream));
-    c->tcur = 0;
-    return upload_propagators(c, T[(size_t)(c->n / 2) * c->n + c->n / 2]);
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !T || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_temperature: bad argument");
+        const size_t bytes = (size_t)c->n * c->n * c->hz * sizeof(float);
+        HIP_TRY(shg_internal_copy(c->T[0], T, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(shg_internal_copy(c->T[1], T, bytes, hipMemcpyHostToDevice, c->stream));
+        c->tcur = 0;
+        return upload_propagators(c, T[(size_t)(c->n / 2) * c->n + c->n / 2]);
+    });
 }
 
 int shg_get_heat_source(shg_ctx *c, float *Q) {
-    if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_heat_source: bad argument");
-    HIP_TRY(shg_internal_copy(Q, c->Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_get_heat_source: bad argument");
+        HIP_TRY(shg_internal_copy(Q, c->Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        return SHG_OK;
+    });
 }
 
 int shg_set_heat_source(shg_ctx *c, const float *Q) {
-    if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_heat_source: bad argument");
-    HIP_TRY(shg_internal_copy(c->Q, Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    return SHG_OK;
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !Q || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_heat_source: bad argument");
+        HIP_TRY(shg_internal_copy(c->Q, Q, (size_t)c->n * c->n * c->hz * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        return SHG_OK;
+    });
 }
 
 }  // extern "C"

@@ -127,3 +127,25 @@ def test_phase_sincos_within_two_ulp(tmp_path):
         out = subprocess.run([str(exe), str(xmax), "300000", "7"], check=True, capture_output=True, text=True).stdout
         max_abs, max_ulp = map(float, out.split())
         assert max_ulp <= 2.0 and max_abs <= 1.5e-7, (xmax, max_abs, max_ulp)
+
+
+def test_entry_point_guard_turns_exceptions_into_codes(tmp_path):
+    # every int entry point runs its body through shg_internal_guard (csrc/guard.h)
+    exe = tmp_path / "guard_emul"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", str(exe), os.path.join(ROOT, "tests", "cpu_emul", "guard_emul.cpp")],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out[0] == "-3 entry_a: host allocation failed"
+    assert out[1] == "-7 entry_b: vector"
+    assert out[2] == "-7 entry_c: unexpected error"
+    assert out[3] == "-4"
+    # and every exported int entry point of the library is wrapped
+    import re
+    wrapped = 0
+    for name in ("shg_capi.cpp", "shg_batch.cpp", "shg_adjoint.cpp"):
+        src = open(os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "csrc", name)).read()
+        for m in re.finditer(r"^int (shg_[a-z_0-9]+)\(.*\) \{\n(.*)\n", src, re.M):
+            if not m.group(1).startswith("shg_internal"):
+                assert "shg_internal_guard(__func__" in m.group(2), (name, m.group(1))
+                wrapped += 1
+    assert wrapped >= 31, wrapped
