@@ -37,9 +37,10 @@ enum RowMode : int {
     ROW_AUTO = 3     // ROW_MID if s + 1 < steps of the instance, else ROW_END
 };
 
-// `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..4, columns 0..7;
-// row 4 and columns 5..7 are the split layouts).
-constexpr int kRowVariantCount = 5, kColVariantCount = 8;
+// `variant` selects a thread layout (pass_cfg.h; 0 = default, rows 0..4, columns 0..9;
+// row 4 and columns 5..7 are the split layouts through LDS, columns 8..9 the field-split
+// register layouts).
+constexpr int kRowVariantCount = 5, kColVariantCount = 10;
 // A set `snap` (one instance only) receives the real-space state after step `step` (the
 // fields after its second RK4) from a MID or END pass: the checkpoints of the adjoint.
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,

@@ -118,6 +118,43 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
         U[(size_t)(t + j * T) * N + x] = make_float4(fa[j].x, fa[j].y, fb[j].x, fb[j].y);
 }
 
+// Column pass with the two fields on separate waves, straight from registers: the
+// diffraction does not couple FH and SH, so each field of CW columns is owned by its own
+// CW*T threads (field 0 the first half of the workgroup, field 1 the second), each
+// holding E elements of one column; the halves run their transforms side by side (half
+// the dependent chain of k_col_pass per thread, twice the waves) and meet only at the
+// barriers.  Same operations as k_col_pass, so the same result bit for bit.
+template <int N, int E, int CW>
+__global__ __launch_bounds__(2 * CW *(N / E)) void k_col_fsplit(const InstDesc *__restrict__ descs,
+                                                                const float2 *__restrict__ twg) {
+    using C = ColCfgT<N, E, CW>;
+    constexpr int T = C::T, HALF = C::THREADS;
+    static_assert(HALF % 64 == 0, "each field on whole waves");
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[blockIdx.y];
+    const auto U = gp(reinterpret_cast<float2 *>(d.U));     // cell i, field f at 2 i + f
+    const auto G = gp(d.G);
+    const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF;
+    const int c = r % CW, t = r / CW;
+    const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
+    __syncthreads();
+    float2 a[E], gy[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    const float2 gx = G[2 * f * N + x];                     // Gx of the field, then its Gy
+#pragma unroll
+    for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];
+    auto sync = [] { block_sync(); };
+    fft1fe<N, E, +1>(a, &lds[f][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) a[j] = c_mul(c_mul(gx, gy[j]), a[j]);
+    fft1fe<N, E, -1>(a, &lds[f][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) U[2 * ((size_t)(t + j * T) * N + x) + f] = a[j];
+}
+
 // ---------------------------------------------------------------------------
 // Split layouts: the workgroup loads its cells one per thread (coalesced, spread over all
 // its waves) into LDS in natural order; each 1D transform of one field of one line runs
@@ -312,10 +349,18 @@ static hipError_t col_split_launch(const InstDesc *d, int ninst, const float2 *t
     return hipGetLastError();
 }
 
+template <int N, int E, int CW>
+static hipError_t col_fsplit_launch(const InstDesc *d, int ninst, const float2 *tw, hipStream_t st) {
+    using C = ColCfgT<N, E, CW>;
+    hipLaunchKernelGGL((k_col_fsplit<N, E, CW>), dim3(N / CW, ninst), dim3(2 * C::THREADS), 0, st, d, tw);
+    return hipGetLastError();
+}
+
 #define ROWV(N, E, W) return row_launch<N, E, W>(d, ninst, tw, step, mode, st, snap)
 #define COLV(N, E, CW) return col_launch<N, E, CW>(d, ninst, tw, st)
 #define ROWS(N, E, W) return row_split_launch<N, E, W>(d, ninst, tw, step, mode, st, snap)
 #define COLS(N, E, CW) return col_split_launch<N, E, CW>(d, ninst, tw, st)
+#define COLF(N, E, CW) return col_fsplit_launch<N, E, CW>(d, ninst, tw, st)
 
 hipError_t launch_row_pass(int n, const InstDesc *d, int ninst, const float2 *tw, int step, int mode, hipStream_t st,
                            int variant, float4 *snap) {
@@ -357,6 +402,8 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 64 * 16 + 5: COLS(64, 4, 1);
         case 64 * 16 + 6: COLS(64, 4, 2);
         case 64 * 16 + 7: COLS(64, 4, 4);
+        case 64 * 16 + 8: COLF(64, 4, 4);
+        case 64 * 16 + 9: COLF(64, 4, 8);
         case 128 * 16 + 0: COLV(128, 16, 8);
         case 128 * 16 + 1: COLV(128, 8, 8);
         case 128 * 16 + 2: COLV(128, 4, 8);
@@ -365,6 +412,8 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 128 * 16 + 5: COLS(128, 4, 1);
         case 128 * 16 + 6: COLS(128, 4, 2);
         case 128 * 16 + 7: COLS(128, 4, 4);
+        case 128 * 16 + 8: COLF(128, 4, 2);
+        case 128 * 16 + 9: COLF(128, 4, 4);
         case 256 * 16 + 0: COLV(256, 16, 8);
         case 256 * 16 + 1: COLV(256, 8, 8);
         case 256 * 16 + 2: COLV(256, 4, 8);
@@ -373,6 +422,8 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 256 * 16 + 5: COLS(256, 4, 1);
         case 256 * 16 + 6: COLS(256, 4, 2);
         case 256 * 16 + 7: COLS(256, 4, 4);
+        case 256 * 16 + 8: COLF(256, 4, 1);
+        case 256 * 16 + 9: COLF(256, 4, 2);
         case 512 * 16 + 0: COLV(512, 8, 4);
         case 512 * 16 + 1: COLV(512, 16, 4);
         case 512 * 16 + 2: COLV(512, 4, 4);
@@ -381,6 +432,8 @@ hipError_t launch_col_pass(int n, const InstDesc *d, int ninst, const float2 *tw
         case 512 * 16 + 5: COLS(512, 8, 1);
         case 512 * 16 + 6: COLS(512, 8, 2);
         case 512 * 16 + 7: COLS(512, 8, 4);
+        case 512 * 16 + 8: COLF(512, 8, 1);
+        case 512 * 16 + 9: COLF(512, 8, 2);
         default: return hipErrorInvalidValue;
     }
 }

@@ -107,7 +107,7 @@ def test_every_layout_variant_matches_oracle32(gpu, n):
     with gpu.Context(p) as ctx:
         ctx.set_fields(fh0, sh0)
         for rv in range(5):
-            for cv in range(8):
+            for cv in range(10):
                 ctx.set_variants(rv, cv)
                 ctx.forward()
                 fh, sh = ctx.get_fields()
@@ -115,6 +115,27 @@ def test_every_layout_variant_matches_oracle32(gpu, n):
                 assert max(e) <= TOL_FIELD, (rv, cv, e)
         tuned = ctx.tune(reps=3)
         print(f"\n  n={n} tuned variants {tuned}")
+
+
+# column layouts with the same elements per thread and columns per workgroup, both fields
+# per thread (k_col_pass) and one field per wave (k_col_fsplit): the same operations
+FSPLIT_TWINS = {64: [(3, 8), (1, 9)], 128: [(3, 8), (4, 9)], 256: [(3, 8), (4, 9)], 512: [(3, 8), (4, 9)]}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_field_split_column_layouts_equal_their_twins(gpu, n):
+    p = plain128_params(nx=n, ny=n, nz=12, lx=4.0 * n, ly=4.0 * n)
+    fh0, sh0 = _inputs(p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(fh0, sh0)
+        for cv_pass, cv_split in FSPLIT_TWINS[n]:
+            out = []
+            for cv in (cv_pass, cv_split):
+                ctx.set_variants(0, cv)
+                ctx.forward()
+                out.append(ctx.get_fields())
+            assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), (n, cv_pass, cv_split)
 
 
 @pytest.mark.gpu

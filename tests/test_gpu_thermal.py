@@ -141,7 +141,7 @@ def test_thermal_every_layout_variant_matches_oracle(gpu, n):
     rfh, rsh, rQ = orc.thermal_cwes(p, T, fh0, sh0)
     with gpu.Context(p) as ctx:
         ctx.set_temperature(T)
-        for rv, cv in [(r, 0) for r in range(7)] + [(6, c) for c in range(1, 8)]:
+        for rv, cv in [(r, 0) for r in range(7)] + [(6, c) for c in range(1, 10)]:
             ctx.set_variants(rv, cv)
             ctx.set_fields(fh0, sh0)
             ctx.forward()
