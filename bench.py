@@ -37,14 +37,14 @@ THERM_ROW_256 = {0: "k_therm_row_pass<256,16,1>", 1: "k_therm_row_pass<256,8,1>"
                  6: "k_therm_row_split<256,4,4>"}
 COL_256 = {0: "k_col_pass<256,16,8>", 1: "k_col_pass<256,8,8>", 2: "k_col_pass<256,4,8>", 3: "k_col_pass<256,4,1>",
            4: "k_col_pass<256,4,2>", 5: "k_col_split<256,4,1>", 6: "k_col_split<256,4,2>",
-           7: "k_col_split<256,4,4>"}
+           7: "k_col_split<256,4,4>", 8: "k_col_fsplit<256,4,1>", 9: "k_col_fsplit<256,4,2>"}
 
 METRIC = "SSFM cell-steps/s (plain, batched, fwd+adjoint) on 1 GPU; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256, the tuned layouts) from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy
 # of known size.
-TRAFFIC_THERMAL_STEP = 6905 * 1024
+TRAFFIC_THERMAL_STEP = 6882 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 # VALU lane-operations per plain cell-step of the batch's passes (rocprofv3 SQ_INSTS_VALU
 # x 64 per cell: 393 row pass + 141 column pass at 512^2) and the chip's FP32 issue rate
