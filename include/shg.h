@@ -214,6 +214,13 @@ typedef struct shg_adjoint_result {
  * temperature of slice 0, as the forward evaluates them). */
 int shg_adjoint(shg_ctx *ctx, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res);
 
+/* The field part of shg_adjoint only (no reference counterpart): eta and d eta / d FH(z=0)
+ * into `d_eta_d_fh` (required) — eta exactly, the gradient to rounding (~1e-6) of
+ * shg_adjoint's — without forming the focus and temperature sensitivities (nor, THERMAL, the dJ/dT map, whose
+ * shg_adjoint_temperature_map is then unavailable).  For callers that need the field
+ * gradient alone, such as gradient ascent of a pump phase mask. */
+int shg_adjoint_field(shg_ctx *ctx, int checkpoint_interval, float *d_eta_d_fh, double *eta);
+
 /* After shg_adjoint on a THERMAL context (no reference counterpart): d eta / d T of every
  * heat voxel (nx*ny*heat_nz doubles, IDX layout) with the rest of the field held fixed,
  * crystal path only (coupling, phase mismatch through DKint, TPA; the propagator and

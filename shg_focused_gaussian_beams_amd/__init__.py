@@ -86,6 +86,7 @@ def load_library(path=LIB_PATH):
         "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_set_heat_source": (ctypes.c_int, [vp, fp]),
         "shg_adjoint": (ctypes.c_int, [vp, ctypes.c_int, fp, ctypes.POINTER(AdjointResult)]),
+        "shg_adjoint_field": (ctypes.c_int, [vp, ctypes.c_int, fp, dp]),
         "shg_adjoint_temperature_map": (ctypes.c_int, [vp, dp]),
         "shg_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
@@ -227,6 +228,14 @@ class Context:
         g = np.empty((self.params.ny, self.params.nx), np.complex64) if want_field else None
         _check(self.lib.shg_adjoint(self.h, checkpoint_interval, _fptr(g) if want_field else None, ctypes.byref(r)))
         return r.as_dict(), g
+
+    def adjoint_field(self, checkpoint_interval=0):
+        """eta and d eta / d FH(z=0) only (shg_adjoint_field): the field gradient of
+        adjoint() without its focus and temperature sensitivities."""
+        g = np.empty((self.params.ny, self.params.nx), np.complex64)
+        eta = ctypes.c_double()
+        _check(self.lib.shg_adjoint_field(self.h, checkpoint_interval, _fptr(g), ctypes.byref(eta)))
+        return eta.value, g
 
     def adjoint_temperature_map(self):
         """After adjoint() on a THERMAL context: d eta / d T per heat voxel (heat_shape())."""

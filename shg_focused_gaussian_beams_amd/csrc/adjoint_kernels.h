@@ -23,10 +23,11 @@ enum AdjMode : int { ADJ_START = 0, ADJ_MID = 1, ADJ_END = 2 };
 hipError_t launch_row_tape(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
                            hipStream_t st);
 hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, hipStream_t st);
+// sens = false: lambda only (the field gradient); g1, g2 and gH are not written.
 hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
-                          double *g1, double *g2, hipStream_t st);
+                          double *g1, double *g2, hipStream_t st, bool sens = true);
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
-                          float a_p, float a_s, double *gH, hipStream_t st);
+                          float a_p, float a_s, double *gH, hipStream_t st, bool sens = true);
 hipError_t launch_adj_seed(const float4 *U, float4 *L, int64_t n, float scale, hipStream_t st);
 // d eta / d A0 = lambda_FH + direct * A0_FH with direct = -2 cp J / P_in and
 // J = cs * red->y / P_in (red: the FP64 output power sums of the forward), written to

@@ -167,8 +167,9 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
 // ADJ_END   at s: R_fwd, RK4_s^T at A_s = tp.a (the segment's checkpoint); leaves the
 //                 real-space lambda of the segment start.
 // g1[(s*N + row)*5] gets the row's sensitivities of the first RK4 of step s (at A_s),
-// g2[...] those of the second (at A''_s).
-template <int N>
+// g2[...] those of the second (at A''_s).  SENS = false propagates lambda only (the
+// field gradient): the sensitivity sums are then never formed.
+template <int N, bool SENS>
 __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *__restrict__ descs,
                                                                const float2 *__restrict__ twg, int step, int mode,
                                                                TapeArgs tp, double *__restrict__ g1,
@@ -216,11 +217,13 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
         lds[0][o] = P;
         lds[1][o] = S;
     }
-    if (mode != ADJ_START) block_store<W>(acc1, g1 + ((size_t)step * N + blockIdx.x) * 5);
-    if (mode == ADJ_START) block_store<W>(acc2, g2 + ((size_t)step * N + blockIdx.x) * 5);
-    else if (mode == ADJ_MID) {
-        __syncthreads();                      // block_store's LDS slots are reused
-        block_store<W>(acc2, g2 + ((size_t)(step - 1) * N + blockIdx.x) * 5);
+    if constexpr (SENS) {
+        if (mode != ADJ_START) block_store<W>(acc1, g1 + ((size_t)step * N + blockIdx.x) * 5);
+        if (mode == ADJ_START) block_store<W>(acc2, g2 + ((size_t)step * N + blockIdx.x) * 5);
+        else if (mode == ADJ_MID) {
+            __syncthreads();                  // block_store's LDS slots are reused
+            block_store<W>(acc2, g2 + ((size_t)(step - 1) * N + blockIdx.x) * 5);
+        }
     }
     __syncthreads();
     if (mode != ADJ_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
@@ -233,8 +236,9 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_adj_row(const InstDesc *
 
 // Adjoint column pass: C_bwd, x conj(Gx Gy), C_fwd.  gH[step*nblk + blk] receives
 // sum Re(conj(lambda_hat) (Gx Gy) x_hat * i (a_p, a_s) (qx + qy)) over the block's cells,
-// the sensitivity of J to a temperature change of the propagator (d log H/dT = i a q).
-template <int N>
+// the sensitivity of J to a temperature change of the propagator (d log H/dT = i a q);
+// not with SENS = false.
+template <int N, bool SENS>
 __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc *__restrict__ descs,
                                                                 const float2 *__restrict__ twg, int step, TapeArgs tp,
                                                                 const float2 *__restrict__ qxy, float a_p, float a_s,
@@ -289,13 +293,15 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_adj_col(const InstDesc 
     if (lane < T) fft_lds_field<N, E, -1>(lds[k], lane, tw);
     __syncthreads();
     lds_to_cols<N, CW, THREADS>(gp(d.U), x0, lds);
-    for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
-    if ((tid & 63) == 0) red[tid >> 6] = gsum;
-    __syncthreads();
-    if (tid == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < THREADS / 64; w++) sum += red[w];
-        gH[(size_t)step * gridDim.x + blockIdx.x] = sum;
+    if constexpr (SENS) {
+        for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = gsum;
+        __syncthreads();
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < THREADS / 64; w++) sum += red[w];
+            gH[(size_t)step * gridDim.x + blockIdx.x] = sum;
+        }
     }
 }
 
@@ -399,17 +405,27 @@ hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step,
 }
 
 hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, int mode, const TapeArgs &tp,
-                          double *g1, double *g2, hipStream_t st) {
+                          double *g1, double *g2, hipStream_t st, bool sens) {
     if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_row<NN>, dim3(NN), dim3(64 * AdjWide<NN>::W), 0, st, d, tw, step, mode,
-                                       tp, g1, g2));
+    if (sens) {
+        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_row<NN, true>), dim3(NN), dim3(64 * AdjWide<NN>::W), 0, st, d, tw,
+                                           step, mode, tp, g1, g2));
+    } else {
+        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_row<NN, false>), dim3(NN), dim3(64 * AdjWide<NN>::W), 0, st, d, tw,
+                                           step, mode, tp, g1, g2));
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
-                          float a_p, float a_s, double *gH, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_adj_col<NN>, dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0, st, d,
-                                       tw, step, tp, qxy, a_p, a_s, gH));
+                          float a_p, float a_s, double *gH, hipStream_t st, bool sens) {
+    if (sens) {
+        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_col<NN, true>), dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0,
+                                           st, d, tw, step, tp, qxy, a_p, a_s, gH));
+    } else {
+        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_col<NN, false>), dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0,
+                                           st, d, tw, step, tp, qxy, a_p, a_s, gH));
+    }
     return hipGetLastError();
 }
 

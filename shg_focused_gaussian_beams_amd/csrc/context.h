@@ -26,7 +26,8 @@ struct AdjointBuffers {
     hipEvent_t ev_u = nullptr;                 // the forward state buffer is free for `ts`
     hipEvent_t ev_tape[2] = {nullptr, nullptr};  // tape slot k written
     hipEvent_t ev_free[2] = {nullptr, nullptr};  // tape slot k read by the adjoint
-    hipGraphExec_t g_adj = nullptr;            // captured adjoint march for interval k
+    hipGraphExec_t g_adj[2] = {nullptr, nullptr};   // captured adjoint march for interval k:
+                                                    // [0] with the sensitivities, [1] lambda only
     double2 *pump_df = nullptr, *pump_dn = nullptr;  // d(pump)/d(focus), d(pump)/d(n_p) per cell (device)
     double pump_T = -1e300;                    // temperature the pump derivative fields were built at
     float2 *grad = nullptr;                    // d eta / d FH(z = 0) per cell

@@ -109,7 +109,8 @@ int ensure_buffers(shg_ctx *c, int K, int nseg) {
         AHIP_TRY(shg_internal_copy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
     }
     if (b.k != K || b.nseg != nseg) {
-        if (b.g_adj) { (void)hipGraphExecDestroy(b.g_adj); b.g_adj = nullptr; }
+        for (hipGraphExec_t &g : b.g_adj)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
         for (float4 **ptr : {&b.ckpt, &b.a2, &b.spec})
             if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
         const int slots = nseg > 1 ? 2 : 1;
@@ -267,7 +268,8 @@ int enqueue_thermal_segment(shg_ctx *c, int s0, int s1, const shg::TapeArgs *tp,
 // current temperature field, which is held fixed: field, focus, and temperature (a
 // uniform shift of the field; per voxel in AdjointBuffers::Ap).  The heat source is not
 // touched.
-int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res) {
+// sens = false: the field gradient and eta only (no focus, temperature or dJ/dT map).
+int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res, bool sens) {
     const int M = c->p.nz - 1, n = c->n;
     const size_t n2 = (size_t)n * n, n3 = n2 * c->hz;
     int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
@@ -325,10 +327,12 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
     //    taped recomputation on `ts` (no host round trip after the forward: the seed
     //    scale needs P_in only, and eta is read back with the gradients)
     AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
-    for (double *ptr : {b.Ap, b.As})
-        AHIP_TRY(shg::launch_fill_zero(ptr, (int64_t)n3, st));
-    for (double *ptr : {b.Gc, b.Dc})
-        AHIP_TRY(shg::launch_fill_zero(ptr, (int64_t)n2, st));
+    if (sens) {
+        for (double *ptr : {b.Ap, b.As})
+            AHIP_TRY(shg::launch_fill_zero(ptr, (int64_t)n3, st));
+        for (double *ptr : {b.Gc, b.Dc})
+            AHIP_TRY(shg::launch_fill_zero(ptr, (int64_t)n2, st));
+    }
     if (nseg > 1) {
         AHIP_TRY(hipEventRecord(b.ev_u, st));
         AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
@@ -357,22 +361,24 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
         aa.dk = dk_tape(j);
         aa.s0 = s0;
         aa.cur = shg_internal_therm_step(c, s1 - 1);
-        AHIP_TRY(shg::launch_therm_adj_row(n, b.dlam, c->tw, shg::ADJ_START, aa, st));
+        AHIP_TRY(shg::launch_therm_adj_row(n, b.dlam, c->tw, shg::ADJ_START, aa, st, sens));
         for (int s = s1 - 1; s >= s0; s--) {
-            AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
+            AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st, sens));
             aa.cur = shg_internal_therm_step(c, s);
             if (s > s0) aa.prev = shg_internal_therm_step(c, s - 1);
-            AHIP_TRY(shg::launch_therm_adj_row(n, b.dlam, c->tw, s > s0 ? shg::ADJ_MID : shg::ADJ_END, aa, st));
+            AHIP_TRY(shg::launch_therm_adj_row(n, b.dlam, c->tw, s > s0 ? shg::ADJ_MID : shg::ADJ_END, aa, st, sens));
         }
         if (j >= 2) AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
     }
-    AHIP_TRY(shg::launch_adj_reduce(nullptr, nullptr, 0, b.gH, cb, M, b.gs, st));
-    AHIP_TRY(shg::launch_therm_dtmap(b.Ap, b.As, c->T[c->tcur], (int64_t)n3, c->p.crystal, (double)k.lp,
-                                     (double)k.ls, b.dpart, kMapParts, st));
-    AHIP_TRY(shg::launch_sum_partials(b.dpart, kMapParts, b.dsum, st));
+    if (sens) {
+        AHIP_TRY(shg::launch_adj_reduce(nullptr, nullptr, 0, b.gH, cb, M, b.gs, st));
+        AHIP_TRY(shg::launch_therm_dtmap(b.Ap, b.As, c->T[c->tcur], (int64_t)n3, c->p.crystal, (double)k.lp,
+                                         (double)k.ls, b.dpart, kMapParts, st));
+        AHIP_TRY(shg::launch_sum_partials(b.dpart, kMapParts, b.dsum, st));
+    }
     ARC_TRY(enqueue_gradient_results(c, cs, cp, d_eta_d_fh != nullptr, st));
     const HostOffsets o = host_offsets(n2, M);
-    AHIP_TRY(hipMemcpyAsync(host_at<double>(b, o.dsum), b.dsum, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (sens) AHIP_TRY(hipMemcpyAsync(host_at<double>(b, o.dsum), b.dsum, sizeof(double), hipMemcpyDeviceToHost, st));
     AHIP_TRY(hipStreamSynchronize(st));
     float fwd_ms = 0.f, adj_ms = 0.f;
     AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
@@ -380,24 +386,177 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
 
     // 3. host assembly (FP64)
     const double J = cs * host_at<double2>(b, o.sums)->y / pin;
+    copy_field_out(c, d_eta_d_fh);
+    res->eta = J;
+    res->forward_ms = fwd_ms;
+    res->adjoint_ms = adj_ms;
+    res->checkpoint_interval = K;
+    res->segments = nseg;
+    if (!sens) return SHG_OK;
     const double *gs = host_at<double>(b, o.gs);
     const double dT_map = *host_at<double>(b, o.dsum);
     const double d_focus = host_at<double2>(b, o.gsum)->x, dT_pump = host_at<double2>(b, o.gsum)->y * dnp;
-    copy_field_out(c, d_eta_d_fh);
     double dT_H = 0.0;
     for (int s = 0; s < M; s++) dT_H += gs[(size_t)s * 6 + 5];
     const double dT_powers = J * (dns / ns - dnp / np);
     b.map_centre_extra = dT_H + dT_powers;
     b.map_valid = true;
 
-    res->eta = J;
     res->d_eta_d_focus = d_focus;
     res->d_eta_d_temp_crystal = dT_map + dT_H + dT_powers;
     res->d_eta_d_temp = res->d_eta_d_temp_crystal + dT_pump;
+    return SHG_OK;
+}
+
+// The plain adjoint; sens = false: the field gradient and eta only.
+int plain_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res, bool sens) {
+    const int M = c->steps, n = c->n;
+    const size_t n2 = (size_t)n * n;
+    int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
+    K = std::max(1, std::min(K, M));
+    const int nseg = (M + K - 1) / K;
+    ARC_TRY(ensure_buffers(c, K, nseg));
+    AdjointBuffers &b = c->adj;
+    hipStream_t st = c->stream;
+    const int rb = shg::adj_row_parts(n), cb = shg::adj_col_parts(n);
+
+    // tape of segment j: slot j % 2, its start state is checkpoint j
+    auto tape_of = [&](int j, bool write_ckpt) {
+        const size_t off = (size_t)(nseg > 1 ? (j & 1) : 0) * K * n2;
+        return shg::TapeArgs{b.ckpt + (size_t)j * n2, b.a2 + off, b.spec + off,
+                             write_ckpt ? b.ckpt + (size_t)j * n2 : nullptr, j * K, 0};
+    };
+
+    // 1. forward: one uninterrupted march whose row passes at the segment boundaries
+    //    leave the checkpoints (the state after step jK - 1), the last segment taped, the
+    //    powers
+    auto enqueue_forward = [&]() -> int {
+        AHIP_TRY(shg::launch_copy_cells(c->U, c->U0, (int64_t)n2, st));
+        if (nseg == 1) {
+            ARC_TRY(enqueue_tape(c, tape_of(0, true), M, st));
+        } else {
+            const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
+            const int s_tape = (nseg - 1) * K;             // first step of the taped segment
+            AHIP_TRY(shg::launch_copy_cells(b.ckpt, c->U0, (int64_t)n2, st));
+            AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, 0, shg::ROW_START, st, rv));
+            for (int s = 0; s < s_tape; s++) {
+                AHIP_TRY(shg::launch_col_pass(n, c->d_desc, 1, c->tw, st, cv));
+                float4 *snap = (s + 1) % K == 0 ? b.ckpt + (size_t)((s + 1) / K) * n2 : nullptr;
+                AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, s, shg::ROW_MID, st, rv, snap));
+            }
+            const shg::TapeArgs tp = tape_of(nseg - 1, false);
+            for (int s = s_tape; s < M; s++) {
+                AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
+                AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < M - 1 ? shg::ROW_MID : shg::ROW_END, tp,
+                                              st));
+            }
+        }
+        AHIP_TRY(shg::launch_power(c->U, (int64_t)n2, c->part, c->nparts, c->red, st));
+        return SHG_OK;
+    };
+
+    // 2. per segment, last first: the adjoint on `st`, the next segment's tape on `ts`
+    const shg::phys::Consts k(c->p);
+    const double T = c->p.temp;
+    const double eps0 = 8.8541878128e-6, cl = 299.792458;
+    const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
+    const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
+    const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
+    const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
+    const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
+    const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
+    const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
+    auto enqueue_adjoint = [&]() -> int {
+        if (nseg > 1) {                                     // forks `ts` off `st`
+            AHIP_TRY(hipEventRecord(b.ev_u, st));
+            AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
+        }
+        for (int j = nseg - 1; j >= 0; j--) {
+            if (j >= 1) {
+                // slot (j-1) % 2 was last read by the adjoint of segment j+1
+                if (j + 1 <= nseg - 1) AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_free[(j - 1) & 1], 0));
+                AHIP_TRY(shg::launch_copy_cells(c->U, b.ckpt + (size_t)(j - 1) * n2, (int64_t)n2, b.ts));
+                ARC_TRY(enqueue_tape(c, tape_of(j - 1, false), j * K, b.ts));
+                AHIP_TRY(hipEventRecord(b.ev_tape[(j - 1) & 1], b.ts));
+            }
+            if (j < nseg - 1) AHIP_TRY(hipStreamWaitEvent(st, b.ev_tape[j & 1], 0));   // joins `ts` at j = 0
+            const int s0 = j * K, s1 = std::min(M, s0 + K);
+            const shg::TapeArgs tp = tape_of(j, false);
+            AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st, sens));
+            for (int s = s1 - 1; s >= s0; s--) {
+                AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st, sens));
+                AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1,
+                                             b.g2, st, sens));
+            }
+            if (j >= 2) AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
+        }
+        if (sens) AHIP_TRY(shg::launch_adj_reduce(b.g1, b.g2, rb, b.gH, cb, M, b.gs, st));
+        return SHG_OK;
+    };
+
+    // The adjoint march is replayed from a graph; the forward is launched directly: its
+    // ~800 launches replayed from a captured graph ran 0.45 ms slower than launched one by
+    // one (measured at 256 x 256, nz 400), unlike the plain march's graph.
+    hipGraphExec_t &g_adj = b.g_adj[sens ? 0 : 1];
+    if (c->use_graph && !g_adj) ARC_TRY(shg_internal_capture(st, enqueue_adjoint, &g_adj));
+
+    // forward, seed and adjoint back to back: the seed scale needs P_in only, and eta is
+    // read back with the gradients, so the host waits once
+    const double pin = c->p_fh_in;
+    ARC_TRY(ensure_pump_fields(c, np, T));
+    AHIP_TRY(hipEventRecord(c->e0, st));
+    ARC_TRY(enqueue_forward());
+    AHIP_TRY(hipEventRecord(c->e1, st));
+    AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
+    if (c->use_graph) {
+        AHIP_TRY(hipGraphLaunch(g_adj, st));
+    } else {
+        ARC_TRY(enqueue_adjoint());
+    }
+    ARC_TRY(enqueue_gradient_results(c, cs, cp, d_eta_d_fh != nullptr, st));
+    AHIP_TRY(hipStreamSynchronize(st));
+    float fwd_ms = 0.f, adj_ms = 0.f;
+    AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
+    AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e1, b.ev_end));
+
+    // 3. gradients on the host (FP64)
+    const HostOffsets o = host_offsets(n2, M);
+    const double J = cs * host_at<double2>(b, o.sums)->y / pin;
+    copy_field_out(c, d_eta_d_fh);
+    res->eta = J;
     res->forward_ms = fwd_ms;
     res->adjoint_ms = adj_ms;
     res->checkpoint_interval = K;
     res->segments = nseg;
+    if (!sens) return SHG_OK;
+    const double *gs = host_at<double>(b, o.gs);
+    const double d_focus = host_at<double2>(b, o.gsum)->x, dT_pump = host_at<double2>(b, o.gsum)->y * dnp;
+
+    // temperature through the step coefficients and the TPA coefficient
+    const double kap = (double)k.kappa_p((float)T), kas = (double)k.kappa_s((float)T);
+    const double dkap = -kap * dnp / np, dkas = -kas * dns / ns;
+    const double ddk = 2 * M_PI * (2 * dnp / (double)k.lp - dns / (double)k.ls);
+    const double dtpa = 0.5 * (double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * dns;
+    const double dkf = (double)k.dk((float)T);
+    double dT_coef = 0.0, dT_H = 0.0;
+    float z = 0.0f;
+    for (int s = 0; s < M; s++) {
+        const double *x = &gs[(size_t)s * 6];
+        const cd c1b(x[0], x[1]), c2b(x[2], x[3]);
+        const double tb = x[4];
+        const double zs = (double)z, th = (double)(float)(z * (float)dkf);
+        const cd I(0, 1);
+        const cd dc1 = I * std::exp(-I * th) * (dkap - I * kap * zs * ddk);
+        const cd dc2 = I * std::exp(I * th) * (dkas + I * kas * zs * ddk);
+        dT_coef += (std::conj(c1b) * dc1).real() + (std::conj(c2b) * dc2).real() + tb * dtpa;
+        dT_H += x[5];
+        z += k.dz;
+    }
+    const double dT_powers = J * (dns / ns - dnp / np);
+
+    res->d_eta_d_focus = d_focus;
+    res->d_eta_d_temp_crystal = dT_coef + dT_H + dT_powers;
+    res->d_eta_d_temp = res->d_eta_d_temp_crystal + dT_pump;
     return SHG_OK;
 }
 
@@ -409,153 +568,20 @@ int shg_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjo
     return shg_internal_guard(__func__, [&]() -> int {
         if (!c || !res) return shg_internal_fail(SHG_E_INVALID, "shg_adjoint: null argument");
         if (!c->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_adjoint: call shg_set_fields first");
-        if (c->p.thermal) return thermal_adjoint(c, checkpoint_interval, d_eta_d_fh, res);
-        const int M = c->steps, n = c->n;
-        const size_t n2 = (size_t)n * n;
-        int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
-        K = std::max(1, std::min(K, M));
-        const int nseg = (M + K - 1) / K;
-        ARC_TRY(ensure_buffers(c, K, nseg));
-        AdjointBuffers &b = c->adj;
-        hipStream_t st = c->stream;
-        const int rb = shg::adj_row_parts(n), cb = shg::adj_col_parts(n);
+        return c->p.thermal ? thermal_adjoint(c, checkpoint_interval, d_eta_d_fh, res, true)
+                            : plain_adjoint(c, checkpoint_interval, d_eta_d_fh, res, true);
+    });
+}
 
-        // tape of segment j: slot j % 2, its start state is checkpoint j
-        auto tape_of = [&](int j, bool write_ckpt) {
-            const size_t off = (size_t)(nseg > 1 ? (j & 1) : 0) * K * n2;
-            return shg::TapeArgs{b.ckpt + (size_t)j * n2, b.a2 + off, b.spec + off,
-                                 write_ckpt ? b.ckpt + (size_t)j * n2 : nullptr, j * K, 0};
-        };
-
-        // 1. forward: one uninterrupted march whose row passes at the segment boundaries
-        //    leave the checkpoints (the state after step jK - 1), the last segment taped, the
-        //    powers
-        auto enqueue_forward = [&]() -> int {
-            AHIP_TRY(shg::launch_copy_cells(c->U, c->U0, (int64_t)n2, st));
-            if (nseg == 1) {
-                ARC_TRY(enqueue_tape(c, tape_of(0, true), M, st));
-            } else {
-                const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
-                const int s_tape = (nseg - 1) * K;             // first step of the taped segment
-                AHIP_TRY(shg::launch_copy_cells(b.ckpt, c->U0, (int64_t)n2, st));
-                AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, 0, shg::ROW_START, st, rv));
-                for (int s = 0; s < s_tape; s++) {
-                    AHIP_TRY(shg::launch_col_pass(n, c->d_desc, 1, c->tw, st, cv));
-                    float4 *snap = (s + 1) % K == 0 ? b.ckpt + (size_t)((s + 1) / K) * n2 : nullptr;
-                    AHIP_TRY(shg::launch_row_pass(n, c->d_desc, 1, c->tw, s, shg::ROW_MID, st, rv, snap));
-                }
-                const shg::TapeArgs tp = tape_of(nseg - 1, false);
-                for (int s = s_tape; s < M; s++) {
-                    AHIP_TRY(shg::launch_col_tape(n, c->d_desc, c->tw, s, tp, st));
-                    AHIP_TRY(shg::launch_row_tape(n, c->d_desc, c->tw, s, s < M - 1 ? shg::ROW_MID : shg::ROW_END, tp,
-                                                  st));
-                }
-            }
-            AHIP_TRY(shg::launch_power(c->U, (int64_t)n2, c->part, c->nparts, c->red, st));
-            return SHG_OK;
-        };
-
-        // 2. per segment, last first: the adjoint on `st`, the next segment's tape on `ts`
-        const shg::phys::Consts k(c->p);
-        const double T = c->p.temp;
-        const double eps0 = 8.8541878128e-6, cl = 299.792458;
-        const double np = shg::phys::index_d(c->p.crystal, (double)k.lp, T), ns = shg::phys::index_d(c->p.crystal, (double)k.ls, T);
-        const double cp = 0.5 * eps0 * cl * np * (double)k.dx * (double)k.dy;
-        const double cs = 0.5 * eps0 * cl * ns * (double)k.dx * (double)k.dy;
-        const double dnp = dn_dT(c->p.crystal, (double)k.lp, T), dns = dn_dT(c->p.crystal, (double)k.ls, T);
-        const double kp = 2 * M_PI * np / (double)k.lp, ks = 2 * M_PI * ns / (double)k.ls;
-        const double dkp = 2 * M_PI * dnp / (double)k.lp, dks = 2 * M_PI * dns / (double)k.ls;
-        const float a_p = (float)((double)k.dz * dkp / (kp * kp)), a_s = (float)((double)k.dz * dks / (ks * ks));
-        auto enqueue_adjoint = [&]() -> int {
-            if (nseg > 1) {                                     // forks `ts` off `st`
-                AHIP_TRY(hipEventRecord(b.ev_u, st));
-                AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_u, 0));
-            }
-            for (int j = nseg - 1; j >= 0; j--) {
-                if (j >= 1) {
-                    // slot (j-1) % 2 was last read by the adjoint of segment j+1
-                    if (j + 1 <= nseg - 1) AHIP_TRY(hipStreamWaitEvent(b.ts, b.ev_free[(j - 1) & 1], 0));
-                    AHIP_TRY(shg::launch_copy_cells(c->U, b.ckpt + (size_t)(j - 1) * n2, (int64_t)n2, b.ts));
-                    ARC_TRY(enqueue_tape(c, tape_of(j - 1, false), j * K, b.ts));
-                    AHIP_TRY(hipEventRecord(b.ev_tape[(j - 1) & 1], b.ts));
-                }
-                if (j < nseg - 1) AHIP_TRY(hipStreamWaitEvent(st, b.ev_tape[j & 1], 0));   // joins `ts` at j = 0
-                const int s0 = j * K, s1 = std::min(M, s0 + K);
-                const shg::TapeArgs tp = tape_of(j, false);
-                AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s1 - 1, shg::ADJ_START, tp, b.g1, b.g2, st));
-                for (int s = s1 - 1; s >= s0; s--) {
-                    AHIP_TRY(shg::launch_adj_col(n, b.dlam, c->tw, s, tp, b.qxy, a_p, a_s, b.gH, st));
-                    AHIP_TRY(shg::launch_adj_row(n, b.dlam, c->tw, s, s > s0 ? shg::ADJ_MID : shg::ADJ_END, tp, b.g1,
-                                                 b.g2, st));
-                }
-                if (j >= 2) AHIP_TRY(hipEventRecord(b.ev_free[j & 1], st));
-            }
-            AHIP_TRY(shg::launch_adj_reduce(b.g1, b.g2, rb, b.gH, cb, M, b.gs, st));
-            return SHG_OK;
-        };
-
-        // The adjoint march is replayed from a graph; the forward is launched directly: its
-        // ~800 launches replayed from a captured graph ran 0.45 ms slower than launched one by
-        // one (measured at 256 x 256, nz 400), unlike the plain march's graph.
-        if (c->use_graph && !b.g_adj) ARC_TRY(shg_internal_capture(st, enqueue_adjoint, &b.g_adj));
-
-        // forward, seed and adjoint back to back: the seed scale needs P_in only, and eta is
-        // read back with the gradients, so the host waits once
-        const double pin = c->p_fh_in;
-        ARC_TRY(ensure_pump_fields(c, np, T));
-        AHIP_TRY(hipEventRecord(c->e0, st));
-        ARC_TRY(enqueue_forward());
-        AHIP_TRY(hipEventRecord(c->e1, st));
-        AHIP_TRY(shg::launch_adj_seed(c->U, b.lam, (int64_t)n2, (float)(2.0 * cs / pin), st));
-        if (c->use_graph) {
-            AHIP_TRY(hipGraphLaunch(b.g_adj, st));
-        } else {
-            ARC_TRY(enqueue_adjoint());
-        }
-        ARC_TRY(enqueue_gradient_results(c, cs, cp, d_eta_d_fh != nullptr, st));
-        AHIP_TRY(hipStreamSynchronize(st));
-        float fwd_ms = 0.f, adj_ms = 0.f;
-        AHIP_TRY(hipEventElapsedTime(&fwd_ms, c->e0, c->e1));
-        AHIP_TRY(hipEventElapsedTime(&adj_ms, c->e1, b.ev_end));
-
-        // 3. gradients on the host (FP64)
-        const HostOffsets o = host_offsets(n2, M);
-        const double J = cs * host_at<double2>(b, o.sums)->y / pin;
-        const double *gs = host_at<double>(b, o.gs);
-        const double d_focus = host_at<double2>(b, o.gsum)->x, dT_pump = host_at<double2>(b, o.gsum)->y * dnp;
-        copy_field_out(c, d_eta_d_fh);
-
-        // temperature through the step coefficients and the TPA coefficient
-        const double kap = (double)k.kappa_p((float)T), kas = (double)k.kappa_s((float)T);
-        const double dkap = -kap * dnp / np, dkas = -kas * dns / ns;
-        const double ddk = 2 * M_PI * (2 * dnp / (double)k.lp - dns / (double)k.ls);
-        const double dtpa = 0.5 * (double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * dns;
-        const double dkf = (double)k.dk((float)T);
-        double dT_coef = 0.0, dT_H = 0.0;
-        float z = 0.0f;
-        for (int s = 0; s < M; s++) {
-            const double *x = &gs[(size_t)s * 6];
-            const cd c1b(x[0], x[1]), c2b(x[2], x[3]);
-            const double tb = x[4];
-            const double zs = (double)z, th = (double)(float)(z * (float)dkf);
-            const cd I(0, 1);
-            const cd dc1 = I * std::exp(-I * th) * (dkap - I * kap * zs * ddk);
-            const cd dc2 = I * std::exp(I * th) * (dkas + I * kas * zs * ddk);
-            dT_coef += (std::conj(c1b) * dc1).real() + (std::conj(c2b) * dc2).real() + tb * dtpa;
-            dT_H += x[5];
-            z += k.dz;
-        }
-        const double dT_powers = J * (dns / ns - dnp / np);
-
-        res->eta = J;
-        res->d_eta_d_focus = d_focus;
-        res->d_eta_d_temp_crystal = dT_coef + dT_H + dT_powers;
-        res->d_eta_d_temp = res->d_eta_d_temp_crystal + dT_pump;
-        res->forward_ms = fwd_ms;
-        res->adjoint_ms = adj_ms;
-        res->checkpoint_interval = K;
-        res->segments = nseg;
-        return SHG_OK;
+int shg_adjoint_field(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, double *eta) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !d_eta_d_fh) return shg_internal_fail(SHG_E_INVALID, "shg_adjoint_field: null argument");
+        if (!c->have_fields) return shg_internal_fail(SHG_E_STATE, "shg_adjoint_field: call shg_set_fields first");
+        shg_adjoint_result r{};
+        const int rc = c->p.thermal ? thermal_adjoint(c, checkpoint_interval, d_eta_d_fh, &r, false)
+                                    : plain_adjoint(c, checkpoint_interval, d_eta_d_fh, &r, false);
+        if (rc == SHG_OK && eta) *eta = r.eta;
+        return rc;
     });
 }
 

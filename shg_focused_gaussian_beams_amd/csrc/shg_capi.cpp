@@ -260,7 +260,8 @@ void release(shg_ctx *c) {
                       (void *)c->adj.grad, (void *)c->adj.gpart, (void *)c->adj.gsum})
         if (ptr) (void)hipFree(ptr);
     if (c->adj.host) (void)hipHostFree(c->adj.host);
-    if (c->adj.g_adj) (void)hipGraphExecDestroy(c->adj.g_adj);
+    for (hipGraphExec_t g : c->adj.g_adj)
+        if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : {c->adj.ev_u, c->adj.ev_tape[0], c->adj.ev_tape[1], c->adj.ev_free[0], c->adj.ev_free[1],
                          c->adj.ev_end})
         if (e) (void)hipEventDestroy(e);

@@ -30,9 +30,10 @@ struct ThermAdjArgs {
     int pad;
 };
 
-// `d` is a one-entry instance table whose U is the adjoint field.
+// `d` is a one-entry instance table whose U is the adjoint field.  sens = false: lambda
+// only (the field gradient); G, Dp, Ap and As are neither read nor written.
 hipError_t launch_therm_adj_row(int n, const InstDesc *d, const float2 *tw, int mode, const ThermAdjArgs &a,
-                                hipStream_t st);
+                                hipStream_t st, bool sens = true);
 // p[0 .. n) = 0 (a kernel, so it may sit in a captured graph).
 hipError_t launch_fill_zero(double *p, int64_t n, hipStream_t st);
 // Ap[v] <- Ap[v] dn_p/dT(T[v]) + As[v] dn_s/dT(T[v]) (FP64 indices, central difference

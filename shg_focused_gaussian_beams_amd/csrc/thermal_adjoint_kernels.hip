@@ -77,7 +77,9 @@ __device__ __forceinline__ void therm_rk4_adj(float2 P, float2 S, float2 &lP, fl
     therm_rk4_adj(P, S, stg, lP, lS, c, ph, st, k, dDK, dnp, dns);
 }
 
-template <int N>
+// SENS = false propagates lambda only (the field gradient): the sensitivities to DKint
+// and to the indices (G, Dp, Ap, As) are neither formed nor stored.
+template <int N, bool SENS>
 __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const InstDesc *__restrict__ descs,
                                                                      const float2 *__restrict__ twg, int mode,
                                                                      ThermAdjArgs a) {
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
         dk_c = a.dk[(size_t)ls * n2 + idx];
         nn_p = a.NT[(size_t)a.prev.slice * n2 + idx];
     }
-    double G = a.G[idx], Dp = a.Dp[idx];
+    double G = SENS ? a.G[idx] : 0.0, Dp = SENS ? a.Dp[idx] : 0.0;
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     if (cell) {
         const float4 v = U[idx];
@@ -128,8 +130,10 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
         if (mode == ADJ_START) {
             double dDK = 0.0, dnp = 0.0, dns = 0.0;
             therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), P, S, cc, dk_c, a.cur, k, dDK, dnp, dns);
-            *Apc += dnp;
-            *Asc += dns;
+            if constexpr (SENS) {
+                *Apc += dnp;
+                *Asc += dns;
+            }
             Dp = dDK;
         } else {
             float2 Ap_ = f2(A.x, A.y), As_ = f2(A.z, A.w);
@@ -145,21 +149,27 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
             double dDK = 0.0, dnp = 0.0, dns = 0.0;
             therm_rk4_adj(Ap_, As_, P, S, cc, dk_c, a.cur, k, dDK, dnp, dns);
             G += Dp + dDK;                                          // dJ/dDKint_s complete
-            *Apc += dnp + G * wp;
-            *Asc += dns + G * ws;
+            if constexpr (SENS) {
+                *Apc += dnp + G * wp;
+                *Asc += dns + G * ws;
+            }
             Dp = 0.0;
             if (mode == ADJ_MID) {
                 double dDK2 = 0.0, dnp2 = 0.0, dns2 = 0.0;
                 therm_rk4_adj(f2(A.x, A.y), f2(A.z, A.w), kept, P, S, cp, php, a.prev, k, dDK2, dnp2, dns2);
-                a.Ap[(size_t)a.prev.slice * n2 + idx] += dnp2;
-                a.As[(size_t)a.prev.slice * n2 + idx] += dns2;
+                if constexpr (SENS) {
+                    a.Ap[(size_t)a.prev.slice * n2 + idx] += dnp2;
+                    a.As[(size_t)a.prev.slice * n2 + idx] += dns2;
+                }
                 Dp = dDK2;
             }
         }
         lds[0][o] = P;
         lds[1][o] = S;
-        a.G[idx] = G;
-        a.Dp[idx] = Dp;
+        if constexpr (SENS) {
+            a.G[idx] = G;
+            a.Dp[idx] = Dp;
+        }
     }
     __syncthreads();
     if (mode != ADJ_END && f < 2 && lane < T) fft_lds_field<N, E, +1>(lds[f], lane, tw);
@@ -195,14 +205,23 @@ __global__ __launch_bounds__(256) void k_therm_dtmap(double *__restrict__ Ap, co
 }
 
 // ---------------------------------------------------------------------------
+template <int N>
+static void therm_adj_row_launch(const InstDesc *d, const float2 *tw, int mode, const ThermAdjArgs &a, hipStream_t st,
+                                 bool sens) {
+    if (sens)
+        hipLaunchKernelGGL((k_therm_adj_row<N, true>), dim3(N), dim3(64 * AdjWide<N>::W), 0, st, d, tw, mode, a);
+    else
+        hipLaunchKernelGGL((k_therm_adj_row<N, false>), dim3(N), dim3(64 * AdjWide<N>::W), 0, st, d, tw, mode, a);
+}
+
 hipError_t launch_therm_adj_row(int n, const InstDesc *d, const float2 *tw, int mode, const ThermAdjArgs &a,
-                                hipStream_t st) {
+                                hipStream_t st, bool sens) {
     if (mode != ADJ_START && mode != ADJ_MID && mode != ADJ_END) return hipErrorInvalidValue;
     switch (n) {
-        case 64: hipLaunchKernelGGL(k_therm_adj_row<64>, dim3(64), dim3(64 * AdjWide<64>::W), 0, st, d, tw, mode, a); break;
-        case 128: hipLaunchKernelGGL(k_therm_adj_row<128>, dim3(128), dim3(64 * AdjWide<128>::W), 0, st, d, tw, mode, a); break;
-        case 256: hipLaunchKernelGGL(k_therm_adj_row<256>, dim3(256), dim3(64 * AdjWide<256>::W), 0, st, d, tw, mode, a); break;
-        case 512: hipLaunchKernelGGL(k_therm_adj_row<512>, dim3(512), dim3(64 * AdjWide<512>::W), 0, st, d, tw, mode, a); break;
+        case 64: therm_adj_row_launch<64>(d, tw, mode, a, st, sens); break;
+        case 128: therm_adj_row_launch<128>(d, tw, mode, a, st, sens); break;
+        case 256: therm_adj_row_launch<256>(d, tw, mode, a, st, sens); break;
+        case 512: therm_adj_row_launch<512>(d, tw, mode, a, st, sens); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

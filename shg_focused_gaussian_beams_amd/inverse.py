@@ -25,8 +25,8 @@ def ascend_phase_mask(ctx, pump0, mask0, iterations, step=0.02, signal=None, che
     for _ in range(iterations):
         field = (pump0 * np.exp(1j * phi)).astype(np.complex64)
         ctx.set_fields(field, sh)
-        res, g = ctx.adjoint(checkpoint_interval=checkpoint_interval)
-        etas.append(res["eta"])
+        eta, g = ctx.adjoint_field(checkpoint_interval=checkpoint_interval)
+        etas.append(eta)
         d = mask_gradient(g, field)
         m = np.abs(d).max()
         if m > 0:

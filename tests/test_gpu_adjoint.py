@@ -220,3 +220,21 @@ def test_torch_autograd_wrapper(gpu):
         res, g = ctx.adjoint()
     assert abs(eta.item() - res["eta"]) <= 1e-12
     assert np.array_equal(a0.grad.numpy(), g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ckpt", [0, 5])
+def test_field_only_adjoint_equals_full_adjoint(gpu, ckpt):
+    # shg_adjoint_field skips the focus / temperature sensitivities but propagates the same
+    # lambda: the same field gradient to rounding (the kernels without the sensitivity
+    # arithmetic are compiled, and their operations fused, differently) and the same eta
+    # (the forward is shared)
+    p = _params(nz=25)
+    pump, noise, sh = _inputs(gpu, p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(pump + noise, sh)
+        res, g_full = ctx.adjoint(checkpoint_interval=ckpt)
+        eta, g_field = ctx.adjoint_field(checkpoint_interval=ckpt)
+        res2, g_full2 = ctx.adjoint(checkpoint_interval=ckpt)        # and back: both graphs kept
+    assert orc.rel_l2(g_field, g_full) <= 1e-5 and eta == res["eta"]
+    assert np.array_equal(g_full, g_full2) and res2["d_eta_d_temp"] == res["d_eta_d_temp"]

@@ -213,3 +213,23 @@ def test_thermal_field_gradient_full_size_vs_gpu_fd(gpu):
     rel = np.linalg.norm(fd - ad) / np.linalg.norm(ad)
     print(f"\neta {res['eta']:.6e}: 32 smooth directions, FD step 1e-3 vs adjoint relL2 {rel:.2e}", flush=True)
     assert rel < 5e-3
+
+
+@pytest.mark.gpu
+def test_thermal_field_only_adjoint_equals_full_adjoint(gpu):
+    p = _params(nz=20)
+    T = _smooth_T(p)
+    fh, sh = _inputs(gpu, p, T)
+    with gpu.Context(p) as ctx:
+        ctx.set_temperature(T)
+        ctx.set_fields(fh, sh)
+        res, g_full = ctx.adjoint()
+        m = ctx.adjoint_temperature_map()
+        eta, g_field = ctx.adjoint_field()
+        with pytest.raises(gpu.ShgError):                              # no map from a field-only run
+            ctx.adjoint_temperature_map()
+        res2, _ = ctx.adjoint()
+        m2 = ctx.adjoint_temperature_map()
+    # the same lambda to rounding (the field-only kernels are compiled differently)
+    assert orc.rel_l2(g_field, g_full) <= 1e-5 and eta == res["eta"]
+    assert np.array_equal(m, m2) and res2["d_eta_d_temp"] == res["d_eta_d_temp"]
