@@ -160,6 +160,112 @@ __global__ __launch_bounds__(128 * AdjCol<N>::CW) void k_col_tape(const InstDesc
     lds_to_cols<N, CW, THREADS>(gp(d.U), x0, lds);
 }
 
+// Field-split register forms of k_col_tape and k_adj_col (N >= 128): the column pass
+// does not couple the fields, so each field of the CW columns gets its own CW*T threads
+// (field 0 the first half of the workgroup), E elements of one column per thread in
+// registers, and the halves transform side by side; no staging of the cells through LDS.
+template <int N> struct AdjColF {
+    static constexpr int E = AdjCol<N>::E, CW = AdjCol<N>::CW, T = N / E, HALF = CW * T, L = LdsLen<N>::value;
+    static_assert(HALF % 64 == 0, "each field on whole waves");
+};
+
+template <int N>
+__global__ __launch_bounds__(2 * AdjColF<N>::HALF) void k_col_tape_f(const InstDesc *__restrict__ descs,
+                                                                    const float2 *__restrict__ twg, int step,
+                                                                    TapeArgs tp) {
+    using C = AdjColF<N>;
+    constexpr int E = C::E, CW = C::CW, T = C::T, HALF = C::HALF;
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    const InstDesc d = descs[0];
+    const auto U = gp(reinterpret_cast<float2 *>(d.U));    // cell i, field f at 2 i + f
+    const auto G = gp(d.G);
+    const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF, c = r % CW, t = r / CW;
+    const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
+    __syncthreads();
+    float2 a[E], gy[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    const float2 gx = G[2 * f * N + x];
+#pragma unroll
+    for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];
+    auto sync = [] { __syncthreads(); };
+    fft1fe<N, E, +1>(a, &lds[f][c], t, CW, tw, sync);
+    if (tp.spec) {
+        const auto spec = gp(reinterpret_cast<float2 *>(tp.spec)) + 2 * (size_t)(step - tp.s0) * N * N;
+#pragma unroll
+        for (int j = 0; j < E; j++) spec[2 * ((size_t)(t + j * T) * N + x) + f] = a[j];
+    }
+#pragma unroll
+    for (int j = 0; j < E; j++) a[j] = c_mul(c_mul(gx, gy[j]), a[j]);
+    fft1fe<N, E, -1>(a, &lds[f][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) U[2 * ((size_t)(t + j * T) * N + x) + f] = a[j];
+}
+
+template <int N, bool SENS>
+__global__ __launch_bounds__(2 * AdjColF<N>::HALF) void k_adj_col_f(const InstDesc *__restrict__ descs,
+                                                                   const float2 *__restrict__ twg, int step,
+                                                                   TapeArgs tp, const float2 *__restrict__ qxy,
+                                                                   float a_p, float a_s, double *__restrict__ gH) {
+    using C = AdjColF<N>;
+    constexpr int E = C::E, CW = C::CW, T = C::T, HALF = C::HALF;
+    __shared__ float2 lds[2][C::L * CW];
+    __shared__ float2 tw[N];
+    __shared__ double red[2 * HALF / 64];
+    const InstDesc d = descs[0];
+    const auto U = gp(reinterpret_cast<float2 *>(d.U));
+    const auto G = gp(d.G);
+    const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF, c = r % CW, t = r / CW;
+    const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
+    __syncthreads();
+    float2 a[E], gy[E], xs[E];
+    float q[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    const float2 gx = G[2 * f * N + x];
+#pragma unroll
+    for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];
+    if constexpr (SENS) {
+        const auto spec = gp(reinterpret_cast<const float2 *>(tp.spec)) + 2 * (size_t)(step - tp.s0) * N * N;
+        const float qx = qxy[x].x;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            xs[j] = spec[2 * ((size_t)(t + j * T) * N + x) + f];
+            q[j] = qx + qxy[t + j * T].y;
+        }
+    }
+    auto sync = [] { __syncthreads(); };
+    fft1fe<N, E, +1>(a, &lds[f][c], t, CW, tw, sync);
+    const float af = f ? a_s : a_p;
+    double gsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        const float2 h = c_mul(gx, gy[j]);
+        if constexpr (SENS) {
+            // Re(conj(l) * h * xs * i a q) = -a q Im(conj(l) h xs)
+            const float2 u = c_mul(c_mul(cj(a[j]), h), xs[j]);
+            gsum += -(double)(af * q[j]) * (double)u.y;
+        }
+        a[j] = c_mul(cj(h), a[j]);
+    }
+    fft1fe<N, E, -1>(a, &lds[f][c], t, CW, tw, sync);
+#pragma unroll
+    for (int j = 0; j < E; j++) U[2 * ((size_t)(t + j * T) * N + x) + f] = a[j];
+    if constexpr (SENS) {
+        for (int off = 32; off > 0; off >>= 1) gsum += __shfl_down(gsum, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = gsum;
+        __syncthreads();
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < 2 * HALF / 64; w++) sum += red[w];
+            gH[(size_t)step * gridDim.x + blockIdx.x] = sum;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint row pass on lambda (descs[0].U is the adjoint state; coef/nl are the forward's).
 // ADJ_START at s: RK4_s^T at A''_s, R_bwd.
@@ -398,9 +504,23 @@ hipError_t launch_row_tape(int n, const InstDesc *d, const float2 *tw, int step,
     return hipGetLastError();
 }
 
+// The column kernels of the adjoint: the field-split register forms from 128 on (their
+// grid is the same N / CW blocks, so the gH partials keep their count), the LDS forms at 64.
+#define SHG_DISPATCH_F(n, CALL)                             \
+    switch (n) {                                            \
+        case 128: { constexpr int NN = 128; CALL; } break; \
+        case 256: { constexpr int NN = 256; CALL; } break; \
+        case 512: { constexpr int NN = 512; CALL; } break; \
+        default: return hipErrorInvalidValue;            \
+    }
+
 hipError_t launch_col_tape(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, hipStream_t st) {
-    SHG_DISPATCH(n, hipLaunchKernelGGL(k_col_tape<NN>, dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0, st, d,
-                                       tw, step, tp));
+    if (n == 64) {
+        hipLaunchKernelGGL(k_col_tape<64>, dim3(col_blocks<64>()), dim3(128 * AdjCol<64>::CW), 0, st, d, tw, step, tp);
+        return hipGetLastError();
+    }
+    SHG_DISPATCH_F(n, hipLaunchKernelGGL(k_col_tape_f<NN>, dim3(col_blocks<NN>()), dim3(2 * AdjColF<NN>::HALF), 0, st,
+                                       d, tw, step, tp));
     return hipGetLastError();
 }
 
@@ -419,12 +539,21 @@ hipError_t launch_adj_row(int n, const InstDesc *d, const float2 *tw, int step, 
 
 hipError_t launch_adj_col(int n, const InstDesc *d, const float2 *tw, int step, const TapeArgs &tp, const float2 *qxy,
                           float a_p, float a_s, double *gH, hipStream_t st, bool sens) {
+    if (n == 64) {
+        if (sens)
+            hipLaunchKernelGGL((k_adj_col<64, true>), dim3(col_blocks<64>()), dim3(128 * AdjCol<64>::CW), 0, st, d, tw,
+                               step, tp, qxy, a_p, a_s, gH);
+        else
+            hipLaunchKernelGGL((k_adj_col<64, false>), dim3(col_blocks<64>()), dim3(128 * AdjCol<64>::CW), 0, st, d, tw,
+                               step, tp, qxy, a_p, a_s, gH);
+        return hipGetLastError();
+    }
     if (sens) {
-        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_col<NN, true>), dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0,
-                                           st, d, tw, step, tp, qxy, a_p, a_s, gH));
+        SHG_DISPATCH_F(n, hipLaunchKernelGGL((k_adj_col_f<NN, true>), dim3(col_blocks<NN>()), dim3(2 * AdjColF<NN>::HALF),
+                                           0, st, d, tw, step, tp, qxy, a_p, a_s, gH));
     } else {
-        SHG_DISPATCH(n, hipLaunchKernelGGL((k_adj_col<NN, false>), dim3(col_blocks<NN>()), dim3(128 * AdjCol<NN>::CW), 0,
-                                           st, d, tw, step, tp, qxy, a_p, a_s, gH));
+        SHG_DISPATCH_F(n, hipLaunchKernelGGL((k_adj_col_f<NN, false>), dim3(col_blocks<NN>()), dim3(2 * AdjColF<NN>::HALF),
+                                           0, st, d, tw, step, tp, qxy, a_p, a_s, gH));
     }
     return hipGetLastError();
 }
