@@ -238,3 +238,27 @@ def test_field_only_adjoint_equals_full_adjoint(gpu, ckpt):
         res2, g_full2 = ctx.adjoint(checkpoint_interval=ckpt)        # and back: both graphs kept
     assert orc.rel_l2(g_field, g_full) <= 1e-5 and eta == res["eta"]
     assert np.array_equal(g_full, g_full2) and res2["d_eta_d_temp"] == res["d_eta_d_temp"]
+
+
+@pytest.mark.gpu
+def test_strang_scheme_gradient_vs_gpu_fd(gpu):
+    # the adjoint of the STRANG scheme (half-length RK4 steps) against central FD of the
+    # GPU forward along smooth modulations of the pump
+    from shg_focused_gaussian_beams_amd.params import SHG_SCHEME_STRANG
+    p = _params(nz=24, scheme=SHG_SCHEME_STRANG)
+    pump, noise, sh = _inputs(gpu, p)
+    a0 = pump + noise
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(a0, sh)
+        res, g = ctx.adjoint()
+        g = g.astype(np.complex128)
+        fd, ad = [], []
+        for v in _smooth_directions(a0, 16, 9):
+            ad.append(float(np.real(np.sum(np.conj(g) * v))))
+            e = 1e-3
+            fd.append((_eta_gpu(ctx, (a0 + e * v).astype(np.complex64), sh) -
+                       _eta_gpu(ctx, (a0 - e * v).astype(np.complex64), sh)) / (2 * e))
+    fd, ad = np.array(fd), np.array(ad)
+    rel = np.linalg.norm(fd - ad) / np.linalg.norm(ad)
+    print(f"\n  strang: 16 smooth directions, FD vs adjoint relL2 {rel:.2e}", flush=True)
+    assert rel < 5e-3
