@@ -262,3 +262,48 @@ def test_strang_scheme_gradient_vs_gpu_fd(gpu):
     rel = np.linalg.norm(fd - ad) / np.linalg.norm(ad)
     print(f"\n  strang: 16 smooth directions, FD vs adjoint relL2 {rel:.2e}", flush=True)
     assert rel < 5e-3
+
+
+@pytest.mark.gpu
+def test_ppln_temperature_gradient_vs_oracle64_fd(gpu):
+    # MgO:PPLN (its own Sellmeier and dn/dT): at 39.234 C with a 6.95 um grating the float
+    # and double phase mismatch agree to 1.2e-6 rad over the crystal (dk L = 1.84 rad)
+    from shg_focused_gaussian_beams_amd.params import SHG_CRYSTAL_PPLN
+    p = _params(nz=30, crystal=SHG_CRYSTAL_PPLN, qpm_period=6.95, temp=39.234)
+    assert abs(orc.dk(p, 32, p.temp) - orc.dk(p, 64, p.temp)) * p.lz < 1e-5
+    pump, noise, sh = _inputs(gpu, p)
+    a0 = (pump + noise).astype(np.complex64)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(a0, sh)
+        res, g = ctx.adjoint()
+    hT = 1e-3 * p.temp
+
+    def eta_T(t):
+        q = p.copy(temp=t)
+        return _eta_orc(q, orc.pump(q, 64) + noise, sh, 64, t)
+    fdT = (eta_T(p.temp + hT) - eta_T(p.temp - hT)) / (2 * hT)
+    print(f"\n  PPLN temp: fd64 {fdT:+.6e} adjoint {res['d_eta_d_temp']:+.6e}")
+    assert abs(fdT - res["d_eta_d_temp"]) / abs(fdT) < 2e-2
+
+
+@pytest.mark.gpu
+def test_walkoff_gradient_vs_gpu_fd(gpu):
+    # walk-off (rho != 0: the propagator's x factor carries exp(-i dz 2 pi fx tan rho)):
+    # field gradient against central FD of the GPU forward along smooth pump modulations
+    p = _params(nz=24, rho_p=2e-3, rho_s=1e-3)
+    pump, noise, sh = _inputs(gpu, p)
+    a0 = pump + noise
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(a0, sh)
+        res, g = ctx.adjoint()
+        g = g.astype(np.complex128)
+        fd, ad = [], []
+        for v in _smooth_directions(a0, 16, 12):
+            ad.append(float(np.real(np.sum(np.conj(g) * v))))
+            e = 1e-3
+            fd.append((_eta_gpu(ctx, (a0 + e * v).astype(np.complex64), sh) -
+                       _eta_gpu(ctx, (a0 - e * v).astype(np.complex64), sh)) / (2 * e))
+    fd, ad = np.array(fd), np.array(ad)
+    rel = np.linalg.norm(fd - ad) / np.linalg.norm(ad)
+    print(f"\n  walk-off: 16 smooth directions, FD vs adjoint relL2 {rel:.2e}", flush=True)
+    assert rel < 5e-3
