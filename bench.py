@@ -115,6 +115,38 @@ def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=12.0, budget_o0_s=4.0):
             "sample_O0": f"the same at -O0: first {k0} z-steps, {dt0:.1f} s"}
 
 
+def cpu_batch_projection(ps, budget_s=1.5):
+    """The sweep as the reference's SHG.sh runs it: one serial solve per point.  oracle32's
+    plain march is timed on one core (pinned) for one instance of every grid size in the
+    sweep, over as many z-steps as fit in about budget_s each, and the per-size rate is
+    extrapolated to every instance's cells x steps."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    affinity = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, {min(affinity)})
+    rates = {}
+    try:
+        for n in sorted({p.nx for p in ps}):
+            q = next(p for p in ps if p.nx == n)
+            fh = orc.pump(q, 32)
+            sh = np.zeros_like(fh)
+            t0 = time.perf_counter()
+            orc.cwes(q, fh, sh, prec=32, nsteps=1)
+            one = time.perf_counter() - t0
+            k = max(1, min(q.nz - 1, int(budget_s / max(one, 1e-6))))
+            t0 = time.perf_counter()
+            orc.cwes(q, fh, sh, prec=32, nsteps=k)
+            rates[n] = n * n * k / (time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, affinity)
+    serial_s = sum(p.nx * p.ny * (p.nz - 1) / rates[p.nx] for p in ps)
+    return {"serial_s_projected": serial_s, "cell_steps_per_s_by_size": {str(n): r for n, r in rates.items()},
+            "cores": 1, "kind": "port",
+            "sample": "oracle32 -O3 plain march, one instance per grid size for about "
+                      f"{budget_s:g} s each, pinned to one core, extrapolated by cells x steps"}
+
+
 def copy_peak_gbps(mib=1024, reps=5):
     """Read+write bandwidth of a device-to-device copy (the practical HBM ceiling)."""
     import torch
@@ -448,6 +480,12 @@ def main():
         cb = cpu_baseline_thermal(p, th["T"], th["fh0"], th["sh0"])
         cb["gpu_speedup"] = value / cb["value"]
         out["cpu_baseline"] = cb
+        bsw = out["config"].get("batched_sweep_256_uneven")
+        if bsw is not None:
+            log("cpu baseline of the sweep (sampled) ...")
+            cs = cpu_batch_projection(shg.uneven_sweep_params(count=args.batch, seed=3))
+            cs["gpu_speedup"] = cs["serial_s_projected"] / bsw["s_per_batch"]
+            bsw["cpu_serial"] = cs
     print(json.dumps(out), flush=True)
 
 
