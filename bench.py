@@ -311,7 +311,9 @@ def bench_inverse(shg, args):
     +-0.1 rad pump phase mask with the field held fixed, judged by a plain forward.
     `inverse_instances` instances ascend at once (one context and host thread each) for
     `inverse_iterations` iterations; instance 0 is also timed alone.  Both are projected
-    to 32 instances x 100 iterations."""
+    to 32 instances x 100 iterations.  The adjoint keeps a full tape (checkpoint interval
+    nz - 1: ~9.4 GB per 512x512, nz 1000 instance) instead of recomputing sqrt(nz)
+    segments: HBM holds it, and the recomputation competes with the adjoint march."""
     import numpy as np
     import torch
     from shg_focused_gaussian_beams_amd.inverse import ascend_phase_mask, ascend_phase_masks
@@ -332,16 +334,17 @@ def bench_inverse(shg, args):
         pumps.append(shg.make_pump(p, float(T[0, p.ny // 2, p.nx // 2])))
         ctxs.append(ctx)
     it = args.inverse_iterations
-    ascend_phase_mask(ctxs[0], pumps[0], masks[0], 1)                # warm-up: graphs, tuning, buffers
+    ck = ctxs[0].params.nz - 1                                         # full tape
+    ascend_phase_mask(ctxs[0], pumps[0], masks[0], 1, checkpoint_interval=ck)   # warm-up: graphs, tuning, buffers
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ascend_phase_mask(ctxs[0], pumps[0], masks[0], it)
+    ascend_phase_mask(ctxs[0], pumps[0], masks[0], it, checkpoint_interval=ck)
     torch.cuda.synchronize()
     alone_s = (time.perf_counter() - t0) / it
-    ascend_phase_masks(ctxs, pumps, masks, 1)                          # warm-up of the others
+    ascend_phase_masks(ctxs, pumps, masks, 1, checkpoint_interval=ck)             # warm-up of the others
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = ascend_phase_masks(ctxs, pumps, masks, it)
+    res = ascend_phase_masks(ctxs, pumps, masks, it, checkpoint_interval=ck)
     torch.cuda.synchronize()
     conc_s = (time.perf_counter() - t0) / it / n
     out = []
@@ -356,7 +359,7 @@ def bench_inverse(shg, args):
         judge.close()
     th_s = float(np.mean([o["thermal_s"] for o in out]))
     cells = 512 * 512 * 999
-    return dict(instances_concurrent=n, iterations_timed=it, per_instance=out,
+    return dict(instances_concurrent=n, iterations_timed=it, checkpoint_interval=ck, per_instance=out,
                 s_per_iteration_alone=alone_s, s_per_instance_iteration_concurrent=conc_s,
                 fwd_adj_cell_steps_per_s_alone=cells / alone_s,
                 fwd_adj_cell_steps_per_s_concurrent=cells / conc_s,
