@@ -381,7 +381,7 @@ def main():
     ap.add_argument("--plain-variants", default="", help="ROW,COL: the same for the plain 256x256 march")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
-    ap.add_argument("--inverse-instances", type=int, default=4)
+    ap.add_argument("--inverse-instances", type=int, default=8)
     ap.add_argument("--inverse-iterations", type=int, default=2)
     ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse"], default="all",
                     help="run one extra workload alone and print its figures (for profiler runs)")
