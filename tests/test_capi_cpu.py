@@ -97,7 +97,9 @@ def test_sweep_rejects_bad_command_lines(tmp_path):
     exe = os.path.join(ROOT, "shg_focused_gaussian_beams_amd", "SHG_sweep")
     if not os.path.exists(exe):
         pytest.skip("SHG_sweep program not built")
-    for bad in (["--power=a:b"], ["--power=5:1:1"], ["--nope=1"], ["stray"]):
+    for bad in (["--power=a:b"], ["--power=5:1:1"], ["--nope=1"], ["stray"],
+                ["--power=30", "--waist=20", "--t1=27", "--workers=0"],
+                ["--power=30", "--waist=20", "--t1=27", "--workers=65"]):
         r = subprocess.run([exe] + bad, capture_output=True, text=True, cwd=tmp_path)
         assert r.returncode == 1 and ("usage" in r.stderr or "option" in r.stderr), (bad, r.stderr)
 
