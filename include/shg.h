@@ -176,6 +176,22 @@ typedef struct shg_thermal_stats {
  * returns the Pump_out/Signal_out state of the last iteration. */
 int shg_thermal_run(shg_ctx *ctx, int pump_from_temperature, shg_thermal_stats *stats);
 
+/* Progress of shg_thermal_run, reported at the points where the reference's THERMAL run
+ * prints to stdout (src/headers/Solver.h:299-333): the "Global iteration #" banner and
+ * its tolerance, the residual line at every check, the "N iterations -> steady-state."
+ * line and the start of the CWES march.  The callback runs on the calling thread, inside
+ * shg_thermal_run; it must not call into the same context. */
+#define SHG_THERMAL_EVENT_OUTER   0  /* outer iteration `outer` begins; value = Jacobi tolerance */
+#define SHG_THERMAL_EVENT_CHECK   1  /* residual check after sweep number `sweeps` (0, check, 2 check, ...);
+                                        value = mean squared update of that sweep (FP64) */
+#define SHG_THERMAL_EVENT_STEADY  2  /* the Jacobi of this iteration stopped after `sweeps` sweeps;
+                                        value = the last residual */
+#define SHG_THERMAL_EVENT_MARCH   3  /* the z-march begins; value = grid-centre temperature of slice 0 */
+typedef void (*shg_thermal_log_fn)(void *user, int event, int32_t outer, int64_t sweeps, double value);
+
+/* Installs (fn != NULL) or removes the progress callback of shg_thermal_run. */
+int shg_set_thermal_log(shg_ctx *ctx, shg_thermal_log_fn fn, void *user);
+
 /* `nsweeps` Jacobi sweeps of heatEquationTopOpen (src/headers/Tfield.h:113-221) on the
  * current temperature and heat source; *mean_sq_update (if non-null) receives
  * sum((Tf-Ti)^2)/(nx*ny*heat_nz) of the last sweep in FP64 (reducedSum, Tfield.h:13-28). */

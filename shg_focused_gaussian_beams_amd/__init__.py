@@ -44,6 +44,11 @@ class AdjointResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+# shg_thermal_log_fn and its events (include/shg.h)
+THERMAL_LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_double)
+SHG_THERMAL_EVENT_OUTER, SHG_THERMAL_EVENT_CHECK, SHG_THERMAL_EVENT_STEADY, SHG_THERMAL_EVENT_MARCH = 0, 1, 2, 3
+
+
 class ShgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -81,6 +86,7 @@ def load_library(path=LIB_PATH):
         "shg_profile_kernels": (ctypes.c_int, [vp, ctypes.c_int, dp, dp]),
         "shg_thermal_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ThermalStats)]),
         "shg_jacobi": (ctypes.c_int, [vp, ctypes.c_int64, dp]),
+        "shg_set_thermal_log": (ctypes.c_int, [vp, THERMAL_LOG_FN, vp]),
         "shg_get_temperature": (ctypes.c_int, [vp, fp]),
         "shg_set_temperature": (ctypes.c_int, [vp, fp]),
         "shg_get_heat_source": (ctypes.c_int, [vp, fp]),
@@ -248,9 +254,19 @@ class Context:
         hz = self.params.heat_nz if self.params.heat_nz > 0 else self.params.nz
         return (hz, self.params.ny, self.params.nx)
 
-    def thermal_run(self, pump_from_temperature=True):
+    def thermal_run(self, pump_from_temperature=True, log=None):
+        """THERMAL Solver::run (src/headers/Solver.h:289-340).  `log`, if given, receives
+        the progress events (event, outer, sweeps, value) of shg_set_thermal_log."""
         st = ThermalStats()
-        _check(self.lib.shg_thermal_run(self.h, 1 if pump_from_temperature else 0, ctypes.byref(st)))
+        cb = None
+        if log is not None:
+            cb = THERMAL_LOG_FN(lambda user, ev, outer, sweeps, value: log(ev, outer, sweeps, value))
+            _check(self.lib.shg_set_thermal_log(self.h, cb, None))
+        try:
+            _check(self.lib.shg_thermal_run(self.h, 1 if pump_from_temperature else 0, ctypes.byref(st)))
+        finally:
+            if cb is not None:
+                _check(self.lib.shg_set_thermal_log(self.h, THERMAL_LOG_FN(), None))
         return st.as_dict()
 
     def jacobi(self, nsweeps, residual=True):

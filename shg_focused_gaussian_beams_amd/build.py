@@ -25,12 +25,21 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
+    """Compiles every translation unit to an object (in parallel, into _build/), links
+    libshg.so, then the two programs.  Compiles every time it is called."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = []
-    for s in SOURCES:
-        srcs += ["-x", "hip", os.path.join(CSRC, s)]
-    _run([hipcc, *HIP_FLAGS, "-shared", "-o", LIB + ".tmp", *srcs])
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        futs = [pool.submit(_run, [hipcc, *HIP_FLAGS, "-c", "-x", "hip", os.path.join(CSRC, s), "-o", o])
+                for s, o in zip(SOURCES, objs)]
+        for f in futs:
+            f.result()
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs])
     os.replace(LIB + ".tmp", LIB)
     for exe, src in ((CLI, "shg_cli.cpp"), (SWEEP, "shg_sweep.cpp")):
         _run(["g++", "-O2", "-std=c++17", "-pthread", "-o", exe, os.path.join(CSRC, src), "-L" + HERE, "-lshg",

@@ -14,6 +14,8 @@
 
 // Device buffers of the adjoint (allocated on first use, kept with the context).
 struct AdjointBuffers {
+    bool base_ready = false;                   // the one-time buffers below are all set up
+    bool th_ready = false;                     // the same for the THERMAL buffers (Ap .. dsum)
     int k = 0, nseg = 0;                       // checkpoint interval and segment count they were sized for
     float4 *lam = nullptr;                     // adjoint state
     float4 *ckpt = nullptr;                    // nseg checkpoints (segment-start states)
@@ -80,6 +82,8 @@ struct shg_ctx {
     int use_graph = 1;
     int row_v = -1, col_v = -1;      // plain-pass layout variants (-1: not tuned yet)
     std::vector<float> host_stage;   // 4 floats per cell
+    shg_thermal_log_fn log_fn = nullptr;   // progress events of shg_thermal_run
+    void *log_user = nullptr;
     AdjointBuffers adj;
 };
 

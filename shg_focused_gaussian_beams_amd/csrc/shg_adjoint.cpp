@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -67,46 +68,81 @@ template <class T> T *host_at(AdjointBuffers &b, size_t off) {
     return reinterpret_cast<T *>(static_cast<char *>(b.host) + off);
 }
 
-int ensure_buffers(shg_ctx *c, int K, int nseg) {
+// Device allocation of the one-time adjoint buffers.  SHG_ADJ_FAIL_ALLOC=k makes the
+// k-th such allocation of a setup fail (a test hook for the recovery path below).
+hipError_t adj_malloc(void **ptr, size_t bytes, int *count) {
+    const char *f = std::getenv("SHG_ADJ_FAIL_ALLOC");
+    if (f && std::atoi(f) == ++*count) return hipErrorOutOfMemory;
+    return hipMalloc(ptr, bytes);
+}
+
+// Frees the one-time buffers of ensure_buffers and resets them, so that a setup that
+// failed part way leaves nothing behind and the next call starts again from scratch.
+void free_base_buffers(AdjointBuffers &b) {
+    for (void **ptr : {(void **)&b.lam, (void **)&b.g1, (void **)&b.g2, (void **)&b.gH, (void **)&b.gs,
+                       (void **)&b.qxy, (void **)&b.dlam, (void **)&b.grad, (void **)&b.gpart, (void **)&b.gsum})
+        if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
+    if (b.host) { (void)hipHostFree(b.host); b.host = nullptr; }
+    b.host_bytes = 0;
+    for (hipEvent_t *e : {&b.ev_end, &b.ev_u, &b.ev_tape[0], &b.ev_tape[1], &b.ev_free[0], &b.ev_free[1]})
+        if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
+    if (b.ts) { (void)hipStreamDestroy(b.ts); b.ts = nullptr; }
+    b.base_ready = false;
+}
+
+int setup_base_buffers(shg_ctx *c) {
     AdjointBuffers &b = c->adj;
     const size_t n2 = (size_t)c->n * c->n;
     const int M = c->steps;
-    if (!b.lam) {
-        if (hipMalloc(&b.lam, n2 * sizeof(float4)) != hipSuccess ||
-            hipMalloc(&b.g1, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.g2, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.gH, (size_t)M * shg::adj_col_parts(c->n) * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.gs, (size_t)M * 6 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.qxy, (size_t)c->n * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&b.dlam, sizeof(shg::InstDesc)) != hipSuccess ||
-            hipMalloc(&b.grad, n2 * sizeof(float2)) != hipSuccess ||
-            hipMalloc(&b.gpart, kGradParts * sizeof(double2)) != hipSuccess ||
-            hipMalloc(&b.gsum, sizeof(double2)) != hipSuccess)
-            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
-        b.host_bytes = host_offsets(n2, M).end;
-        if (hipHostMalloc(&b.host, b.host_bytes, hipHostMallocDefault) != hipSuccess) {
-            b.host = nullptr;
-            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: pinned host allocation failed");
+    int count = 0;
+    if (adj_malloc((void **)&b.lam, n2 * sizeof(float4), &count) != hipSuccess ||
+        adj_malloc((void **)&b.g1, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double), &count) != hipSuccess ||
+        adj_malloc((void **)&b.g2, (size_t)M * shg::adj_row_parts(c->n) * 5 * sizeof(double), &count) != hipSuccess ||
+        adj_malloc((void **)&b.gH, (size_t)M * shg::adj_col_parts(c->n) * sizeof(double), &count) != hipSuccess ||
+        adj_malloc((void **)&b.gs, (size_t)M * 6 * sizeof(double), &count) != hipSuccess ||
+        adj_malloc((void **)&b.qxy, (size_t)c->n * sizeof(float2), &count) != hipSuccess ||
+        adj_malloc((void **)&b.dlam, sizeof(shg::InstDesc), &count) != hipSuccess ||
+        adj_malloc((void **)&b.grad, n2 * sizeof(float2), &count) != hipSuccess ||
+        adj_malloc((void **)&b.gpart, kGradParts * sizeof(double2), &count) != hipSuccess ||
+        adj_malloc((void **)&b.gsum, sizeof(double2), &count) != hipSuccess)
+        return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+    b.host_bytes = host_offsets(n2, M).end;
+    if (hipHostMalloc(&b.host, b.host_bytes, hipHostMallocDefault) != hipSuccess) {
+        b.host = nullptr;
+        return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: pinned host allocation failed");
+    }
+    AHIP_TRY(hipEventCreate(&b.ev_end));
+    AHIP_TRY(hipStreamCreateWithFlags(&b.ts, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&b.ev_u, &b.ev_tape[0], &b.ev_tape[1], &b.ev_free[0], &b.ev_free[1]})
+        AHIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    // 2 pi^2 f^2 on the fftshifted index grid (DC at 0), x and y spacings
+    const shg::phys::Consts k(c->p);
+    std::vector<float> q(2 * (size_t)c->n);
+    const double PI = 3.14159265358979323846;
+    for (int i = 0; i < c->n; i++) {
+        const int orig = (i + c->n / 2) % c->n;
+        const double fx = (orig - 0.5 * c->n) / ((double)k.dx * c->n), fy = (orig - 0.5 * c->n) / ((double)k.dy * c->n);
+        q[2 * i] = (float)(2 * PI * PI * fx * fx);
+        q[2 * i + 1] = (float)(2 * PI * PI * fy * fy);
+    }
+    AHIP_TRY(shg_internal_copy(b.qxy, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    shg::InstDesc d{};
+    AHIP_TRY(shg_internal_copy(&d, c->d_desc, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+    d.U = b.lam;
+    AHIP_TRY(shg_internal_copy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+    b.base_ready = true;
+    return SHG_OK;
+}
+
+int ensure_buffers(shg_ctx *c, int K, int nseg) {
+    AdjointBuffers &b = c->adj;
+    const size_t n2 = (size_t)c->n * c->n;
+    if (!b.base_ready) {
+        const int rc = setup_base_buffers(c);
+        if (rc) {
+            free_base_buffers(b);
+            return rc;
         }
-        AHIP_TRY(hipEventCreate(&b.ev_end));
-        AHIP_TRY(hipStreamCreateWithFlags(&b.ts, hipStreamNonBlocking));
-        for (hipEvent_t *e : {&b.ev_u, &b.ev_tape[0], &b.ev_tape[1], &b.ev_free[0], &b.ev_free[1]})
-            AHIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        // 2 pi^2 f^2 on the fftshifted index grid (DC at 0), x and y spacings
-        const shg::phys::Consts k(c->p);
-        std::vector<float> q(2 * (size_t)c->n);
-        const double PI = 3.14159265358979323846;
-        for (int i = 0; i < c->n; i++) {
-            const int orig = (i + c->n / 2) % c->n;
-            const double fx = (orig - 0.5 * c->n) / ((double)k.dx * c->n), fy = (orig - 0.5 * c->n) / ((double)k.dy * c->n);
-            q[2 * i] = (float)(2 * PI * PI * fx * fx);
-            q[2 * i + 1] = (float)(2 * PI * PI * fy * fy);
-        }
-        AHIP_TRY(shg_internal_copy(b.qxy, q.data(), q.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        shg::InstDesc d{};
-        AHIP_TRY(shg_internal_copy(&d, c->d_desc, sizeof(d), hipMemcpyDeviceToHost, c->stream));
-        d.U = b.lam;
-        AHIP_TRY(shg_internal_copy(b.dlam, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
     }
     if (b.k != K || b.nseg != nseg) {
         for (hipGraphExec_t &g : b.g_adj)
@@ -208,11 +244,19 @@ void copy_field_out(shg_ctx *c, float *d_eta_d_fh) {
 int ensure_thermal_buffers(shg_ctx *c, int K, int nseg) {
     AdjointBuffers &b = c->adj;
     const size_t n2 = (size_t)c->n * c->n, n3 = n2 * c->hz;
-    if (!b.Ap) {
-        if (hipMalloc(&b.Ap, n3 * sizeof(double)) != hipSuccess || hipMalloc(&b.As, n3 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.Gc, n2 * sizeof(double)) != hipSuccess || hipMalloc(&b.Dc, n2 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b.dpart, kMapParts * sizeof(double)) != hipSuccess || hipMalloc(&b.dsum, sizeof(double)) != hipSuccess)
+    if (!b.th_ready) {
+        int count = 0;
+        if (adj_malloc((void **)&b.Ap, n3 * sizeof(double), &count) != hipSuccess ||
+            adj_malloc((void **)&b.As, n3 * sizeof(double), &count) != hipSuccess ||
+            adj_malloc((void **)&b.Gc, n2 * sizeof(double), &count) != hipSuccess ||
+            adj_malloc((void **)&b.Dc, n2 * sizeof(double), &count) != hipSuccess ||
+            adj_malloc((void **)&b.dpart, kMapParts * sizeof(double), &count) != hipSuccess ||
+            adj_malloc((void **)&b.dsum, sizeof(double), &count) != hipSuccess) {
+            for (double **ptr : {&b.Ap, &b.As, &b.Gc, &b.Dc, &b.dpart, &b.dsum})
+                if (*ptr) { (void)hipFree(*ptr); *ptr = nullptr; }
             return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+        }
+        b.th_ready = true;
     }
     if (b.th_k != K || b.th_nseg != nseg) {
         for (float **ptr : {&b.dkt, &b.dkc})

@@ -331,17 +331,25 @@ int thermal_init_temperature(shg_ctx *c) {
     return SHG_OK;
 }
 
-int timed_march(shg_ctx *c) {
-    if (c->row_v < 0) {
-        const char *nt = std::getenv("SHG_NO_TUNE");
-        if (nt && nt[0] == '1') {
-            c->row_v = 0;
-            c->col_v = 0;
-        } else {
-            RC_TRY(tune_variants(c, 20));
-            HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-        }
+// Picks the pass layouts on first use (SHG_NO_TUNE=1: the first layouts).  Tuning runs
+// the passes on the context's buffers, so the fields are restored from U0 (the
+// shg_set_fields state) afterwards; callers that stage other fields in U tune first.
+int ensure_tuned(shg_ctx *c) {
+    if (c->row_v >= 0) return SHG_OK;
+    const char *nt = std::getenv("SHG_NO_TUNE");
+    if (nt && nt[0] == '1') {
+        c->row_v = 0;
+        c->col_v = 0;
+        return SHG_OK;
     }
+    RC_TRY(tune_variants(c, 20));
+    HIP_TRY(hipMemcpyAsync(c->U, c->U0, (size_t)c->n * c->n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SHG_OK;
+}
+
+int timed_march(shg_ctx *c) {
+    RC_TRY(ensure_tuned(c));
     auto enqueue = [c] { return c->p.thermal ? enqueue_march_thermal(c) : enqueue_march_plain(c, 0, c->steps); };
     if (c->use_graph && !c->gexec) RC_TRY(shg_internal_capture(c->stream, enqueue, &c->gexec));
     HIP_TRY(hipEventRecord(c->e0, c->stream));
@@ -717,6 +725,10 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
         const int check = c->p.jacobi_check > 0 ? c->p.jacobi_check : 5000;
         const int num_of_iter = c->p.max_outer > 0 ? c->p.max_outer : 5;
         const double tol = c->p.jacobi_tol > 0 ? (double)(float)c->p.jacobi_tol : (double)(float)(5e-4 / (double)n3);
+        // Tune before the loop: each iteration stages its own pump in U, which tuning
+        // inside the first march would overwrite with the shg_set_fields state.  Tuning
+        // also writes heat-source slices, which the initialisation below clears.
+        RC_TRY(ensure_tuned(c));
         RC_TRY(thermal_init_temperature(c));
         // the caller's z = 0 fields (SH input; FH used when pump_from_temperature == 0)
         std::vector<float> fh0(2 * ncell), sh0(2 * ncell), stage(4 * ncell);
@@ -729,7 +741,11 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
         shg_thermal_stats s{};
         double jac_ms = 0.0, march_ms = 0.0;
         int global_count = 0;
+        auto log = [c](int event, int outer, int64_t sweeps, double value) {
+            if (c->log_fn) c->log_fn(c->log_user, event, outer, sweeps, value);
+        };
         while (global_count < num_of_iter) {
+            log(SHG_THERMAL_EVENT_OUTER, global_count, 0, tol);
             // Jacobi to steady state, residual checked every `check` sweeps (src/headers/Solver.h:302-325)
             double reduced = 1.0, aux = 0.0;
             int64_t counter = 0;
@@ -739,6 +755,7 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
                 if (counter % check == 0) {
                     double r = 0.0;
                     RC_TRY(jacobi_run(c, 1, true, &r));
+                    log(SHG_THERMAL_EVENT_CHECK, global_count, counter, r);
                     if (counter > 0) aux = reduced;
                     reduced = r;
                     if (counter > 0 && reduced == aux) reduced = 0.5 * tol;
@@ -759,6 +776,7 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
             jac_ms += ms;
             s.sweeps_last = counter;
             s.sweeps_total += counter;
+            log(SHG_THERMAL_EVENT_STEADY, global_count, counter, s.residual_last);
             // fields and propagators at the temperature of the grid centre, slice 0 (Solver.h:329-331)
             float tcen = 0.f;
             HIP_TRY(shg_internal_copy(&tcen, c->T[c->tcur] + (size_t)(c->n / 2) * c->n + c->n / 2, sizeof(float),
@@ -767,6 +785,7 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
                 shg::phys::make_pump(k, (float)c->p.power, (float)c->p.focus, (float)c->p.waist, tcen, fh0.data());
             RC_TRY(set_u_from_host(c, c->U, fh0.data(), sh0.data()));
             RC_TRY(upload_propagators(c, tcen));
+            log(SHG_THERMAL_EVENT_MARCH, global_count, counter, (double)tcen);
             RC_TRY(timed_march(c));
             march_ms += c->last_ms;
             s.outer_iterations++;
@@ -781,6 +800,15 @@ int shg_thermal_run(shg_ctx *c, int pump_from_temperature, shg_thermal_stats *st
         s.jacobi_ms = jac_ms;
         s.march_ms = march_ms;
         if (st) *st = s;
+        return SHG_OK;
+    });
+}
+
+int shg_set_thermal_log(shg_ctx *c, shg_thermal_log_fn fn, void *user) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c) return fail(SHG_E_INVALID, "shg_set_thermal_log: null context");
+        c->log_fn = fn;
+        c->log_user = user;
         return SHG_OK;
     });
 }

@@ -61,6 +61,32 @@ void usage() {
 
 }  // namespace
 
+// Prints the reference's THERMAL progress lines (src/headers/Solver.h:299-333): the outer
+// iteration banner and tolerance (the reference switches stdout to 20 digits after its
+// first tolerance line), the mean squared update at every check, the sweep count, and
+// the start of the CWES march.  The residual is printed in float, as the reference sums it.
+static void print_thermal_event(void *, int event, int32_t outer, int64_t sweeps, double value) {
+    switch (event) {
+    case SHG_THERMAL_EVENT_OUTER:
+        std::cout << "\n\nGlobal iteration #" << outer << "\n\n" << std::endl;
+        std::cout << "Solving 3D Heat Equation in the steady-state...\n" << std::endl;
+        std::cout << "\nTemperature tolerance = " << (float)value << "\n" << std::endl;
+        std::cout.precision(20);
+        break;
+    case SHG_THERMAL_EVENT_CHECK:
+        std::cout << "\u03A3|Tf-Ti|\u00B2/N\u00B3 at #" << sweeps << " iteration is: " << (float)value << std::endl;
+        break;
+    case SHG_THERMAL_EVENT_STEADY:
+        std::cout << sweeps << " iterations -> steady-state." << std::endl;
+        break;
+    case SHG_THERMAL_EVENT_MARCH:
+        std::cout << "\n\nSolving Coupled-Wave Equations (CWEs)...\n" << std::endl;
+        break;
+    default:
+        break;
+    }
+}
+
 int main(int argc, char *argv[]) {
     std::cout << "#######---SHG efficiency calculator---#######\n\n" << std::endl;
     const double t_start = seconds();
@@ -111,12 +137,15 @@ int main(int argc, char *argv[]) {
     shg_ctx *ctx = nullptr;
     if ((rc = shg_create(&p, &ctx))) return die("shg_create", rc);
     if ((rc = shg_set_fields(ctx, fh.data(), sh.data()))) return die("shg_set_fields", rc);
-    std::cout << "\n\nEvolution in the crystal...\n" << std::endl;
+    if (!p.thermal) std::cout << "\n\nEvolution in the crystal...\n" << std::endl;
     if (p.thermal) {
         // THERMAL branch of Solver::run (src/headers/Solver.h:289-340): Jacobi, dk(T), the
-        // march and the heat-source feedback, up to max_outer global iterations.
+        // march and the heat-source feedback, up to max_outer global iterations, with the
+        // reference's progress lines.
+        if ((rc = shg_set_thermal_log(ctx, print_thermal_event, nullptr))) return die("shg_set_thermal_log", rc);
         shg_thermal_stats st{};
         if ((rc = shg_thermal_run(ctx, 1, &st))) return die("shg_thermal_run", rc);
+        std::cout.precision(6);
         std::cout << "Global iterations: " << st.outer_iterations << ", Jacobi sweeps: " << st.sweeps_total
                   << ", last mean squared update: " << st.residual_last << std::endl;
         if (save_only_last) {

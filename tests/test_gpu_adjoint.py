@@ -2,12 +2,11 @@
 central finite differences.
 
 - 64 seeded random field directions, central FD of the GPU forward with a step of 1e-3
-  relative to |A0| (as specified for the adjoint check).  FP32 rounding of eta puts a
-  noise of ~1e-4 absolute on these FD slopes, so they are judged as a 64-vector (relative
-  L2 and correlation); with a 1e-2 step the per-direction agreement is checked tightly.
-  At the gradient workload's full size (256x256, nz 400) the directions are smooth
-  modulations of the pump, along which eta moves at O(eta) and the FD is clean: there
-  the 64-vector must agree to 5e-3.
+  relative to |A0| (as specified for the adjoint check), judged as a 64-vector (relative
+  L2 within 5e-3, and correlation); with a 1e-2 step the first 8 directions are checked
+  one by one.  The directions are smooth modulations of the pump (random 2D Fourier modes
+  |k| <= 2), along which eta moves at O(eta); white-noise directions mostly miss the beam,
+  and the FP32 rounding of eta then dominates their FD slopes.
 - field and focus derivatives against central FD of the FP32 restatement (oracle32, the
   same arithmetic as the GPU forward, so the same operating point), step 1e-2 relative.
 - field and temperature derivatives against central FD of the FP64 restatement
@@ -81,8 +80,8 @@ def _smooth_directions(a0, count, seed):
 
 
 FD_CASES = {
-    "n64_nz24": (lambda: _params(nx=64, ny=64, nz=24), _directions, 5e-2),
-    "n128_nz40": (lambda: _params(nx=128, ny=128, nz=40), _directions, 5e-2),
+    "n64_nz24": (lambda: _params(nx=64, ny=64, nz=24), _smooth_directions, 5e-3),
+    "n128_nz40": (lambda: _params(nx=128, ny=128, nz=40), _smooth_directions, 5e-3),
     # the gradient workload at full size: 256x256, nz 400, P = 10 W, w0 = 35 um
     "n256_nz400_p10_w35": (lambda: plain128_params(nx=256, ny=256, nz=400, lx=280.0, ly=280.0, power=10.0,
                                                    waist=35.0), _smooth_directions, 5e-3),

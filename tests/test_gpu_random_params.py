@@ -5,10 +5,19 @@ the heat depth, the heat boundary and a random smooth temperature field.  Every 
 meet the parity bar (fields within relative L2 1e-4; THERMAL also the heat source),
 whatever layout the tuner picks for it.  The draws are fixed by their seeds, so a failure
 names a reproducible case.  Inputs are the workloads' (pump with 1e-6 relative noise,
-SH = 0).  With an SH input of white noise at 1e-3 of the pump and weak conversion, one
-draw's output SH, dominated by that noise's highest spatial frequencies, differed by
-7.5e-5 relative: their diffraction phase reaches ~280 rad per step, where a float phase
-is only good to ~3e-5 rad.
+SH = 0).
+
+White-noise SH input (test_white_noise_signal_draws): an SH of white noise at 1e-3 of the
+pump with weak conversion leaves an output SH dominated by the noise's highest spatial
+frequencies, whose diffraction phase reaches hundreds of rad per step.  The reference
+evaluates that phase in float per cell (src/headers/Efields.h:121-122), good to ~1e-5
+rad at that size, and the error repeats every step; this build evaluates the separable
+factors in FP64 and rounds them once.  There the distance to the FP32 restatement is
+set by the restatement's own float phases, so those draws are judged against both
+restatements at a temperature where the float and double phase mismatch agree: the GPU
+must be closer to the FP64 restatement than the FP32 one is (it is the more accurate of
+the two), and within the bar of the FP32 one up to that restatement's own distance from
+FP64: rel-L2(GPU, oracle32) <= max(1e-4, 1.25 rel-L2(oracle32, oracle64)).
 """
 import numpy as np
 import pytest
@@ -91,3 +100,31 @@ def test_random_thermal_draw_matches_oracle(gpu, seed):
     e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh), float(np.linalg.norm(Q - rQ) / np.linalg.norm(rQ)))
     print(f"  relL2 FH {e[0]:.2e} SH {e[1]:.2e} Q {e[2]:.2e}")
     assert max(e) <= TOL, (seed, e)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(300, 304))
+def test_white_noise_signal_draws(gpu, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.choice([64, 128]))
+    waist = float(rng.uniform(15.0, 30.0))
+    # Lambda = 8.0265 um at 26.913 C: the float and double phase mismatch agree there
+    p = plain128_params(nx=n, ny=n, nz=int(rng.integers(12, 30)), power=float(rng.uniform(0.5, 3.0)), waist=waist,
+                        lx=waist * 6.0, ly=waist * 6.0, temp=26.913, alpha_p=1e-7, alpha_s=4e-6, beta_s=1e-7)
+    assert abs(orc.dk(p, 32, p.temp) - orc.dk(p, 64, p.temp)) * p.lz < 1e-5
+    pump = gpu.make_pump(p)
+    amp = 1e-3 * np.abs(pump).max()
+    sh0 = (amp * (rng.standard_normal(pump.shape) + 1j * rng.standard_normal(pump.shape))).astype(np.complex64)
+    print(f"\nwhite-noise SH draw {seed}: n={n} nz={p.nz} P={p.power:.2f} w0={waist:.1f}", flush=True)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(pump, sh0)
+        ctx.forward()
+        fh, sh = ctx.get_fields()
+    rfh, rsh = orc.cwes(p, pump, sh0, prec=32)
+    dfh, dsh = orc.cwes(p, pump.astype(np.complex128), sh0.astype(np.complex128), prec=64)
+    e32 = max(orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
+    e64 = max(orc.rel_l2(fh, dfh), orc.rel_l2(sh, dsh))
+    d32 = max(orc.rel_l2(rfh, dfh), orc.rel_l2(rsh, dsh))
+    print(f"  relL2 GPU-oracle32 {e32:.2e}, GPU-oracle64 {e64:.2e}, oracle32-oracle64 {d32:.2e}")
+    assert e64 < d32
+    assert e32 <= max(TOL, 1.25 * d32)
