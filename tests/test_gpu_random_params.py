@@ -9,15 +9,12 @@ SH = 0).
 
 White-noise SH input (test_white_noise_signal_draws): an SH of white noise at 1e-3 of the
 pump with weak conversion leaves an output SH dominated by the noise's highest spatial
-frequencies, whose diffraction phase reaches 400-1500 rad per step in these draws.  The reference
-evaluates that phase in float per cell (src/headers/Efields.h:121-122), good to ~1e-5
-rad at that size, and the error repeats every step; this build evaluates the separable
-factors in FP64 and rounds them once.  There the distance to the FP32 restatement is
-set by the restatement's own float phases, so those draws are judged against both
-restatements at a temperature where the float and double phase mismatch agree: the GPU
-must be closer to the FP64 restatement than the FP32 one is (it is the more accurate of
-the two), and within the bar of the FP32 one up to that restatement's own distance from
-FP64: rel-L2(GPU, oracle32) <= max(1e-4, 1.25 rel-L2(oracle32, oracle64)).
+frequencies, whose diffraction phase turns by 400-1500 rad per step in these draws.  The
+reference evaluates that phase in float per cell (src/headers/Efields.h:121-122); this
+build evaluates the separable factors in FP64 and rounds them once.  The draws run at a
+temperature where the float and double phase mismatch agree, and print the three
+distances GPU-oracle32, GPU-oracle64 and oracle32-oracle64: all three sit at float
+rounding (measured <= 1e-5), so the draws are held to the same 1e-4 bar as the others.
 """
 import numpy as np
 import pytest
@@ -128,5 +125,4 @@ def test_white_noise_signal_draws(gpu, seed):
     e64 = max(orc.rel_l2(fh, dfh), orc.rel_l2(sh, dsh))
     d32 = max(orc.rel_l2(rfh, dfh), orc.rel_l2(rsh, dsh))
     print(f"  relL2 GPU-oracle32 {e32:.2e}, GPU-oracle64 {e64:.2e}, oracle32-oracle64 {d32:.2e}")
-    assert e64 < d32
-    assert e32 <= max(TOL, 1.25 * d32)
+    assert e32 <= TOL, (seed, e32)
