@@ -293,6 +293,15 @@ int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 /* The layouts the next march runs (-1 while untuned). */
 int shg_get_variants(const shg_ctx *ctx, int *row_variant, int *col_variant);
 
+/* Layout of the THERMAL Jacobi sweeps (no reference counterpart; the reference's sweep is
+ * heatEquationTopOpen, src/headers/Tfield.h:113-221).  Sweeps run two per pass, marching
+ * in z through LDS tiles; layouts 0..8 are 3 tile shapes x 3 workgroup-wave counts (the
+ * z-chunk depth follows from the device's resident slots), 9 is two single sweeps.  Every layout gives the reference's float sweeps bit for bit.  shg_jacobi and
+ * shg_thermal_run tune it on first use (SHG_NO_TUNE=1: layout 0); -1 while untuned. */
+#define SHG_JACOBI_LAYOUTS 10
+int shg_set_jacobi_layout(shg_ctx *ctx, int layout);
+int shg_get_jacobi_layout(const shg_ctx *ctx, int *layout);
+
 /* Measurement hook (no reference counterpart): average device time of one row pass and
  * one column pass (the tuned layouts), each replayed `reps` times back to back from a
  * graph (the fastest of three replays).  Clobbers the current fields; the next shg_forward starts again from the

@@ -47,6 +47,7 @@ class AdjointResult(ctypes.Structure):
 # shg_thermal_log_fn and its events (include/shg.h)
 THERMAL_LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_double)
 SHG_THERMAL_EVENT_OUTER, SHG_THERMAL_EVENT_CHECK, SHG_THERMAL_EVENT_STEADY, SHG_THERMAL_EVENT_MARCH = 0, 1, 2, 3
+SHG_JACOBI_LAYOUTS = 10                 # shg_set_jacobi_layout range (include/shg.h)
 
 
 class ShgError(RuntimeError):
@@ -97,6 +98,8 @@ def load_library(path=LIB_PATH):
         "shg_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_get_variants": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "shg_set_jacobi_layout": (ctypes.c_int, [vp, ctypes.c_int]),
+        "shg_get_jacobi_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
         "shg_batch_destroy": (None, [vp]),
         "shg_batch_init_fields": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint64]),
@@ -268,6 +271,14 @@ class Context:
             if cb is not None:
                 _check(self.lib.shg_set_thermal_log(self.h, THERMAL_LOG_FN(), None))
         return st.as_dict()
+
+    def set_jacobi_layout(self, layout):
+        _check(self.lib.shg_set_jacobi_layout(self.h, layout))
+
+    def jacobi_layout(self):
+        v = ctypes.c_int()
+        _check(self.lib.shg_get_jacobi_layout(self.h, ctypes.byref(v)))
+        return v.value
 
     def jacobi(self, nsweeps, residual=True):
         r = ctypes.c_double()

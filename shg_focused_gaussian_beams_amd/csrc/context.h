@@ -81,6 +81,8 @@ struct shg_ctx {
     hipGraphExec_t gexec = nullptr;
     int use_graph = 1;
     int row_v = -1, col_v = -1;      // plain-pass layout variants (-1: not tuned yet)
+    int jac_v = -1;                  // Jacobi pair layout (launch_jacobi2 variant, or two single sweeps)
+    shg::Jacobi2Plan jplan[shg::kJacobi2VariantCount] = {};   // tiles and z-chunks of each two-sweep layout
     std::vector<float> host_stage;   // 4 floats per cell
     shg_thermal_log_fn log_fn = nullptr;   // progress events of shg_thermal_run
     void *log_user = nullptr;

@@ -207,6 +207,8 @@ template <class JC> inline JC jacobi_const(const Consts &k, float hdz, float k_t
     c.fdenz = 1.0f + 2.0f * c.az + 2.0f * c.bz + Biz;
     c.bixT = Bix * t_inf; c.biyT = Biy * t_inf; c.bizT = Biz * t_inf;
     c.tinfBiD = t_inf * BiD;
+    c.rdxx = 1.0f / c.dxx; c.rdyy = 1.0f / c.dyy; c.rdzz = 1.0f / c.dzz; c.rkth = 1.0f / c.kth;
+    c.rfdenx = 1.0f / c.fdenx; c.rfdeny = 1.0f / c.fdeny; c.rfdenz = 1.0f / c.fdenz;
     const double hv = 0.5 * (double)t_inf;
     c.sv = hv * (double)(Bix + Biz);
     c.sh = hv * (double)(Biy + Biz);

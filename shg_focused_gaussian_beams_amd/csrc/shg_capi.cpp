@@ -286,22 +286,85 @@ int set_u_from_host(shg_ctx *c, float4 *dst, const float *fh, const float *sh) {
     return SHG_OK;
 }
 
-// Jacobi sweeps on the current temperature; with `want_resid` the mean squared update
-// (FP64) of the last sweep is returned in *resid.
+// Two Jacobi sweeps of the current temperature as `variant` runs them: one two-sweep pass
+// into the other buffer (< kJacobi2VariantCount), or two single sweeps there and back
+// (== kJacobi2VariantCount).  With `part`, the second sweep's partial residuals; *nparts
+// receives how many.
+int jacobi_pair(shg_ctx *c, int variant, double *part, int *nparts) {
+    const int n = c->n, hz = c->hz;
+    float *Ti = c->T[c->tcur], *Tf = c->T[c->tcur ^ 1];
+    if (variant < shg::kJacobi2VariantCount) {
+        HIP_TRY(shg::launch_jacobi2(Tf, Ti, c->Q, n, n, hz, c->jc, part, c->njpart, c->stream, c->jplan[variant]));
+        c->tcur ^= 1;
+        *nparts = c->jplan[variant].blocks;
+    } else {
+        HIP_TRY(shg::launch_jacobi(Tf, Ti, c->Q, n, n, hz, c->jc, nullptr, c->njpart, c->stream));
+        HIP_TRY(shg::launch_jacobi(Ti, Tf, c->Q, n, n, hz, c->jc, part, c->njpart, c->stream));
+        *nparts = shg::jacobi_blocks(n, n, hz);
+    }
+    return SHG_OK;
+}
+
+// Times every two-sweep layout (and two single sweeps) on this context's heat grid, each
+// replayed from a graph, and keeps the fastest.  Every timed launch reads the current
+// temperature and writes the other buffer only, so the state is unchanged.
+int tune_jacobi(shg_ctx *c) {
+    const char *nt = std::getenv("SHG_NO_TUNE");
+    if (nt && nt[0] == '1') {
+        c->jac_v = 0;
+        return SHG_OK;
+    }
+    const int n = c->n, hz = c->hz;
+    float *Ti = c->T[c->tcur], *Tf = c->T[c->tcur ^ 1];
+    float best = 1e30f;
+    int bv = 0;
+    for (int v = 0; v <= shg::kJacobi2VariantCount; v++) {
+        float ms = 0.f;
+        RC_TRY(shg_internal_time_graph(c->stream, c->e0, c->e1, 4, [&] {
+            hipError_t e;
+            if (v < shg::kJacobi2VariantCount) {
+                e = shg::launch_jacobi2(Tf, Ti, c->Q, n, n, hz, c->jc, nullptr, c->njpart, c->stream, c->jplan[v]);
+            } else {
+                e = shg::launch_jacobi(Tf, Ti, c->Q, n, n, hz, c->jc, nullptr, c->njpart, c->stream);
+                if (e == hipSuccess) e = shg::launch_jacobi(Tf, Ti, c->Q, n, n, hz, c->jc, nullptr, c->njpart, c->stream);
+            }
+            return e == hipSuccess ? (int)SHG_OK : fail(SHG_E_DEVICE, std::string("tuning Jacobi: ") + hipGetErrorString(e));
+        }, &ms));
+        if (ms < best) { best = ms; bv = v; }
+    }
+    c->jac_v = bv;
+    return SHG_OK;
+}
+
+// Jacobi sweeps on the current temperature, two per pass where it can (the layout tuned
+// on first use); with `want_resid` the mean squared update (FP64) of the last sweep is
+// returned in *resid.
 int jacobi_run(shg_ctx *c, int64_t nsweeps, bool want_resid, double *resid) {
     const size_t n3 = (size_t)c->n * c->n * c->hz;
-    for (int64_t i = 0; i < nsweeps; i++) {
-        const bool r = want_resid && i == nsweeps - 1;
+    if (nsweeps >= 2 && c->jac_v < 0) RC_TRY(tune_jacobi(c));
+    int64_t i = 0;
+    int nparts = 0;
+    bool have_part = false;
+    while (nsweeps - i >= 2) {
+        const bool r = want_resid && nsweeps - i == 2;
+        RC_TRY(jacobi_pair(c, c->jac_v, r ? c->jpart : nullptr, &nparts));
+        have_part = r;
+        i += 2;
+    }
+    if (i < nsweeps) {
         float *Ti = c->T[c->tcur], *Tf = c->T[c->tcur ^ 1];
-        HIP_TRY(shg::launch_jacobi(Tf, Ti, c->Q, c->n, c->n, c->hz, c->jc, r ? c->jpart : nullptr, c->njpart, c->stream));
+        HIP_TRY(shg::launch_jacobi(Tf, Ti, c->Q, c->n, c->n, c->hz, c->jc, want_resid ? c->jpart : nullptr, c->njpart,
+                                   c->stream));
         c->tcur ^= 1;
-        if (r) {
-            HIP_TRY(shg::launch_sum_partials(c->jpart, c->njpart, c->jres, c->stream));
-            double h;
-            HIP_TRY(hipMemcpyAsync(&h, c->jres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            *resid = h / (double)n3;
-        }
+        nparts = shg::jacobi_blocks(c->n, c->n, c->hz);
+        have_part = want_resid;
+    }
+    if (have_part) {
+        HIP_TRY(shg::launch_sum_partials(c->jpart, nparts, c->jres, c->stream));
+        double h;
+        HIP_TRY(hipMemcpyAsync(&h, c->jres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        *resid = h / (double)n3;
     }
     return SHG_OK;
 }
@@ -468,6 +531,11 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         } else {
             const size_t n3 = ncell * c->hz;
             c->njpart = shg::jacobi_blocks(c->n, c->n, c->hz);
+            for (int v = 0; v < shg::kJacobi2VariantCount; v++) {
+                if (shg::jacobi2_plan(c->n, c->n, c->hz, v, &c->jplan[v]) != 0)
+                    return bad(fail(SHG_E_DEVICE, "shg_create: Jacobi layout query failed"));
+                c->njpart = std::max(c->njpart, c->jplan[v].blocks);
+            }
             if (hipMalloc(&c->Uout, ncell * sizeof(float4)) != hipSuccess ||
                 hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
                 hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
@@ -683,6 +751,24 @@ int shg_get_variants(const shg_ctx *c, int *row_variant, int *col_variant) {
         if (!c || !row_variant || !col_variant) return fail(SHG_E_INVALID, "shg_get_variants: bad argument");
         *row_variant = c->row_v;
         *col_variant = c->col_v;
+        return SHG_OK;
+    });
+}
+
+int shg_set_jacobi_layout(shg_ctx *c, int layout) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        static_assert(SHG_JACOBI_LAYOUTS == shg::kJacobi2VariantCount + 1, "Jacobi layout count");
+        if (!c || !c->p.thermal) return fail(SHG_E_INVALID, "shg_set_jacobi_layout: needs a THERMAL context");
+        if (layout < 0 || layout >= SHG_JACOBI_LAYOUTS) return fail(SHG_E_INVALID, "shg_set_jacobi_layout: layout out of range");
+        c->jac_v = layout;
+        return SHG_OK;
+    });
+}
+
+int shg_get_jacobi_layout(const shg_ctx *c, int *layout) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!c || !layout) return fail(SHG_E_INVALID, "shg_get_jacobi_layout: bad argument");
+        *layout = c->jac_v;
         return SHG_OK;
     });
 }

@@ -17,6 +17,8 @@ struct JacobiConst {
     float av, bv, cv, ah, bh, ch, ad, bd, cd;     // edges
     float AV, BV, CV, GV, tinfBiD;                // vertices
     double V, H, D, sv, sh, sd;                   // double parts of the edge formulas
+    float rdxx, rdyy, rdzz, rkth;                 // RN(1/x) of the divisors above (division by a known
+    float rfdenx, rfdeny, rfdenz;                 //   divisor in three operations, bit-identical)
     int bc;                                       // SHG_BC_*: which voxels a sweep writes
 };
 
@@ -61,6 +63,15 @@ struct ThermRowArgs {
 
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                          double *part, int nparts, hipStream_t st);
+// Two sweeps in one pass (z-marching LDS tiles, temporal blocking), bit-exact with two
+// launch_jacobi sweeps; with `part` the second sweep's squared updates, one FP64 partial
+// per block (plan.blocks of them).  Variants: 3 tile shapes x 3 wave counts; a plan
+// (jacobi2_plan, which queries the current device) fixes the tiles and z-chunks.
+constexpr int kJacobi2VariantCount = 9;
+struct Jacobi2Plan { int shape, tiles_x, zc, nchunks, blocks; };
+int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan);   // 0 on success
+hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
+                          double *part, int nparts, hipStream_t st, const Jacobi2Plan &plan);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
 // Per heat voxel: tab = {n_p(T), n_s(T)} and ct = {kp, ks, dk, tpa} with the reference's
 // float operations (T is frozen during one march, so the row pass reads them instead of

@@ -9,6 +9,8 @@
 // rounding is amplified about a thousandfold.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fft_lds.h"
 #include "nl_step.h"
 #include "thermal_cell.h"
@@ -19,55 +21,93 @@
 namespace shg {
 
 // ---------------------------------------------------------------------------
-// One Jacobi sweep Tf <- J(Ti, Q) of heatEquationTopOpen (src/headers/Tfield.h:113-221):
-// thread per voxel; the y = 0 plane (Dirichlet ovens) is never written.  With
-// c.bc == SHG_BC_OVEN_SURROUNDED, kernelHeatEquationOvenSurrounded (:224-258): the four
-// lateral faces x = 0, nx-1, y = 0, ny-1 are Dirichlet ovens and are never written; the
-// voxels it updates (inner points and the z = 0, z = L faces) use the same formulas as
-// the top-open sweep's inner points and z faces.  With RESID the FP64 sum of
-// (Tf - Ti)^2 over the block goes to part[block].
+// One Jacobi sweep of heatEquationTopOpen (src/headers/Tfield.h:113-221) at a voxel with
+// y >= 1 of x class xc and z class zc (jacobi_cls) below the top (yi) or on it: the new
+// value from the old values of its neighbours, ld(dx, dy, dz) = Ti at (x+dx, y+dy, z+dz),
+// and q() = Q at the voxel.  Every formula keeps the
+// reference's float/double typing and operation order, with contraction off, so that
+// whichever kernel calls it, a sweep is the reference's float sweep bit for bit.  The
+// voxels kernelHeatEquationOvenSurrounded updates (:224-258: inner points, z = 0 and
+// z = L faces inside the lateral ovens) use the same inner and z-face formulas.
+// a / d for a divisor d known in advance, with r = RN(1/d): q = RN(a r) is within an ulp
+// of a/d, the FMA gives the exact remainder a - q d, and one more FMA step returns the
+// correctly rounded quotient (Markstein's theorem; barring overflow and underflow, which
+// the heat equation's values stay far from).  Three operations instead of the IEEE
+// division's sequence, bit-identical to it.
+__device__ __forceinline__ float div_const(float a, float d, float r) {
+    const float q = a * r;
+    const float e = __builtin_fmaf(-q, d, a);
+    return __builtin_fmaf(e, r, q);
+}
+
+// Boundary class of a coordinate along x or z: 0 inside, +1 on the low face (its one
+// neighbour at +1), -1 on the high face (neighbour at -1).
+__device__ __forceinline__ int jacobi_cls(int i, int n) { return i == 0 ? 1 : (i == n - 1 ? -1 : 0); }
+
+template <class LD, class LQ>
+__device__ __forceinline__ float jacobi_voxel(const JacobiConst &c, int xc, bool yi, int zc, LD ld, LQ q) {
+#pragma clang fp contract(off)
+    const bool xi = xc == 0, zi = zc == 0;
+    const int xn = xc, zn = zc;                        // the one x (z) neighbour on a face
+    float out;
+    if (yi) {
+        if (xi && zi) {
+            out = c.L2 * (div_const(ld(1, 0, 0) + ld(-1, 0, 0), c.dxx, c.rdxx) +
+                          div_const(ld(0, 1, 0) + ld(0, -1, 0), c.dyy, c.rdyy) +
+                          div_const(ld(0, 0, 1) + ld(0, 0, -1), c.dzz, c.rdzz) + div_const(q(), c.kth, c.rkth));
+        } else if (!xi && zi) {
+            out = div_const(ld(xn, 0, 0) + c.ax * (ld(0, 1, 0) + ld(0, -1, 0)) + c.bx * (ld(0, 0, 1) + ld(0, 0, -1)) +
+                                c.bixT, c.fdenx, c.rfdenx);
+        } else if (xi && !zi) {
+            out = div_const(ld(0, 0, zn) + c.az * (ld(1, 0, 0) + ld(-1, 0, 0)) + c.bz * (ld(0, 1, 0) + ld(0, -1, 0)) +
+                                c.bizT, c.fdenz, c.rfdenz);
+        } else {
+            const float f = c.av * ld(xn, 0, 0) + c.bv * ld(0, 0, zn) + c.cv * (ld(0, 1, 0) + ld(0, -1, 0));
+            out = (float)(((double)f + c.sv) * c.V);
+        }
+    } else {
+        if (xi && zi) {
+            out = div_const(ld(0, -1, 0) + c.ay * (ld(1, 0, 0) + ld(-1, 0, 0)) + c.by * (ld(0, 0, 1) + ld(0, 0, -1)) +
+                                c.biyT, c.fdeny, c.rfdeny);
+        } else if (xi && !zi) {
+            const float f = c.ah * ld(0, -1, 0) + c.bh * ld(0, 0, zn) + c.ch * (ld(-1, 0, 0) + ld(1, 0, 0));
+            out = (float)(((double)f + c.sh) * c.H);
+        } else if (!xi && zi) {
+            const float f = c.ad * ld(0, -1, 0) + c.bd * ld(xn, 0, 0) + c.cd * (ld(0, 0, 1) + ld(0, 0, -1));
+            out = (float)(((double)f + c.sd) * c.D);
+        } else {
+            out = (c.AV * ld(xn, 0, 0) + c.BV * ld(0, -1, 0) + c.CV * ld(0, 0, zn) + c.tinfBiD) * c.GV;
+        }
+    }
+    return out;
+}
+
+// Whether a sweep writes voxel (x, y): everything above the y = 0 ovens (top open), or
+// the voxels inside the four lateral oven faces (surrounded).
+__device__ __forceinline__ bool jacobi_updates(const JacobiConst &c, int x, int y, int nx, int ny) {
+    if (c.bc == SHG_BC_OVEN_SURROUNDED) return x >= 1 && x < nx - 1 && y >= 1 && y < ny - 1;
+    return y >= 1;
+}
+
+// One Jacobi sweep Tf <- J(Ti, Q), thread per voxel (64 x 4 voxels per workgroup, one z
+// plane).  Voxels the sweep does not update are never written.  With RESID the FP64 sum
+// of (Tf - Ti)^2 over the block goes to part[block].  Used for single sweeps (the
+// residual checks of the reference's loop, odd sweep counts).
 template <bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                 const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
                                                 double *__restrict__ part) {
-#pragma clang fp contract(off)
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
     double r = 0.0;
-    const bool oven = c.bc == SHG_BC_OVEN_SURROUNDED;
-    if (x < nx && y >= 1 && y < ny && (!oven || (x >= 1 && x < nx - 1 && y < ny - 1))) {
-        const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * ny;
+    if (x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
         const size_t i = (size_t)z * sz + (size_t)y * sy + x;
-        const bool xi = x > 0 && x < nx - 1, zi = z > 0 && z < hz - 1, yi = y < ny - 1;
-        const size_t ixn = x == 0 ? i + sx : i - sx;       // the one x-neighbour of an x face
-        const size_t izn = z == 0 ? i + sz : i - sz;
-        float out;
-        if (yi) {
-            if (xi && zi) {
-                out = c.L2 * ((Ti[i + sx] + Ti[i - sx]) / c.dxx + (Ti[i + sy] + Ti[i - sy]) / c.dyy +
-                              (Ti[i + sz] + Ti[i - sz]) / c.dzz + Q[i] / c.kth);
-            } else if (!xi && zi) {
-                out = (Ti[ixn] + c.ax * (Ti[i + sy] + Ti[i - sy]) + c.bx * (Ti[i + sz] + Ti[i - sz]) + c.bixT) / c.fdenx;
-            } else if (xi && !zi) {
-                out = (Ti[izn] + c.az * (Ti[i + sx] + Ti[i - sx]) + c.bz * (Ti[i + sy] + Ti[i - sy]) + c.bizT) / c.fdenz;
-            } else {
-                const float f = c.av * Ti[ixn] + c.bv * Ti[izn] + c.cv * (Ti[i + sy] + Ti[i - sy]);
-                out = (float)(((double)f + c.sv) * c.V);
-            }
-        } else {
-            if (xi && zi) {
-                out = (Ti[i - sy] + c.ay * (Ti[i + sx] + Ti[i - sx]) + c.by * (Ti[i + sz] + Ti[i - sz]) + c.biyT) / c.fdeny;
-            } else if (xi && !zi) {
-                const float f = c.ah * Ti[i - sy] + c.bh * Ti[izn] + c.ch * (Ti[i - sx] + Ti[i + sx]);
-                out = (float)(((double)f + c.sh) * c.H);
-            } else if (!xi && zi) {
-                const float f = c.ad * Ti[i - sy] + c.bd * Ti[ixn] + c.cd * (Ti[i + sz] + Ti[i - sz]);
-                out = (float)(((double)f + c.sd) * c.D);
-            } else {
-                out = (c.AV * Ti[ixn] + c.BV * Ti[i - sy] + c.CV * Ti[izn] + c.tinfBiD) * c.GV;
-            }
-        }
+        const float out = jacobi_voxel(
+            c, jacobi_cls(x, nx), y < ny - 1, jacobi_cls(z, hz),
+            [&](int dx, int dy, int dz) { return Ti[(ptrdiff_t)i + dx + (ptrdiff_t)dy * (ptrdiff_t)sy + (ptrdiff_t)dz * (ptrdiff_t)sz]; },
+            [&] { return Q[i]; });
         Tf[i] = out;
         if (RESID) {
             const double d = (double)(out - Ti[i]);
@@ -81,6 +121,173 @@ __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const fl
         __syncthreads();
         if (threadIdx.x == 0)
             part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+    }
+}
+
+// Two Jacobi sweeps per pass, Tf <- J(J(Ti, Q), Q), marching in z (temporal blocking).
+// Workgroup = one TX x TY tile of the x-y plane and one chunk of zc planes.  It keeps in
+// LDS a ring of three planes of Ti over the tile plus a halo of 2 (region R2), a ring of
+// three planes of Q over the tile plus a halo of 1 (R1), and a ring of three planes of
+// the first sweep's values T1 over R1.  Step k of the march stages Ti plane k and Q plane
+// k-1 (both loaded into registers one step ahead, so their latency overlaps the
+// previous step's arithmetic), computes T1 plane k-1 on R1, and then the second sweep's plane k-2
+// on the tile, which it writes.  T1 never leaves the chip: a pass reads Ti once over R2
+// (plus the z halo of its chunk) and Q once over R1, and writes Tf once, where two single
+// sweeps read and write everything twice.  The halo values of T1 are recomputed by every
+// tile that needs them, with the same formulas, so the result is two single sweeps bit
+// for bit.  With RESID the FP64 sum of (T2 - T1)^2 over the block's voxels (the second
+// sweep's mean squared update) goes to part[block].
+// Block order: workgroups go to the 8 XCDs round-robin; the logical block (tile, chunk),
+// chunk fastest, is dealt out so that each XCD runs a contiguous range of them, whose
+// z-halo and x-y halo planes its L2 then shares.
+template <int TX, int TY, bool RESID>
+__global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                 const float *__restrict__ Q, int nx, int ny, int hz, int zc,
+                                                 int tiles_x, int nchunks, JacobiConst c, double *__restrict__ part) {
+    constexpr int W2 = TX + 4, H2 = TY + 4, N2 = W2 * H2;
+    constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
+    constexpr int N0 = TX * TY;
+    constexpr int R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256, R0 = (N0 + 255) / 256;
+    __shared__ float Li[3][N2];
+    __shared__ float Lq[3][N1];
+    __shared__ float L1[3][N1];
+    const int tid = threadIdx.x;
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb >> 3, rem = nb & 7, xcd = b & 7;
+    const int lb = xcd * per + min(xcd, rem) + (b >> 3);
+    const int tile = lb / nchunks, chunk = lb - tile * nchunks;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int za = chunk * zc, zb = min(hz, za + zc);
+    const size_t sz = (size_t)nx * ny;
+
+    // Per thread, everything about its cells that does not change along the march: the
+    // plane offsets of its R2 (Ti staging) and R1 (Q staging) cells, or -1 outside the
+    // domain; for its R1 cells (first sweep) their place in the R2 ring planes and what
+    // the sweep does there; for its tile cells (second sweep) their place in the R1 ring
+    // planes, plane offset and what the sweep does there.  Cell kinds: 0 outside the
+    // domain, 1 an oven (kept), 2 updated; bits 2-3 the x class + 1, bit 4 below the top.
+    int off2[R2], off1[R1], c2r[R1], k1[R1], c1r[R0], g0[R0], k0[R0];
+#pragma unroll
+    for (int r = 0; r < R2; r++) {
+        const int t = tid + 256 * r;
+        const int row = t / W2, col = t - row * W2;
+        const int gx = x0 - 2 + col, gy = y0 - 2 + row;
+        off2[r] = (t < N2 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+    }
+    auto kind = [&](int gx, int gy) {
+        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) return 0;
+        return (jacobi_updates(c, gx, gy, nx, ny) ? 2 : 1) | ((jacobi_cls(gx, nx) + 1) << 2) | ((gy < ny - 1) << 4);
+    };
+#pragma unroll
+    for (int r = 0; r < R1; r++) {
+        const int t = tid + 256 * r;
+        const int row = t / W1, col = t - row * W1;
+        const int gx = x0 - 1 + col, gy = y0 - 1 + row;
+        off1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+        c2r[r] = (row + 1) * W2 + col + 1;                 // the same cell in R2
+        k1[r] = t < N1 ? kind(gx, gy) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R0; r++) {
+        const int t = tid + 256 * r;
+        const int row = t / TX, col = t - row * TX;
+        const int gx = x0 + col, gy = y0 + row;
+        c1r[r] = (row + 1) * W1 + col + 1;                 // the same cell in R1
+        g0[r] = gy * nx + gx;
+        k0[r] = t < N0 ? kind(gx, gy) : 0;
+    }
+    float pre[R2], qpre[R1];
+    auto fetch = [&](int k) {                        // Ti plane k over R2, Q plane k-1 over R1
+#pragma unroll
+        for (int r = 0; r < R2; r++) pre[r] = (k >= 0 && k < hz && off2[r] >= 0) ? Ti[(size_t)k * sz + off2[r]] : 0.f;
+#pragma unroll
+        for (int r = 0; r < R1; r++)
+            qpre[r] = (k >= 1 && k <= hz && off1[r] >= 0) ? Q[(size_t)(k - 1) * sz + off1[r]] : 0.f;
+    };
+    double res = 0.0;
+    fetch(za - 2);
+    for (int k = za - 2; k <= zb + 1; k++) {
+        // (a) stage Ti plane k and Q plane k-1, then load the next ones
+        if (k >= 0 && k < hz) {
+            float *dst = Li[(k + 3) % 3];
+#pragma unroll
+            for (int r = 0; r < R2; r++) {
+                const int t = tid + 256 * r;
+                if (t < N2) dst[t] = pre[r];
+            }
+        }
+        if (k >= 1 && k <= hz) {
+            float *dst = Lq[(k + 2) % 3];
+#pragma unroll
+            for (int r = 0; r < R1; r++) {
+                const int t = tid + 256 * r;
+                if (t < N1) dst[t] = qpre[r];
+            }
+        }
+        if (k + 1 <= zb + 1) fetch(k + 1);
+        __syncthreads();
+        // (b) first sweep, plane p1 = k - 1, over R1
+        const int p1 = k - 1;
+        if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
+            const float *lm = Li[(p1 + 2) % 3], *l0 = Li[(p1 + 3) % 3], *lp = Li[(p1 + 4) % 3];
+            const float *lq = Lq[(p1 + 3) % 3];
+            float *dst = L1[(p1 + 3) % 3];
+            const int zc1 = jacobi_cls(p1, hz);
+#pragma unroll
+            for (int r = 0; r < R1; r++) {
+                const int t = tid + 256 * r;
+                if (t < N1) {
+                    const int c2 = c2r[r], kd = k1[r];
+                    float v = 0.f;
+                    if ((kd & 3) == 2) {
+                        v = jacobi_voxel(
+                            c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc1,
+                            [&](int dx, int dy, int dz) {
+                                const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                                return pl[c2 + dy * W2 + dx];
+                            },
+                            [&] { return lq[t]; });
+                    } else if ((kd & 3) == 1) {
+                        v = l0[c2];                                      // ovens keep their value
+                    }
+                    dst[t] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // (c) second sweep, plane p2 = k - 2, over the tile
+        const int p2 = k - 2;
+        if (p2 >= za && p2 < zb) {
+            const float *lm = L1[(p2 + 2) % 3], *l0 = L1[(p2 + 3) % 3], *lp = L1[(p2 + 4) % 3];
+            const float *lq = Lq[(p2 + 3) % 3];
+            const int zc2 = jacobi_cls(p2, hz);
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const int c1 = c1r[r], kd = k0[r];
+                if ((kd & 3) == 2) {
+                    const float out = jacobi_voxel(
+                        c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2,
+                        [&](int dx, int dy, int dz) {
+                            const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                            return pl[c1 + dy * W1 + dx];
+                        },
+                        [&] { return lq[c1]; });
+                    Tf[(size_t)p2 * sz + g0[r]] = out;
+                    if (RESID) {
+                        const double d = (double)(out - l0[c1]);
+                        res += d * d;
+                    }
+                }
+            }
+        }
+    }
+    if (RESID) {
+        for (int off = 32; off > 0; off >>= 1) res += __shfl_down(res, off, 64);
+        __shared__ double w[4];
+        if ((tid & 63) == 0) w[tid >> 6] = res;
+        __syncthreads();
+        if (tid == 0) part[lb] = w[0] + w[1] + w[2] + w[3];
     }
 }
 
@@ -361,6 +568,68 @@ hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int
         hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
     }
     return hipGetLastError();
+}
+
+// Layouts of the two-sweep pass: a tile shape and how many waves of workgroups the device
+// runs it in.  The z-chunk depth is chosen so that the workgroups (tiles x chunks) fill
+// that many waves of the device's resident slots (CUs x workgroups per CU, queried for
+// the kernel): with one chunk depth for every grid, a last partial wave of workgroups ran
+// nearly alone and cost as much as a full one.
+static constexpr int kJ2Shape[3][2] = {{64, 16}, {64, 8}, {128, 8}};
+static constexpr int kJ2Waves[3] = {1, 2, 4};
+
+template <int TX, int TY>
+static int j2_slots() {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_jacobi2<TX, TY, false>, 256, 0) !=
+        hipSuccess)
+        return 0;
+    return cus * per_cu;
+}
+
+int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan) {
+    if (variant < 0 || variant >= kJacobi2VariantCount || !plan || nx < 2 || ny < 2 || hz < 2) return -1;
+    const int shape = variant / 3, waves = kJ2Waves[variant % 3];
+    const int tx = kJ2Shape[shape][0], ty = kJ2Shape[shape][1];
+    const int slots = shape == 0 ? j2_slots<64, 16>() : shape == 1 ? j2_slots<64, 8>() : j2_slots<128, 8>();
+    if (slots <= 0) return -1;
+    Jacobi2Plan p{};
+    p.shape = shape;
+    p.tiles_x = (nx + tx - 1) / tx;
+    const int tiles = p.tiles_x * ((ny + ty - 1) / ty);
+    const int chunks = std::max(1, waves * slots / tiles);
+    p.zc = std::max(4, (hz + chunks - 1) / chunks);
+    p.nchunks = (hz + p.zc - 1) / p.zc;
+    p.blocks = tiles * p.nchunks;
+    *plan = p;
+    return 0;
+}
+
+template <int TX, int TY>
+static hipError_t j2_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const Jacobi2Plan &p,
+                            const JacobiConst &c, double *part, hipStream_t st) {
+    if (part)
+        hipLaunchKernelGGL((k_jacobi2<TX, TY, true>), dim3(p.blocks), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, p.zc,
+                           p.tiles_x, p.nchunks, c, part);
+    else
+        hipLaunchKernelGGL((k_jacobi2<TX, TY, false>), dim3(p.blocks), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, p.zc,
+                           p.tiles_x, p.nchunks, c, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
+                          double *part, int nparts, hipStream_t st, const Jacobi2Plan &p) {
+    if (p.blocks <= 0 || p.zc < 1 || hz < 2 || ny < 2 || nx < 2) return hipErrorInvalidValue;
+    if (p.nchunks * p.zc < hz || p.tiles_x * kJ2Shape[p.shape][0] < nx) return hipErrorInvalidValue;
+    if (part && nparts < p.blocks) return hipErrorInvalidValue;
+    switch (p.shape) {
+        case 0: return j2_launch<64, 16>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
+        case 1: return j2_launch<64, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
+        case 2: return j2_launch<128, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st) {

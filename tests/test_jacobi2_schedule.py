@@ -1,0 +1,111 @@
+"""CPU check of the two-sweep Jacobi pass's schedule (k_jacobi2 in
+shg_focused_gaussian_beams_amd/csrc/thermal_kernels.hip), restated step for step: the
+XCD-dealt block order, the tile / z-chunk decomposition, the LDS rings of three planes of
+Ti (halo 2) and of the first sweep (halo 1), the staging at march step k, the first sweep
+of plane k-1 and the second of plane k-2.  A toy stencil that reads every in-domain
+neighbour stands in for the Jacobi formulas (which the kernel shares with the
+single-sweep kernel); the schedule must give two plain sweeps exactly, read only staged
+ring entries, and stay inside its rings — on grids that are not multiples of the tile,
+with chunks that do not divide the depth, with both boundary kinds.  The bit-exactness of
+the real formulas is a GPU test (tests/test_gpu_jacobi2.py).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+def sweep(T, Q, upd):
+    hz, ny, nx = T.shape
+    out = T.copy()
+    for z, y, x in itertools.product(range(hz), range(ny), range(nx)):
+        if not upd(x, y, nx, ny): continue
+        out[z, y, x] = voxel(lambda dx, dy, dz: T[z+dz, y+dy, x+dx], x, y, z, nx, ny, hz) + Q[z, y, x]
+    return out
+
+W = {(1,0,0): .11, (-1,0,0): .13, (0,1,0): .17, (0,-1,0): .19, (0,0,1): .23, (0,0,-1): .29}
+def voxel(ld, x, y, z, nx, ny, hz):
+    v = 0.0
+    for (dx, dy, dz), w in W.items():
+        if 0 <= x+dx < nx and 0 <= y+dy < ny and 0 <= z+dz < hz:
+            v += w * ld(dx, dy, dz)
+    return v
+
+def emulate(Ti, Q, TX, TY, zc, upd):
+    hz, ny, nx = Ti.shape
+    W2, H2, N2 = TX+4, TY+4, (TX+4)*(TY+4)
+    W1, H1, N1 = TX+2, TY+2, (TX+2)*(TY+2)
+    tiles_x, tiles_y = -(-nx//TX), -(-ny//TY)
+    nch = -(-hz//zc)
+    nb = tiles_x*tiles_y*nch
+    Tf = np.full_like(Ti, np.nan); Tf[:, 0, :] = Ti[:, 0, :]
+    for y in range(ny):
+        for x in range(nx):
+            if not upd(x, y, nx, ny): Tf[:, y, x] = Ti[:, y, x]
+    seen = set()
+    for b in range(nb):
+        per, rem, xcd = nb >> 3, nb & 7, b & 7
+        lb = xcd*per + min(xcd, rem) + (b >> 3)
+        assert 0 <= lb < nb and lb not in seen; seen.add(lb)
+        tile, chunk = divmod(lb, nch)
+        ty, tx = divmod(tile, tiles_x)
+        x0, y0 = tx*TX, ty*TY
+        za, zb = chunk*zc, min(hz, chunk*zc + zc)
+        Li = np.full((3, N2), np.nan); L1 = np.full((3, N1), np.nan); Lq = np.full((3, N1), np.nan)
+        for k in range(za-2, zb+2):
+            if 0 <= k < hz:
+                for t in range(N2):
+                    row, col = divmod(t, W2); gx, gy = x0-2+col, y0-2+row
+                    Li[(k+3) % 3, t] = Ti[k, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+            if 1 <= k <= hz:
+                for t in range(N1):
+                    row, col = divmod(t, W1); gx, gy = x0-1+col, y0-1+row
+                    Lq[(k+2) % 3, t] = Q[k-1, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+            p1 = k-1
+            if max(za-1, 0) <= p1 <= min(zb, hz-1):
+                for t in range(N1):
+                    row, col = divmod(t, W1); gx, gy = x0-1+col, y0-1+row
+                    c2 = (row+1)*W2 + col+1
+                    v = 0.0
+                    if 0 <= gx < nx and 0 <= gy < ny:
+                        if upd(gx, gy, nx, ny):
+                            def ld(dx, dy, dz):
+                                i = c2 + dy*W2 + dx; assert 0 <= i < N2
+                                val = Li[(p1+dz+3) % 3, i]; assert not np.isnan(val), (b, k, gx, gy, dx, dy, dz); return val
+                            q = Lq[(p1+3) % 3, t]; assert not np.isnan(q)
+                            v = voxel(ld, gx, gy, p1, nx, ny, hz) + q
+                        else:
+                            v = Li[(p1+3) % 3, c2]
+                    L1[(p1+3) % 3, t] = v
+            p2 = k-2
+            if za <= p2 < zb:
+                for t in range(TX*TY):
+                    row, col = divmod(t, TX); gx, gy = x0+col, y0+row
+                    c1 = (row+1)*W1 + col+1
+                    if gx < nx and gy < ny and upd(gx, gy, nx, ny):
+                        def ld(dx, dy, dz):
+                            i = c1 + dy*W1 + dx; assert 0 <= i < N1
+                            val = L1[(p2+dz+3) % 3, i]; assert not np.isnan(val); return val
+                        q = Lq[(p2+3) % 3, c1]; assert not np.isnan(q)
+                        Tf[p2, gy, gx] = voxel(ld, gx, gy, p2, nx, ny, hz) + q
+    assert len(seen) == nb
+    return Tf
+
+def _top(x, y, nx, ny):
+    return y >= 1
+
+
+def _oven(x, y, nx, ny):
+    return 1 <= x < nx - 1 and 1 <= y < ny - 1
+
+
+@pytest.mark.parametrize("nx,ny,hz,TX,TY,zc,bc", [(20, 13, 11, 8, 4, 3, "top"), (16, 9, 7, 8, 4, 4, "oven"),
+                                                   (12, 12, 2, 16, 8, 5, "top"), (24, 10, 9, 8, 4, 2, "top")])
+def test_two_sweep_schedule_equals_two_sweeps(nx, ny, hz, TX, TY, zc, bc):
+    upd = _top if bc == "top" else _oven
+    rng = np.random.default_rng(nx * 100 + hz)
+    Ti = rng.random((hz, ny, nx))
+    Q = 0.01 * rng.random((hz, ny, nx))
+    ref = sweep(sweep(Ti, Q, upd), Q, upd)
+    em = emulate(Ti, Q, TX, TY, zc, upd)
+    assert not np.isnan(em).any()
+    assert np.array_equal(em, ref)
