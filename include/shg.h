@@ -197,6 +197,24 @@ int shg_set_thermal_log(shg_ctx *ctx, shg_thermal_log_fn fn, void *user);
  * sum((Tf-Ti)^2)/(nx*ny*heat_nz) of the last sweep in FP64 (reducedSum, Tfield.h:13-28). */
 int shg_jacobi(shg_ctx *ctx, int64_t nsweeps, double *mean_sq_update);
 
+/* No counterpart in the reference: shg_jacobi for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Runs `nsweeps` Jacobi sweeps of the heat grid described by `p` (nx == ny in 64..512,
+ * heat_nz or nz slices, the box, k_th, h_conv, t_inf, thermal_bc) on caller-owned device
+ * buffers, with no context: T (nx*ny*heat_nz floats, IDX layout) in and out, T_work the
+ * same size (clobbered), Q the heat source.  `layout` is a Jacobi layout of
+ * shg_set_jacobi_layout, or -1 for single sweeps; every layout gives the same bits.  If
+ * `mean_sq_update` (a device double) is not NULL it receives the last sweep's mean
+ * squared update, as shg_jacobi returns it.  Work is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting.  It calls no
+ * allocating routine: the residual's partial sums live in a fixed 2 MiB array of the
+ * library on each device (resolved on the device's first call), which limits the heat
+ * grid to 2^18 single-sweep partials, 512 x 512 x 256 voxels (larger grids with a
+ * residual are rejected with SHG_E_INVALID).  Every call on a device shares that array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+int shg_jacobi_noalloc(const shg_params *p, float *T, float *T_work, const float *Q, int64_t nsweeps, int layout,
+                       double *mean_sq_update, void *stream);
+
 /* Temperature and heat source (nx*ny*heat_nz floats, IDX(x,y,z) = z*nx*ny + y*nx + x).
  * shg_set_temperature also rebuilds the propagators at the grid-centre temperature. */
 int shg_get_temperature(shg_ctx *ctx, float *T);

@@ -99,6 +99,7 @@ def load_library(path=LIB_PATH):
         "shg_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_get_variants": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "shg_set_jacobi_layout": (ctypes.c_int, [vp, ctypes.c_int]),
+        "shg_jacobi_noalloc": (ctypes.c_int, [P, vp, vp, vp, ctypes.c_int64, ctypes.c_int, vp, vp]),
         "shg_get_jacobi_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "shg_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
         "shg_batch_destroy": (None, [vp]),
@@ -131,6 +132,14 @@ def _fptr(a):
 
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def jacobi_noalloc(params, T_ptr, T_work_ptr, Q_ptr, nsweeps, layout=-1, mean_sq_update_ptr=None, stream=None):
+    """shg_jacobi_noalloc: Jacobi sweeps on caller-owned device buffers (integer device
+    addresses, e.g. torch tensors' data_ptr()) with no device allocation; enqueued on
+    `stream` (a raw hipStream_t handle) and not waited for."""
+    _check(load_library().shg_jacobi_noalloc(ctypes.byref(params), T_ptr, T_work_ptr, Q_ptr, nsweeps, layout,
+                                             mean_sq_update_ptr, stream))
 
 
 def make_pump(params, t_centre=None):
