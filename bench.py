@@ -47,7 +47,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256, the tuned layouts) from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy
 # of known size.
-TRAFFIC_THERMAL_STEP = 6882 * 1024
+TRAFFIC_THERMAL_STEP = 5554 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 # VALU lane-operations per plain cell-step of the batch's passes (rocprofv3 SQ_INSTS_VALU
 # x 64 per cell: 393 row pass + 141 column pass at 512^2) and the chip's FP32 issue rate

@@ -68,7 +68,6 @@ struct shg_ctx {
     int tcur = 0;                    // index of the buffer holding the latest temperature
     float *Q = nullptr, *DKint = nullptr;
     float2 *ntab = nullptr;          // {n_p(T), n_s(T)} per heat voxel, refreshed per march
-    float4 *ctab = nullptr;          // {kp, ks, dk, tpa} per heat voxel, refreshed per march
     double *jpart = nullptr, *jres = nullptr;
     int njpart = 0;
     shg::JacobiConst jc{};

@@ -280,7 +280,6 @@ int enqueue_thermal_segment(shg_ctx *c, int s0, int s1, const shg::TapeArgs *tp,
     const int n = c->n, rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
     shg::ThermRowArgs a{};
     a.k = c->tc;
-    a.ct = c->ctab;
     const int tflag = tp ? shg::TROW_TAPE : 0;
     if (tp) {
         a.tape_s0 = s0;
@@ -351,7 +350,7 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
     // 1. forward at the fixed temperature: checkpoints (fields, DKint) of every segment,
     //    the last segment taped, the output powers
     AHIP_TRY(hipEventRecord(c->e0, st));
-    AHIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)n3, c->tc, st));
+    AHIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)n3, c->tc, st));
     AHIP_TRY(shg::launch_copy_cells(c->U, c->U0, (int64_t)n2, st));
     for (int j = 0; j < nseg; j++) {
         const int s0 = j * K, s1 = std::min(M, s0 + K);

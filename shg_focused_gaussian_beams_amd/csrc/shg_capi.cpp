@@ -137,7 +137,6 @@ shg::ThermStep therm_step(const shg_ctx *c, int s);
 static shg::ThermRowArgs therm_mid_args(const shg_ctx *c) {
     shg::ThermRowArgs a{};
     a.k = c->tc;
-    a.ct = c->ctab;
     a.prev = therm_step(c, 0);
     a.cur = therm_step(c, 1);
     a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
@@ -156,7 +155,7 @@ int tune_variants(shg_ctx *c, int reps) {
     const int nrow = c->p.thermal ? shg::kThermRowVariantCount : shg::kRowVariantCount;
     shg::ThermRowArgs a{};
     if (c->p.thermal) {
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
         a = therm_mid_args(c);
     }
     float best = 1e30f;
@@ -205,7 +204,6 @@ int enqueue_march_thermal(shg_ctx *c, float4 *snaps = nullptr) {
     const size_t n2 = (size_t)c->n * c->n;
     shg::ThermRowArgs a{};
     a.k = c->tc;
-    a.ct = c->ctab;
     a.cur = therm_step(c, 0);
     a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
     const float2 *NT = c->ntab;
@@ -217,7 +215,11 @@ int enqueue_march_thermal(shg_ctx *c, float4 *snaps = nullptr) {
         float4 *snap = c->Uout;
         if (s < nz - 1) {
             a.cur = therm_step(c, s + 1);
-            a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV | shg::TROW_NCUR | shg::TROW_BWD;
+            // the heat source of slice(s) only from the last step that maps to it: the
+            // reference's setQ after every step leaves the same final Q, since the later
+            // steps of a slice overwrite the earlier ones (heat grid coarser than the march)
+            a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_NCUR | shg::TROW_BWD;
+            if (a.prev.slice != a.cur.slice) a.flags |= shg::TROW_QPREV;
             if (s + 1 == nz - 1 || snaps) a.flags |= shg::TROW_SNAP;
             if (snaps) snap = snaps + (size_t)(s + 1) * n2;     // the state before step s+1
         } else {
@@ -252,7 +254,7 @@ void release(shg_ctx *c) {
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
                       (void *)c->d_coef, (void *)c->d_desc,
                       (void *)c->red, (void *)c->T[0], (void *)c->T[1], (void *)c->Q, (void *)c->DKint,
-                      (void *)c->ntab, (void *)c->ctab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
+                      (void *)c->ntab, (void *)c->jpart, (void *)c->jres, (void *)c->adj.lam, (void *)c->adj.ckpt,
                       (void *)c->adj.a2, (void *)c->adj.spec, (void *)c->adj.g1, (void *)c->adj.g2, (void *)c->adj.gH,
                       (void *)c->adj.gs, (void *)c->adj.qxy, (void *)c->adj.dlam, (void *)c->adj.Ap, (void *)c->adj.As,
                       (void *)c->adj.Gc, (void *)c->adj.Dc, (void *)c->adj.dpart, (void *)c->adj.dsum,
@@ -417,7 +419,7 @@ int timed_march(shg_ctx *c) {
     if (c->use_graph && !c->gexec) RC_TRY(shg_internal_capture(c->stream, enqueue, &c->gexec));
     HIP_TRY(hipEventRecord(c->e0, c->stream));
     if (c->p.thermal)
-        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
+        HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
     if (c->use_graph) {
         HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
     } else {
@@ -540,7 +542,6 @@ int shg_create(const shg_params *p, shg_ctx **out) {
                 hipMalloc(&c->T[0], n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->T[1], n3 * sizeof(float)) != hipSuccess ||
                 hipMalloc(&c->Q, n3 * sizeof(float)) != hipSuccess || hipMalloc(&c->DKint, ncell * sizeof(float)) != hipSuccess ||
                 hipMalloc(&c->ntab, n3 * sizeof(float2)) != hipSuccess ||
-                hipMalloc(&c->ctab, n3 * sizeof(float4)) != hipSuccess ||
                 hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
                 hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
                 return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
@@ -549,6 +550,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
             c->jc.bc = p->thermal_bc;
             shg::ThermConst &tc = c->tc;
             tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
+            tc.rlp = 1.0f / k.lp; tc.rls = 1.0f / k.ls;
             tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
             tc.tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
             tc.q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
@@ -647,7 +649,7 @@ int shg_forward_snapshots(shg_ctx *c, float *fh_steps, float *sh_steps) {
             HIP_TRY(hipMemcpyAsync(c->U, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
             HIP_TRY(hipMemcpyAsync(snaps, c->U0, n2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
             if (c->p.thermal) {
-                HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)n2 * c->hz, c->tc, c->stream));
+                HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)n2 * c->hz, c->tc, c->stream));
                 RC_TRY(enqueue_march_thermal(c, snaps));
                 HIP_TRY(hipMemcpyAsync(c->Uout, snaps + (size_t)(nz - 1) * n2, n2 * sizeof(float4),
                                        hipMemcpyDeviceToDevice, c->stream));
@@ -777,7 +779,7 @@ int shg_profile_kernels(shg_ctx *c, int reps, double *row_ms, double *col_ms) {
     return shg_internal_guard(__func__, [&]() -> int {
         if (!c || reps <= 0 || !row_ms || !col_ms) return fail(SHG_E_INVALID, "shg_profile_kernels: bad argument");
         if (c->p.thermal)
-            HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, c->ctab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
+            HIP_TRY(shg::launch_index_table(c->T[c->tcur], c->ntab, (int64_t)c->n * c->n * c->hz, c->tc, c->stream));
         const int rv = c->row_v < 0 ? 0 : c->row_v, cv = c->col_v < 0 ? 0 : c->col_v;
         const int mode = c->steps > 1 ? shg::ROW_MID : shg::ROW_END;
         const shg::ThermRowArgs a = c->p.thermal ? therm_mid_args(c) : shg::ThermRowArgs{};

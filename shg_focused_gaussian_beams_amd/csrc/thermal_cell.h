@@ -33,12 +33,25 @@ __device__ __forceinline__ float index_dev(int crystal, float L, float T) {
     return (float)sqrt(s);
 }
 
+// a / d for a divisor d known in advance, with r = RN(1/d): q = RN(a r) is within an ulp
+// of a/d, the FMA gives the exact remainder a - q d, and one more FMA step returns the
+// correctly rounded quotient (Markstein's theorem; barring overflow and underflow, far
+// from the values here).  Three operations instead of the IEEE division's sequence,
+// bit-identical to it.
+__device__ __forceinline__ float div_const(float a, float d, float r) {
+    const float q = a * r;
+    const float e = __builtin_fmaf(-q, d, a);
+    return __builtin_fmaf(e, r, q);
+}
+
 struct CellPhys {
     float np, ns, kp, ks, dk, tpa;
 };
 
 // The coupling magnitudes, the phase mismatch (src/headers/PhaseMatching.h:47) and the
-// TPA coefficient from the cell's indices, with the reference's float operations.
+// TPA coefficient from the cell's indices, with the reference's float operations (the
+// divisions by the wavelengths through their reciprocals, bit-identical).  The row pass
+// evaluates them per cell from the march's {n_p, n_s} table.
 __device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float2 nn) {
 #pragma clang fp contract(off)
     CellPhys c;
@@ -46,20 +59,8 @@ __device__ __forceinline__ CellPhys cell_phys(const ThermConst &k, float2 nn) {
     c.ns = nn.y;
     c.kp = ((1.0f * k.pi) * k.dq) / (c.np * k.lp);
     c.ks = ((0.5f * k.pi) * k.dq) / (c.ns * k.ls);
-    c.dk = (2.0f * k.pi) * (((2.0f * c.np) / k.lp - c.ns / k.ls) + k.inv_period);
+    c.dk = (2.0f * k.pi) * ((div_const(2.0f * c.np, k.lp, k.rlp) - div_const(c.ns, k.ls, k.rls)) + k.inv_period);
     c.tpa = (float)(0.5 * (k.tpa_base * (double)c.ns));
-    return c;
-}
-
-// The same values read from the per-march table: nn = {n_p, n_s}, cf = {kp, ks, dk, tpa}.
-__device__ __forceinline__ CellPhys cell_from_table(float2 nn, float4 cf) {
-    CellPhys c;
-    c.np = nn.x;
-    c.ns = nn.y;
-    c.kp = cf.x;
-    c.ks = cf.y;
-    c.dk = cf.z;
-    c.tpa = cf.w;
     return c;
 }
 

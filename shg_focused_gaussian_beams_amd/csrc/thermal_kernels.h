@@ -26,6 +26,7 @@ struct JacobiConst {
 // (src/headers/Tfield.h:277-290).
 struct ThermConst {
     float pi, dq, lp, ls, inv_period;             // dk = 2 pi (2 n_p/lp - n_s/ls + 1/Lambda)
+    float rlp, rls;                               // RN(1/lp), RN(1/ls)
     float lossP, lossS, h6;                       // 0.5 alpha_p, 0.5 alpha_s, RK4 step / 6
     double tpa_base;                              // beta_s * 0.5 * eps0 * c   (times n_s(T))
     double q_p, q_s, q_i;                         // alpha_p*0.5*c*eps0, alpha_s*0.5*c*eps0, 0.5*c*eps0
@@ -54,7 +55,6 @@ enum ThermRowFlags : int {
 struct ThermRowArgs {
     ThermStep prev, cur;
     ThermConst k;
-    const float4 *ct;                             // {kp, ks, dk, tpa} per heat voxel (launch_index_table)
     int flags;
     int tape_s0;
     float4 *tape_a2;
@@ -73,11 +73,10 @@ int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan);   // 0
 hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                           double *part, int nparts, hipStream_t st, const Jacobi2Plan &plan);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
-// Per heat voxel: tab = {n_p(T), n_s(T)} and ct = {kp, ks, dk, tpa} with the reference's
-// float operations (T is frozen during one march, so the row pass reads them instead of
-// evaluating the FP64 Sellmeier fits and four float divisions per cell and step).
-hipError_t launch_index_table(const float *T, float2 *tab, float4 *ct, int64_t n, const ThermConst &k,
-                              hipStream_t st);
+// Per heat voxel tab = {n_p(T), n_s(T)} with the reference's typing (T is frozen during
+// one march, so the row pass reads them instead of evaluating the FP64 Sellmeier fits per
+// cell and step).
+hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, const ThermConst &k, hipStream_t st);
 // Row-pass layouts per size (the tuner times all of them): variants 0-3 are <E, W> row
 // layouts, 4-5 wide layouts (one row per workgroup, one cell per thread or two), 6 the
 // split layout (the two fields transformed by two waves side by side).

@@ -21,29 +21,18 @@
 namespace shg {
 
 // ---------------------------------------------------------------------------
-// One Jacobi sweep of heatEquationTopOpen (src/headers/Tfield.h:113-221) at a voxel with
-// y >= 1 of x class xc and z class zc (jacobi_cls) below the top (yi) or on it: the new
-// value from the old values of its neighbours, ld(dx, dy, dz) = Ti at (x+dx, y+dy, z+dz),
-// and q() = Q at the voxel.  Every formula keeps the
-// reference's float/double typing and operation order, with contraction off, so that
-// whichever kernel calls it, a sweep is the reference's float sweep bit for bit.  The
-// voxels kernelHeatEquationOvenSurrounded updates (:224-258: inner points, z = 0 and
-// z = L faces inside the lateral ovens) use the same inner and z-face formulas.
-// a / d for a divisor d known in advance, with r = RN(1/d): q = RN(a r) is within an ulp
-// of a/d, the FMA gives the exact remainder a - q d, and one more FMA step returns the
-// correctly rounded quotient (Markstein's theorem; barring overflow and underflow, which
-// the heat equation's values stay far from).  Three operations instead of the IEEE
-// division's sequence, bit-identical to it.
-__device__ __forceinline__ float div_const(float a, float d, float r) {
-    const float q = a * r;
-    const float e = __builtin_fmaf(-q, d, a);
-    return __builtin_fmaf(e, r, q);
-}
-
 // Boundary class of a coordinate along x or z: 0 inside, +1 on the low face (its one
 // neighbour at +1), -1 on the high face (neighbour at -1).
 __device__ __forceinline__ int jacobi_cls(int i, int n) { return i == 0 ? 1 : (i == n - 1 ? -1 : 0); }
 
+// One Jacobi sweep of heatEquationTopOpen (src/headers/Tfield.h:113-221) at a voxel with
+// y >= 1 of x class xc and z class zc (jacobi_cls) below the top (yi) or on it: the new
+// value from the old values of its neighbours, ld(dx, dy, dz) = Ti at (x+dx, y+dy, z+dz),
+// and q() = Q at the voxel.  Every formula keeps the reference's float/double typing and
+// operation order, with contraction off, so that whichever kernel calls it, a sweep is
+// the reference's float sweep bit for bit.  The voxels kernelHeatEquationOvenSurrounded
+// updates (:224-258: inner points, z = 0 and z = L faces inside the lateral ovens) use the
+// same inner and z-face formulas.
 template <class LD, class LQ>
 __device__ __forceinline__ float jacobi_voxel(const JacobiConst &c, int xc, bool yi, int zc, LD ld, LQ q) {
 #pragma clang fp contract(off)
@@ -304,19 +293,15 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
-// Refractive indices of every heat voxel, {n_p(T), n_s(T)}, and the coupling constants,
-// phase mismatch and TPA coefficient derived from them: the temperature is frozen during
-// a march, so the FP64 Sellmeier fits and the float divisions run once per voxel instead
-// of twice per cell per row pass (the values are the same floats).
-__global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T, float2 *__restrict__ tab,
-                                                     float4 *__restrict__ ct, int64_t n, ThermConst k) {
+// Refractive indices of every heat voxel, {n_p(T), n_s(T)}: the temperature is frozen
+// during a march, so the FP64 Sellmeier fits run once per voxel instead of twice per cell
+// and step.
+__global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T, float2 *__restrict__ tab, int64_t n,
+                                                     ThermConst k) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float t = T[i];
-    const float2 nn = make_float2(index_dev(k.crystal, k.lp, t), index_dev(k.crystal, k.ls, t));
-    const CellPhys c = cell_phys(k, nn);
-    tab[i] = nn;
-    ct[i] = make_float4(c.kp, c.ks, c.dk, c.tpa);
+    tab[i] = make_float2(index_dev(k.crystal, k.lp, t), index_dev(k.crystal, k.ls, t));
 }
 
 // Row pass of the THERMAL z-march (flags in ThermRowFlags):
@@ -333,12 +318,13 @@ __global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T
 // One cell of the pass between the transforms (idx: cell index in the plane), from its
 // already loaded phase accumulator and index-table entries.
 __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx, size_t n2, size_t sprev, float dkprev,
-                                               float2 nnp, float4 cfp, float2 nnc, float4 cfc,
-                                               float4 *__restrict__ Uout, float *__restrict__ Q,
-                                               float *__restrict__ DKint, const ThermRowArgs &a) {
+                                               float2 nnp, float2 nnc, bool same, float4 *__restrict__ Uout,
+                                               float *__restrict__ Q, float *__restrict__ DKint,
+                                               const ThermRowArgs &a) {
     const ThermConst &k = a.k;
+    CellPhys cp{};
     if (a.flags & (TROW_NPREV | TROW_QPREV)) {
-        const CellPhys cp = cell_from_table(nnp, cfp);
+        cp = cell_phys(k, nnp);
         if ((a.flags & (TROW_TAPE | TROW_NPREV)) == (TROW_TAPE | TROW_NPREV))
             a.tape_a2[(size_t)(a.prev.step - a.tape_s0) * n2 + idx] = make_float4(P.x, P.y, S.x, S.y);
         if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
@@ -346,7 +332,7 @@ __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx,
     }
     if (a.flags & TROW_SNAP) Uout[idx] = make_float4(P.x, P.y, S.x, S.y);
     if (a.flags & TROW_NCUR) {
-        const CellPhys cc = cell_from_table(nnc, cfc);
+        const CellPhys cc = same ? cp : cell_phys(k, nnc);
         const float dk = dkprev + cc.dk;
         DKint[idx] = dk;
         if (a.flags & TROW_TAPE) a.tape_dk[(size_t)(a.cur.step - a.tape_s0) * n2 + idx] = dk;
@@ -360,11 +346,10 @@ __device__ __forceinline__ void therm_cell(float2 &P, float2 &S, size_t idx, siz
                                            float *__restrict__ Q, float *__restrict__ DKint, const ThermRowArgs &a) {
     const float dkprev = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
+    const bool same = pv && a.prev.slice == a.cur.slice;
     const float2 nnp = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-    const float2 nnc = cu ? NT[scur + idx] : make_float2(0.f, 0.f);
-    const float4 cfp = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 cfc = cu ? a.ct[scur + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
-    therm_cell_pre(P, S, idx, n2, sprev, dkprev, nnp, cfp, nnc, cfc, Uout, Q, DKint, a);
+    const float2 nnc = (cu && !same) ? NT[scur + idx] : nnp;
+    therm_cell_pre(P, S, idx, n2, sprev, dkprev, nnp, nnc, same, Uout, Q, DKint, a);
 }
 
 // Layout <N, E, W> as in pass_cfg.h.  Between the two transforms the thread's E cells
@@ -442,7 +427,6 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
     float dkp[CPT];
     float2 nnp[CPT], nnc[CPT];
-    float4 cfp[CPT], cfc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
     const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
 #pragma unroll
@@ -450,9 +434,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
         const size_t idx = base + tid + q * THREADS;
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
         nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
-        cfc[q] = !cu ? make_float4(0.f, 0.f, 0.f, 0.f) : same ? cfp[q] : a.ct[scur + idx];
     }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
@@ -478,7 +460,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     for (int q = 0; q < CPT; q++) {
         const int cc = tid + q * THREADS, o = lds_pad(cc);
         float2 P = lds[0][o], S = lds[1][o];
-        therm_cell_pre(P, S, base + cc, n2, sprev, dkp[q], nnp[q], cfp[q], nnc[q], cfc[q], Uout, Q, DKint, a);
+        therm_cell_pre(P, S, base + cc, n2, sprev, dkp[q], nnp[q], nnc[q], same, Uout, Q, DKint, a);
         lds[0][o] = P;
         lds[1][o] = S;
     }
@@ -513,7 +495,6 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
     const size_t sprev = (size_t)a.prev.slice * n2, scur = (size_t)a.cur.slice * n2;
     float dkp[CPT];
     float2 nnp[CPT], nnc[CPT];
-    float4 cfp[CPT], cfc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
     const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
 #pragma unroll
@@ -522,9 +503,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
         const size_t idx = base + (c < N ? c : 0);
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        cfp[q] = pv ? a.ct[sprev + idx] : make_float4(0.f, 0.f, 0.f, 0.f);
         nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
-        cfc[q] = !cu ? make_float4(0.f, 0.f, 0.f, 0.f) : same ? cfp[q] : a.ct[scur + idx];
     }
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     for (int c = tid; c < N; c += THREADS) {
@@ -540,7 +519,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
         const int c = tid + q * THREADS;
         if (c < N) {
             float2 P = lds[0][lds_pad(c)], S = lds[1][lds_pad(c)];
-            therm_cell_pre(P, S, base + c, n2, sprev, dkp[q], nnp[q], cfp[q], nnc[q], cfc[q], Uout, Q, DKint, a);
+            therm_cell_pre(P, S, base + c, n2, sprev, dkp[q], nnp[q], nnc[q], same, Uout, Q, DKint, a);
             lds[0][lds_pad(c)] = P;
             lds[1][lds_pad(c)] = S;
         }
@@ -637,10 +616,9 @@ hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_index_table(const float *T, float2 *tab, float4 *ct, int64_t n, const ThermConst &k,
-                              hipStream_t st) {
+hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, const ThermConst &k, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_index_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, tab, ct, n, k);
+    hipLaunchKernelGGL(k_index_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, T, tab, n, k);
     return hipGetLastError();
 }
 
