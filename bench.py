@@ -477,6 +477,55 @@ def bench_inverse(shg, args):
                 projected_s_32x100_concurrent=32 * (th_s + 100 * conc_s))
 
 
+def bench_inverse_full(shg, args):
+    """The large batched inverse design end to end: 32 instances of THERMAL 512x512,
+    nz = 1000, heat grid 512x512x200, P = 50 W, each with its temperature field from a
+    bounded THERMAL outer loop and its seeded +-0.1 rad pump phase mask, 100 ascent
+    iterations each, judged by a plain forward.  The instances run `inverse_instances` at
+    a time (one context and host thread each, full tapes), wave after wave."""
+    import numpy as np
+    import torch
+    from shg_focused_gaussian_beams_amd.inverse import ascend_phase_masks
+    total, per_wave, it = 32, max(1, args.inverse_instances), 100
+    out = []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for w0 in range(0, total, per_wave):
+        ids = list(range(w0, min(total, w0 + per_wave)))
+        ctxs, pumps, masks = [], [], []
+        for i in ids:
+            p = shg.inverse512_params(max_outer=1, jacobi_check=500, jacobi_max=1000, jacobi_tol=1e-6)
+            masks.append(np.random.default_rng(5 + i).uniform(-0.1, 0.1, (p.ny, p.nx)))
+            ctx = shg.Context(p)
+            pump = shg.make_pump(p)
+            ctx.set_fields(pump, np.zeros_like(pump))
+            ctx.thermal_run(pump_from_temperature=True)
+            T = ctx.get_temperature()
+            pumps.append(shg.make_pump(p, float(T[0, p.ny // 2, p.nx // 2])))
+            ctxs.append(ctx)
+        res = ascend_phase_masks(ctxs, pumps, masks, it, checkpoint_interval=999)
+        for i, c, pump, (mask, etas) in zip(ids, ctxs, pumps, res):
+            c.close()
+            judge = shg.Context(shg.inverse512_params(thermal=0))
+            fld = (pump * np.exp(1j * mask)).astype(np.complex64)
+            judge.set_fields(fld, np.zeros_like(fld))
+            judge.forward()
+            out.append(dict(instance=i, eta_first=etas[0], eta_last=etas[-1], eta_plain_judge=judge.powers()[2]))
+            judge.close()
+        torch.cuda.synchronize()
+        log(f"inverse design: instances {ids[0]}..{ids[-1]} done at {time.perf_counter() - t_all:.1f} s")
+    wall = time.perf_counter() - t_all
+    gains = [o["eta_last"] / o["eta_first"] for o in out]
+    return dict(instances=total, iterations=it, instances_concurrent=per_wave, wall_s=wall,
+                s_per_instance_iteration=wall / (total * it),
+                fwd_adj_cell_steps_per_s=total * it * 512 * 512 * 999 / wall,
+                eta_first_mean=float(np.mean([o["eta_first"] for o in out])),
+                eta_last_mean=float(np.mean([o["eta_last"] for o in out])),
+                eta_gain_min=float(min(gains)), eta_gain_max=float(max(gains)),
+                judge_vs_last_max_rel=float(max(abs(o["eta_plain_judge"] - o["eta_last"]) / o["eta_last"] for o in out)),
+                per_instance=out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -493,7 +542,7 @@ def main():
     ap.add_argument("--single-sample", type=int, default=32)
     ap.add_argument("--inverse-instances", type=int, default=8)
     ap.add_argument("--inverse-iterations", type=int, default=2)
-    ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse"], default="all",
+    ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse", "inverse-full"], default="all",
                     help="run one extra workload alone and print its figures (for profiler runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (calibrates rocprof byte counters)")
@@ -524,6 +573,8 @@ def main():
             r = bench_batch(shg, args)
         elif args.section == "inverse":
             r = bench_inverse(shg, args)
+        elif args.section == "inverse-full":
+            r = bench_inverse_full(shg, args)
         else:
             r = bench_adjoint(shg, args)
             r["res"] = {k: v for k, v in r["res"].items()}
