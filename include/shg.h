@@ -300,12 +300,48 @@ int shg_batch_powers(shg_batch *b, double *p_fh_in, double *p_sh_out, double *et
 int64_t shg_batch_cell_steps(const shg_batch *b);
 int shg_batch_last_ms(shg_batch *b, double *ms);
 
+/* ----- Batched THERMAL sweeps ----------------------------------------------------------
+ * Replace the sweep driver's per-point `./SHG` processes (src/SHG.sh:39-81) for a THERMAL
+ * build: `count` instances of the THERMAL branch of Solver::run
+ * (src/headers/Solver.h:289-340) that share the crystal, grid and heat-equation constants
+ * and differ only in power, waist, focus, temp, t_peltier1 and t_peltier2 (others are
+ * rejected with SHG_E_INVALID).  Their outer loops run in lockstep on one device state:
+ * one Jacobi launch sweeps every instance still converging, every instance is checked at
+ * the sweep counts its own run checks at and stops by its own residual, stagnation,
+ * sweep-cap and early-stop rules, and the z-marches of all instances that march in an
+ * outer iteration share each launch.  Each instance's result equals shg_thermal_run's on
+ * a context with the same parameters (pump from the temperature), bit for bit with the
+ * same row and column layouts. */
+typedef struct shg_thermal_batch shg_thermal_batch;
+
+int shg_thermal_batch_create(const shg_params *params, int count, shg_thermal_batch **out);
+void shg_thermal_batch_destroy(shg_thermal_batch *b);
+
+/* SH input (z = 0, nx*ny complex64) of one instance; zero until set. */
+int shg_thermal_batch_set_signal(shg_thermal_batch *b, int index, const float *sh);
+
+/* Row (0..6) and column (0..9) layouts of the batched march; tuned on the first run
+ * otherwise (SHG_NO_TUNE=1: 0, 0).  Get returns -1, -1 while untuned. */
+int shg_thermal_batch_set_variants(shg_thermal_batch *b, int row_variant, int col_variant);
+int shg_thermal_batch_get_variants(const shg_thermal_batch *b, int *row_variant, int *col_variant);
+
+/* Runs every instance (stats: array of `count`, may be null).  jacobi_ms and march_ms are
+ * the batch's device times, the same in every entry. */
+int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats);
+
+/* After shg_thermal_batch_run: one instance's Pump_out/Signal_out fields, its temperature
+ * (nx*ny*heat_nz floats), and the FP64 powers and efficiency of every instance (arrays of
+ * `count`; any may be null). */
+int shg_thermal_batch_get_fields(shg_thermal_batch *b, int index, float *fh, float *sh);
+int shg_thermal_batch_get_temperature(shg_thermal_batch *b, int index, float *T);
+int shg_thermal_batch_powers(shg_thermal_batch *b, double *p_fh_in, double *p_sh_out, double *eta);
+
 /* ----- Launch-shape autotuning (no reference counterpart) ------------------------------
  * The row and column passes exist in several thread layouts (FFT elements per thread,
  * rows or columns per workgroup).  shg_forward / shg_batch_run tune them on first use
  * (timing each (row, column) pair, replayed as the march runs them, on the context's own
  * data) unless the environment sets SHG_NO_TUNE=1; these calls tune explicitly or pin a
- * choice.  Rows 0..4 (THERMAL contexts: 0..6), columns 0..7. */
+ * choice.  Rows 0..4 (THERMAL contexts: 0..6), columns 0..9. */
 int shg_tune(shg_ctx *ctx, int reps, int *row_variant, int *col_variant);
 int shg_set_variants(shg_ctx *ctx, int row_variant, int col_variant);
 /* The layouts the next march runs (-1 while untuned). */

@@ -112,6 +112,16 @@ def load_library(path=LIB_PATH):
         "shg_batch_powers": (ctypes.c_int, [vp, dp, dp, dp]),
         "shg_batch_cell_steps": (ctypes.c_int64, [vp]),
         "shg_batch_last_ms": (ctypes.c_int, [vp, dp]),
+        "shg_thermal_batch_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(vp)]),
+        "shg_thermal_batch_destroy": (None, [vp]),
+        "shg_thermal_batch_set_signal": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "shg_thermal_batch_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "shg_thermal_batch_get_variants": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int),
+                                                          ctypes.POINTER(ctypes.c_int)]),
+        "shg_thermal_batch_run": (ctypes.c_int, [vp, ctypes.POINTER(ThermalStats)]),
+        "shg_thermal_batch_get_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "shg_thermal_batch_get_temperature": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "shg_thermal_batch_powers": (ctypes.c_int, [vp, dp, dp, dp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -396,6 +406,80 @@ class Batch:
         v = ctypes.c_double()
         _check(self.lib.shg_batch_last_ms(self.h, ctypes.byref(v)))
         return v.value
+
+
+class ThermalBatch:
+    """Batched THERMAL sweep points: the THERMAL Solver::run (src/headers/Solver.h:289-340)
+    of every instance, run in lockstep on one device state (replaces the per-point
+    processes of src/SHG.sh:39-81).  Instances share everything but power, waist, focus,
+    temp, t_peltier1 and t_peltier2."""
+
+    def __init__(self, params_list):
+        self.lib = load_library()
+        self.params = [p.copy() for p in params_list]
+        arr = (ShgParams * len(self.params))(*self.params)
+        h = ctypes.c_void_p()
+        _check(self.lib.shg_thermal_batch_create(arr, len(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.shg_thermal_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def heat_shape(self):
+        p = self.params[0]
+        return (p.heat_nz if p.heat_nz > 0 else p.nz, p.ny, p.nx)
+
+    def set_signal(self, i, sh):
+        sh = np.ascontiguousarray(sh, np.complex64)
+        if sh.size != self.params[0].nx * self.params[0].ny:
+            raise ValueError("the signal must hold nx*ny cells")
+        _check(self.lib.shg_thermal_batch_set_signal(self.h, i, _fptr(sh)))
+
+    def set_variants(self, row_variant, col_variant):
+        _check(self.lib.shg_thermal_batch_set_variants(self.h, row_variant, col_variant))
+
+    def variants(self):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.shg_thermal_batch_get_variants(self.h, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    def run(self):
+        """Runs every instance; returns one stats dict per instance."""
+        st = (ThermalStats * len(self.params))()
+        _check(self.lib.shg_thermal_batch_run(self.h, st))
+        return [s.as_dict() for s in st]
+
+    def get_fields(self, i):
+        shape = (self.params[0].ny, self.params[0].nx)
+        fh = np.empty(shape, np.complex64)
+        sh = np.empty(shape, np.complex64)
+        _check(self.lib.shg_thermal_batch_get_fields(self.h, i, _fptr(fh), _fptr(sh)))
+        return fh, sh
+
+    def get_temperature(self, i):
+        t = np.empty(self.heat_shape(), np.float32)
+        _check(self.lib.shg_thermal_batch_get_temperature(self.h, i, _fptr(t)))
+        return t
+
+    def powers(self):
+        n = len(self.params)
+        a, b, c = (np.zeros(n) for _ in range(3))
+        _check(self.lib.shg_thermal_batch_powers(self.h, _dptr(a), _dptr(b), _dptr(c)))
+        return a, b, c
 
 
 class Solver:

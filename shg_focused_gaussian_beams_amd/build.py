@@ -17,7 +17,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function"]
 SOURCES = ["ssfm_kernels.hip", "thermal_kernels.hip", "batch_kernels.hip", "adjoint_kernels.hip",
-           "thermal_adjoint_kernels.hip", "shg_capi.cpp", "shg_batch.cpp", "shg_adjoint.cpp", "shg_noalloc.hip"]
+           "thermal_adjoint_kernels.hip", "shg_capi.cpp", "shg_batch.cpp", "shg_adjoint.cpp", "shg_noalloc.hip",
+           "shg_thermal_batch.cpp"]
 
 
 def _run(cmd):

@@ -91,6 +91,11 @@ struct shg_ctx {
 // Shared helpers of the library's translation units (defined in shg_capi.cpp).
 int shg_internal_fail(int code, const std::string &msg);
 shg::ThermStep shg_internal_therm_step(const shg_ctx *c, int s);
+// The step-independent constants of a context's passes (THERMAL ones only if p.thermal)
+// and the descriptor of THERMAL field step s on a heat grid of hz slices.
+void shg_internal_thermal_consts(const shg_params &p, int hz, shg::NlConst *nl, shg::JacobiConst *jc,
+                                 shg::ThermConst *tc);
+shg::ThermStep shg_internal_therm_step_of(const shg_params &p, int hz, int s);
 int shg_internal_reduce_powers(shg_ctx *c, const float4 *U, double *sum_fh, double *sum_sh);
 double shg_internal_power_of(const shg_ctx *c, int which, double sum, double temp);
 // A copy ordered on `st` that has finished on return.  The library copies through its own

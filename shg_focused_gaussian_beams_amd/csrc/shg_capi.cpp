@@ -47,6 +47,42 @@ bool size_supported(int n) { return n == 64 || n == 128 || n == 256 || n == 512;
 
 int shg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
 
+void shg_internal_thermal_consts(const shg_params &p, int hz, shg::NlConst *nl, shg::JacobiConst *jc,
+                                 shg::ThermConst *tc) {
+    const shg::phys::Consts k(p);
+    const float T = (float)p.temp;
+    const float h = p.scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
+    nl->lossP = (float)(0.5 * (double)k.alpha_p);
+    nl->lossS = (float)(0.5 * (double)k.alpha_s);
+    nl->tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
+    nl->h6 = h / 6.0f;
+    if (!p.thermal) return;
+    const float hdz = shg::phys::heat_dz(k, hz);
+    *jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p.k_th, (float)p.h_conv, (float)p.t_inf);
+    jc->bc = p.thermal_bc;
+    tc->pi = k.pi; tc->dq = k.dq; tc->lp = k.lp; tc->ls = k.ls; tc->inv_period = 1.0f / k.period;
+    tc->rlp = 1.0f / k.lp; tc->rls = 1.0f / k.ls;
+    tc->lossP = nl->lossP; tc->lossS = nl->lossS; tc->h6 = nl->h6;
+    tc->tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
+    tc->q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
+    tc->q_s = (double)k.alpha_s * 0.5 * (double)k.c * (double)k.eps0;
+    tc->q_i = 0.5 * (double)k.c * (double)k.eps0;
+    tc->beta_s = k.beta_s;
+    tc->crystal = p.crystal;
+}
+
+shg::ThermStep shg_internal_therm_step_of(const shg_params &p, int hz, int s) {
+    const shg::phys::Consts k(p);
+    shg::ThermStep st{};
+    const float incs[3] = {0.0f, 0.25f, 0.5f};     // 0.5 * {0, .5, 1} (src/headers/Solver.h:201-212)
+    for (int i = 0; i < 3; i++) st.zfac[i] = (double)(k.dz * incs[i]) + (s + 1.0) * (double)k.dz;
+    st.sp1 = s + 1.0;
+    for (int i = 0; i < 3; i++) st.zr[i] = st.zfac[i] / st.sp1;
+    st.slice = shg::phys::heat_slice(s, p.nz, hz);
+    st.step = s;
+    return st;
+}
+
 hipError_t shg_internal_copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
     const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
     return e == hipSuccess ? hipStreamSynchronize(st) : e;
@@ -184,17 +220,7 @@ int tune_variants(shg_ctx *c, int reps) {
     return SHG_OK;
 }
 
-shg::ThermStep therm_step(const shg_ctx *c, int s) {
-    const shg::phys::Consts k(c->p);
-    shg::ThermStep st{};
-    const float incs[3] = {0.0f, 0.25f, 0.5f};     // 0.5 * {0, .5, 1} (src/headers/Solver.h:201-212)
-    for (int i = 0; i < 3; i++) st.zfac[i] = (double)(k.dz * incs[i]) + (s + 1.0) * (double)k.dz;
-    st.sp1 = s + 1.0;
-    for (int i = 0; i < 3; i++) st.zr[i] = st.zfac[i] / st.sp1;
-    st.slice = shg::phys::heat_slice(s, c->p.nz, c->hz);
-    st.step = s;
-    return st;
-}
+shg::ThermStep therm_step(const shg_ctx *c, int s) { return shg_internal_therm_step_of(c->p, c->hz, s); }
 
 // THERMAL build: all nz steps of Solver::CWES (src/headers/Solver.h:245-276) with the
 // heat-source update after every step; Uout receives the fields before the last step.
@@ -515,11 +541,7 @@ int shg_create(const shg_params *p, shg_ctx **out) {
         const auto tw = shg::phys::twiddle_table(c->n);
         if (shg_internal_copy(c->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return bad(fail(SHG_E_DEVICE, "shg_create: table upload failed"));
-        const float h = p->scheme == SHG_SCHEME_STRANG ? 0.5f * k.dz : k.dz;
-        c->nl.lossP = (float)(0.5 * (double)k.alpha_p);
-        c->nl.lossS = (float)(0.5 * (double)k.alpha_s);
-        c->nl.tpa = (float)(0.5 * ((double)k.beta_s * 0.5 * (double)k.c * (double)k.eps0 * (double)k.n_s(T)));
-        c->nl.h6 = h / 6.0f;
+        shg_internal_thermal_consts(*p, c->hz, &c->nl, &c->jc, &c->tc);
         c->host_stage.resize(4 * ncell);
         if (hipMalloc(&c->d_desc, sizeof(shg::InstDesc)) != hipSuccess)
             return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
@@ -545,19 +567,6 @@ int shg_create(const shg_params *p, shg_ctx **out) {
                 hipMalloc(&c->jpart, (size_t)c->njpart * sizeof(double)) != hipSuccess ||
                 hipMalloc(&c->jres, sizeof(double)) != hipSuccess)
                 return bad(fail(SHG_E_NOMEM, "shg_create: device allocation failed"));
-            const float hdz = shg::phys::heat_dz(k, c->hz);
-            c->jc = shg::phys::jacobi_const<shg::JacobiConst>(k, hdz, (float)p->k_th, (float)p->h_conv, (float)p->t_inf);
-            c->jc.bc = p->thermal_bc;
-            shg::ThermConst &tc = c->tc;
-            tc.pi = k.pi; tc.dq = k.dq; tc.lp = k.lp; tc.ls = k.ls; tc.inv_period = 1.0f / k.period;
-            tc.rlp = 1.0f / k.lp; tc.rls = 1.0f / k.ls;
-            tc.lossP = c->nl.lossP; tc.lossS = c->nl.lossS; tc.h6 = c->nl.h6;
-            tc.tpa_base = (double)k.beta_s * 0.5 * (double)k.eps0 * (double)k.c;
-            tc.q_p = (double)k.alpha_p * 0.5 * (double)k.c * (double)k.eps0;
-            tc.q_s = (double)k.alpha_s * 0.5 * (double)k.c * (double)k.eps0;
-            tc.q_i = 0.5 * (double)k.c * (double)k.eps0;
-            tc.beta_s = k.beta_s;
-            tc.crystal = p->crystal;
             if (int rc = thermal_init_temperature(c)) return bad(rc);
         }
         shg::InstDesc hd{};

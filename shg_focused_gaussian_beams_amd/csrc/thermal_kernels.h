@@ -59,6 +59,12 @@ struct ThermRowArgs {
     int tape_s0;
     float4 *tape_a2;
     float *tape_dk;
+    // batched THERMAL instances (blockIdx.y = instance): the per-instance strides of the
+    // field-plane arrays (U, Uout, DKint: cells) and of the heat-grid arrays (NT, Q:
+    // voxels), and a per-instance flag (null: every instance) that lets instances which
+    // no longer march skip the pass
+    size_t inst_cells, inst_voxels;
+    const int *active;
 };
 
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
@@ -73,6 +79,13 @@ int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan);   // 0
 hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                           double *part, int nparts, hipStream_t st, const Jacobi2Plan &plan);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
+// Batched THERMAL instances of one heat grid (instance i at T + i nx ny hz, likewise Tf,
+// Q): one single sweep of every instance whose active[i] is set; with `part`, each
+// instance's residual partials (jacobi_blocks of them, instance after instance, zeros for
+// inactive instances), summed per instance into out[i] by launch_sum_partials_batch.
+hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int count,
+                               const int *active, const JacobiConst &c, double *part, hipStream_t st);
+hipError_t launch_sum_partials_batch(const double *part, int per_instance, int count, double *out, hipStream_t st);
 // Per heat voxel tab = {n_p(T), n_s(T)} with the reference's typing (T is frozen during
 // one march, so the row pass reads them instead of evaluating the FP64 Sellmeier fits per
 // cell and step).
@@ -82,7 +95,8 @@ hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, const Ther
 // split layout (the two fields transformed by two waves side by side).
 constexpr int kThermRowVariantCount = 7;
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
-                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0);
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0,
+                                   int count = 1);
 // Grid size the Jacobi launcher uses (number of residual partials it writes).
 int jacobi_blocks(int nx, int ny, int hz);
 

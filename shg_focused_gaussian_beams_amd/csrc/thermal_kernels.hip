@@ -293,6 +293,57 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
+// Batched single sweep: blockIdx.z runs over the instances' planes (instance = z / hz);
+// the same voxel formulas as k_jacobi.  Inactive instances write zero partials.
+template <bool RESID>
+__global__ __launch_bounds__(256) void k_jacobi_batch(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                      const float *__restrict__ Q, int nx, int ny, int hz,
+                                                      const int *__restrict__ active, JacobiConst c,
+                                                      double *__restrict__ part) {
+    const int inst = blockIdx.z / hz, z = blockIdx.z - inst * hz;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const bool on = active[inst] != 0;
+    double r = 0.0;
+    if (on && x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+        const size_t i = (size_t)inst * sz * hz + (size_t)z * sz + (size_t)y * sy + x;
+        const float out = jacobi_voxel(
+            c, jacobi_cls(x, nx), y < ny - 1, jacobi_cls(z, hz),
+            [&](int dx, int dy, int dz) { return Ti[(ptrdiff_t)i + dx + (ptrdiff_t)dy * (ptrdiff_t)sy + (ptrdiff_t)dz * (ptrdiff_t)sz]; },
+            [&] { return Q[i]; });
+        Tf[i] = out;
+        if (RESID) {
+            const double d = (double)(out - Ti[i]);
+            r = d * d;
+        }
+    }
+    if (RESID) {
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        __shared__ double w[4];
+        if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = r;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+    }
+}
+
+// Per-instance sums of consecutive ranges of `per` partials (block = instance), in the
+// order and tree of k_sum_partials.
+__global__ __launch_bounds__(256) void k_sum_partials_batch(const double *__restrict__ part, int per, double *out) {
+    __shared__ double s[256];
+    const double *p = part + (size_t)blockIdx.x * per;
+    double r = 0.0;
+    for (int i = threadIdx.x; i < per; i += 256) r += p[i];
+    s[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
 // Refractive indices of every heat voxel, {n_p(T), n_s(T)}: the temperature is frozen
 // during a march, so the FP64 Sellmeier fits run once per voxel instead of twice per cell
 // and step.
@@ -302,6 +353,21 @@ __global__ __launch_bounds__(256) void k_index_table(const float *__restrict__ T
     if (i >= n) return;
     const float t = T[i];
     tab[i] = make_float2(index_dev(k.crystal, k.lp, t), index_dev(k.crystal, k.ls, t));
+}
+
+// Batched instances (blockIdx.y): moves the pass's array pointers to this block's instance
+// and tells whether the instance sits this pass out.
+__device__ __forceinline__ bool therm_batch_skip(float4 *__restrict__ &U, float4 *__restrict__ &Uout,
+                                                 const float2 *__restrict__ &NT, float *__restrict__ &Q,
+                                                 float *__restrict__ &DKint, const ThermRowArgs &a) {
+    const int inst = blockIdx.y;
+    if (a.active && !a.active[inst]) return true;
+    U += inst * a.inst_cells;
+    Uout += inst * a.inst_cells;
+    DKint += inst * a.inst_cells;
+    NT += inst * a.inst_voxels;
+    Q += inst * a.inst_voxels;
+    return false;
 }
 
 // Row pass of the THERMAL z-march (flags in ThermRowFlags):
@@ -363,6 +429,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
                                                            float *__restrict__ DKint, ThermRowArgs a) {
     using C = RowCfgT<N, E, W>;
     constexpr int TT = C::T;
+    if (therm_batch_skip(U, Uout, NT, Q, DKint, a)) return;
     __shared__ float2 lds[2][C::ROWS][C::L];
     __shared__ float2 tw[N];
     const int lane = threadIdx.x;
@@ -419,6 +486,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
                                                            const float2 *__restrict__ NT, float *__restrict__ Q,
                                                            float *__restrict__ DKint, ThermRowArgs a) {
     constexpr int TT = N / E, THREADS = 64 * W, L = LdsLen<N>::value, CPT = N / THREADS;
+    if (therm_batch_skip(U, Uout, NT, Q, DKint, a)) return;
     static_assert(TT <= 64 && N % THREADS == 0, "the transforms run in one wavefront");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
@@ -487,6 +555,7 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
                                                             const float2 *__restrict__ NT, float *__restrict__ Q,
                                                             float *__restrict__ DKint, ThermRowArgs a) {
     constexpr int T = N / E, THREADS = 64 * W, L = LdsLen<N>::value, CPT = (N + THREADS - 1) / THREADS;
+    if (therm_batch_skip(U, Uout, NT, Q, DKint, a)) return;
     static_assert(W >= 2 && T <= 64, "one wavefront per field transform");
     __shared__ float2 lds[2][L];
     __shared__ float2 tw[N];
@@ -611,6 +680,22 @@ hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, in
     }
 }
 
+hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int count,
+                               const int *active, const JacobiConst &c, double *part, hipStream_t st) {
+    if (nx % 64 != 0 || hz < 2 || ny < 2 || count < 1 || !active) return hipErrorInvalidValue;
+    const dim3 grid(nx / 64, (ny + 3) / 4, (unsigned)(count * hz));
+    if (part)
+        hipLaunchKernelGGL(k_jacobi_batch<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, active, c, part);
+    else
+        hipLaunchKernelGGL(k_jacobi_batch<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, active, c, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_partials_batch(const double *part, int per_instance, int count, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_partials_batch, dim3(count), dim3(256), 0, st, part, per_instance, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, nparts, out);
     return hipGetLastError();
@@ -624,34 +709,35 @@ hipError_t launch_index_table(const float *T, float2 *tab, int64_t n, const Ther
 
 template <int N, int E, int W>
 static hipError_t therm_row_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
-                                   float *DKint, const ThermRowArgs &a, hipStream_t st) {
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int count) {
     using C = RowCfgT<N, E, W>;
-    hipLaunchKernelGGL((k_therm_row_pass<N, E, W>), dim3(N / C::ROWS), dim3(C::THREADS), 0, st, U, Uout, tw, NT, Q,
+    hipLaunchKernelGGL((k_therm_row_pass<N, E, W>), dim3(N / C::ROWS, count), dim3(C::THREADS), 0, st, U, Uout, tw, NT, Q,
                        DKint, a);
     return hipGetLastError();
 }
 
 template <int N, int E, int W>
 static hipError_t therm_wide_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
-                                    float *DKint, const ThermRowArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_therm_row_wide<N, E, W>), dim3(N), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
+                                    float *DKint, const ThermRowArgs &a, hipStream_t st, int count) {
+    hipLaunchKernelGGL((k_therm_row_wide<N, E, W>), dim3(N, count), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
     return hipGetLastError();
 }
 
 template <int N, int E, int W>
 static hipError_t therm_split_launch(float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
-                                     float *DKint, const ThermRowArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL((k_therm_row_split<N, E, W>), dim3(N), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
+                                     float *DKint, const ThermRowArgs &a, hipStream_t st, int count) {
+    hipLaunchKernelGGL((k_therm_row_split<N, E, W>), dim3(N, count), dim3(64 * W), 0, st, U, Uout, tw, NT, Q, DKint, a);
     return hipGetLastError();
 }
 
-#define THRV(N, E, W) return therm_row_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
-#define THRS(N, E, W) return therm_split_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
-#define THRW(N, E, W) return therm_wide_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st)
+#define THRV(N, E, W) return therm_row_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st, count)
+#define THRS(N, E, W) return therm_split_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st, count)
+#define THRW(N, E, W) return therm_wide_launch<N, E, W>(U, Uout, tw, NT, Q, DKint, a, st, count)
 
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
-                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant) {
-    if (variant < 0 || variant >= kThermRowVariantCount) return hipErrorInvalidValue;
+                                   float *DKint, const ThermRowArgs &a, hipStream_t st, int variant, int count) {
+    if (variant < 0 || variant >= kThermRowVariantCount || count < 1 || (count > 1 && !a.inst_cells))
+        return hipErrorInvalidValue;
     switch (n * 16 + variant) {
         case 64 * 16 + 0: THRV(64, 8, 1);
         case 64 * 16 + 1: THRV(64, 4, 1);

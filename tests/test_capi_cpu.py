@@ -56,6 +56,34 @@ def test_create_rejects_bad_parameters_without_device(bad):
     assert lib.shg_last_error().decode().startswith("shg_create")
 
 
+@pytest.mark.parametrize("case", ["plain", "nz", "lx", "bc", "heat_nz", "count"])
+def test_thermal_batch_rejects_mixed_or_bad_builds_without_device(case):
+    # instances may differ only in the beam and the ovens; every check runs before any
+    # device call
+    lib = shg.load_library()
+    a = default_params(thermal=1, nx=64, ny=64, nz=12)
+    b = a.copy(power=3.0, waist=40.0, focus=100.0, temp=30.0, t_peltier1=30.0, t_peltier2=31.0)
+    count = 2
+    if case == "plain":
+        a, b = a.copy(thermal=0), b.copy(thermal=0)
+    elif case == "nz":
+        b = b.copy(nz=14)
+    elif case == "lx":
+        b = b.copy(lx=a.lx * 2)
+    elif case == "bc":
+        b = b.copy(thermal_bc=1)
+    elif case == "heat_nz":
+        a, b = a.copy(heat_nz=500), b.copy(heat_nz=500)
+    else:
+        count = 0
+    arr = (shg.ShgParams * 2)(a, b)
+    h = ctypes.c_void_p()
+    rc = lib.shg_thermal_batch_create(arr, count, ctypes.byref(h))
+    assert rc == -1 and not h.value
+    msg = lib.shg_last_error().decode()
+    assert msg.startswith("shg_thermal_batch_create") or (case == "heat_nz" and msg.startswith("shg_create")), msg
+
+
 @pytest.mark.parametrize("mk", [default_params, plain128_params])
 def test_host_pump_is_bit_exact_with_oracle32(mk):
     p = mk()
