@@ -353,6 +353,77 @@ def bench_reference_default_run(shg, args):
     return dict(p=p, wall=wall, stats=st, eta=eta)
 
 
+def bench_thermal_sweep(shg, args):
+    """The temperature sweep of the reference's sweep script (src/SHG.sh:40, T1 = 54..59 C
+    in steps of 0.2, T2 = T1) at its default THERMAL run (64x64x200, P = 30 W, w0 = 29 um,
+    the reference's Jacobi stopping rule): 26 points as one lockstep batch, as 8
+    concurrent contexts (one host thread and stream each), and one at a time.  The
+    batch's and the contexts' efficiencies must agree bit for bit (same layouts)."""
+    import threading
+    import numpy as np
+    import torch
+    base = shg.default_params(thermal=1)
+    t1s = [round(54.0 + 0.2 * k, 6) for k in range(26)]
+    ps = [base.copy(temp=t, t_peltier1=t, t_peltier2=t) for t in t1s]
+    n = base.nx
+    zero = np.zeros((n, n), np.complex64)
+    ctx = shg.Context(ps[0])
+    ctx.set_fields(shg.make_pump(ps[0]), zero)
+    rv, cv = ctx.tune()
+    ctx.close()
+
+    def one(k, out):
+        c = shg.Context(ps[k])
+        c.set_fields(shg.make_pump(ps[k]), zero)
+        c.set_variants(rv, cv)
+        st = c.thermal_run()
+        out[k] = (c.powers()[2], st)
+        c.close()
+
+    b = shg.ThermalBatch(ps)
+    b.set_variants(rv, cv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bst = b.run()
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t0
+    eta_b = b.powers()[2]
+    jac_layout = b.jacobi_layout()
+    b.close()
+    res = [None] * len(ps)
+    nxt = iter(range(len(ps)))
+    lock = threading.Lock()
+
+    def work():
+        while True:
+            with lock:
+                k = next(nxt, None)
+            if k is None:
+                return
+            one(k, res)
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work) for _ in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    ctx_s = time.perf_counter() - t0
+    eta_c = np.array([r[0] for r in res])
+    seq = [None] * len(ps)
+    t0 = time.perf_counter()
+    for k in range(len(ps)):
+        one(k, seq)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    return dict(points=len(ps), batch_s=batch_s, contexts8_s=ctx_s, one_at_a_time_s=seq_s,
+                batch_jacobi_ms=bst[0]["jacobi_ms"], batch_march_ms=bst[0]["march_ms"], jacobi_layout=jac_layout,
+                variants=[rv, cv], sweeps_total=[s["sweeps_total"] for s in bst],
+                outer_iterations=[s["outer_iterations"] for s in bst],
+                eta_max_abs_diff_batch_vs_contexts=float(np.abs(eta_b - eta_c).max()),
+                eta_min=float(eta_b.min()), eta_max=float(eta_b.max()), t1_of_eta_max=t1s[int(np.argmax(eta_b))])
+
+
 def cpu_small_workloads(shg, default_run_stats):
     """oracle32 -O3 on one pinned core: the plain 128x128, nz 200 workload in full, the
     plain forward of the gradient workload (256x256, nz 400) in full, the Jacobi sweep on
@@ -542,7 +613,8 @@ def main():
     ap.add_argument("--single-sample", type=int, default=32)
     ap.add_argument("--inverse-instances", type=int, default=8)
     ap.add_argument("--inverse-iterations", type=int, default=2)
-    ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse", "inverse-full"], default="all",
+    ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse", "inverse-full",
+                                          "thermal-sweep"], default="all",
                     help="run one extra workload alone and print its figures (for profiler runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (calibrates rocprof byte counters)")
@@ -575,6 +647,8 @@ def main():
             r = bench_inverse(shg, args)
         elif args.section == "inverse-full":
             r = bench_inverse_full(shg, args)
+        elif args.section == "thermal-sweep":
+            r = bench_thermal_sweep(shg, args)
         else:
             r = bench_adjoint(shg, args)
             r["res"] = {k: v for k, v in r["res"].items()}
@@ -627,6 +701,8 @@ def main():
         log("the reference's default run (THERMAL 64x64x200) ...")
         dr = bench_reference_default_run(shg, args)
         out["config"]["reference_default_run_64x64x200"] = {**dr["stats"], "wall_s": dr["wall"], "eta": dr["eta"]}
+        log("temperature sweep of the default run (26 points, batch / contexts / one at a time) ...")
+        out["config"]["thermal_sweep_26_points_64x64x200"] = bench_thermal_sweep(shg, args)
         log("plain 128x128, nz 200 ...")
         p1 = bench_plain(shg, args, shg.plain128_params())
         out["config"]["plain_128x128_nz200"] = {
@@ -679,6 +755,14 @@ def main():
                                                        cw["jacobi_256x256x100"]["voxel_sweeps_per_s"])
             cw["reference_default_run"]["gpu_speedup"] = (cw["reference_default_run"]["s_projected"] /
                                                           c["reference_default_run_64x64x200"]["wall_s"])
+            sw = c["thermal_sweep_26_points_64x64x200"]
+            rd = cw["reference_default_run"]
+            proj = sum(s * rd["s_per_sweep"] + o * rd["s_per_march"]
+                       for s, o in zip(sw["sweeps_total"], sw["outer_iterations"]))
+            cw["thermal_sweep_26_points"] = {
+                "s_projected": proj, "gpu_speedup_batch": proj / sw["batch_s"],
+                "method": "each point's sweep and outer-iteration counts times the default run's timed "
+                          "sweep and march"}
             out["cpu_baseline"]["other_workloads"] = cw
     print(json.dumps(out), flush=True)
 

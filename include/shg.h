@@ -325,6 +325,11 @@ int shg_thermal_batch_set_signal(shg_thermal_batch *b, int index, const float *s
 int shg_thermal_batch_set_variants(shg_thermal_batch *b, int row_variant, int col_variant);
 int shg_thermal_batch_get_variants(const shg_thermal_batch *b, int *row_variant, int *col_variant);
 
+/* Jacobi layout of the batched sweeps (0..SHG_JACOBI_LAYOUTS-1, as shg_set_jacobi_layout:
+ * every layout gives the same sweeps bit for bit); tuned on the first run otherwise. */
+int shg_thermal_batch_set_jacobi_layout(shg_thermal_batch *b, int layout);
+int shg_thermal_batch_get_jacobi_layout(const shg_thermal_batch *b, int *layout);
+
 /* Runs every instance (stats: array of `count`, may be null).  jacobi_ms and march_ms are
  * the batch's device times, the same in every entry. */
 int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats);

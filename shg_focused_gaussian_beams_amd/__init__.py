@@ -118,6 +118,8 @@ def load_library(path=LIB_PATH):
         "shg_thermal_batch_set_variants": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "shg_thermal_batch_get_variants": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int),
                                                           ctypes.POINTER(ctypes.c_int)]),
+        "shg_thermal_batch_set_jacobi_layout": (ctypes.c_int, [vp, ctypes.c_int]),
+        "shg_thermal_batch_get_jacobi_layout": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "shg_thermal_batch_run": (ctypes.c_int, [vp, ctypes.POINTER(ThermalStats)]),
         "shg_thermal_batch_get_fields": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
         "shg_thermal_batch_get_temperature": (ctypes.c_int, [vp, ctypes.c_int, fp]),
@@ -456,6 +458,14 @@ class ThermalBatch:
         r, c = ctypes.c_int(), ctypes.c_int()
         _check(self.lib.shg_thermal_batch_get_variants(self.h, ctypes.byref(r), ctypes.byref(c)))
         return r.value, c.value
+
+    def set_jacobi_layout(self, layout):
+        _check(self.lib.shg_thermal_batch_set_jacobi_layout(self.h, layout))
+
+    def jacobi_layout(self):
+        v = ctypes.c_int()
+        _check(self.lib.shg_thermal_batch_get_jacobi_layout(self.h, ctypes.byref(v)))
+        return v.value
 
     def run(self):
         """Runs every instance; returns one stats dict per instance."""

@@ -3,7 +3,7 @@
 // point (THERMAL), with the same per-point folder and file names.
 //
 //   SHG_sweep --power=LIST --waist=LIST --t1=LIST [--t2=LIST] [--save_temperature=0|1]
-//             [--folder=Simulations] [--workers=N] [--key=value ...]
+//             [--folder=Simulations] [--thermal_batch=0|1] [--workers=N] [--key=value ...]
 //
 // LIST is `v1,v2,...` or `start:stop:step` (inclusive, as the awk loops of SHG.sh).  The
 // points are the product power x waist x t1 (x t2; by default t2 = t1, as SHG.sh sets
@@ -12,9 +12,10 @@
 // THERMAL temperature Tfinal_<z>.dat when asked, and <same name>.txt with the point's
 // powers and efficiency (src/SHG.sh:65-80).  Every SHG override (--nx, --nz, --thermal,
 // --seed, --noise, ...) applies to all points; the SH noise of point k uses seed + k.
-// THERMAL points run as one context each, N at a time from N host threads (default 8):
-// every context has its own stream, and one point's latency-bound passes leave most of
-// the GPU free for the others.
+// THERMAL points run as one lockstep batch (shg_thermal_batch: one Jacobi launch and one
+// march launch cover every point still running), or with --thermal_batch=0 as one
+// context each, N at a time from N host threads (--workers, default 8), each context on
+// its own stream.  Both give every point the result of its own SHG run.
 #include <atomic>
 #include <filesystem>
 #include <sstream>
@@ -54,7 +55,8 @@ std::string num(double v) {
 
 void usage() {
     std::cerr << "usage: SHG_sweep --power=LIST --waist=LIST --t1=LIST [--t2=LIST] [--save_temperature=0|1]"
-                 " [--folder=DIR] [--workers=N] [--key=value ...]   (LIST: v1,v2,... or start:stop:step)\n";
+                 " [--folder=DIR] [--thermal_batch=0|1] [--workers=N] [--key=value ...]"
+                 "   (LIST: v1,v2,... or start:stop:step)\n";
 }
 
 struct Point {
@@ -76,6 +78,7 @@ int main(int argc, char *argv[]) {
     std::string unused_outdir;
     bool focus_set = false;
     int workers = 8;
+    bool thermal_batch = true;
     for (int a = 1; a < argc; a++) {
         const std::string s = argv[a];
         const size_t eq = s.find('=');
@@ -90,6 +93,7 @@ int main(int argc, char *argv[]) {
         else if (key == "save_only_last") ok = std::atoi(val.c_str()) != 0;   // sweeps save the last slice
         else if (key == "folder") { folder = val; if (!folder.empty() && folder.back() != '/') folder += "/"; }
         else if (key == "workers") ok = (workers = std::atoi(val.c_str())) >= 1 && workers <= 64;
+        else if (key == "thermal_batch") thermal_batch = std::atoi(val.c_str()) != 0;
         else {
             if (key == "focus") focus_set = true;
             ok = parse_override(base, key, val, seed, noise, unused_outdir);
@@ -142,6 +146,32 @@ int main(int argc, char *argv[]) {
                 return die("shg_batch_get_fields", rc, "SHG_sweep");
         }
         shg_batch_destroy(b);
+    } else if (thermal_batch) {
+        shg_thermal_batch *b = nullptr;
+        if ((rc = shg_thermal_batch_create(ps.data(), count, &b))) return die("shg_thermal_batch_create", rc, "SHG_sweep");
+        for (int k = 0; k < count && !rc; k++) {
+            std::vector<float> sh(2 * n2);
+            gaussian_noise(sh, noise, seed + (uint64_t)k);
+            rc = shg_thermal_batch_set_signal(b, k, sh.data());
+        }
+        std::vector<shg_thermal_stats> st(count);
+        if (!rc) rc = shg_thermal_batch_run(b, st.data());
+        if (!rc) rc = shg_thermal_batch_powers(b, pin.data(), pout.data(), eta.data());
+        for (int k = 0; k < count && !rc; k++) {
+            outs_fh[k].resize(2 * n2);
+            outs_sh[k].resize(2 * n2);
+            rc = shg_thermal_batch_get_fields(b, k, outs_fh[k].data(), outs_sh[k].data());
+            if (!rc && save_temperature) {
+                temps[k].resize(n2 * (size_t)(base.heat_nz > 0 ? base.heat_nz : base.nz));
+                rc = shg_thermal_batch_get_temperature(b, k, temps[k].data());
+            }
+        }
+        if (rc) {
+            std::cerr << "SHG_sweep: THERMAL batch: " << shg_last_error() << "\n";
+            shg_thermal_batch_destroy(b);
+            return 1;
+        }
+        shg_thermal_batch_destroy(b);
     } else {
         std::atomic<int> next{0};
         std::atomic<bool> failed{false};

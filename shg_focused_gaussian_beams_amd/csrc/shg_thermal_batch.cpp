@@ -63,7 +63,12 @@ struct shg_thermal_batch {
     float *T[2] = {nullptr, nullptr};
     float *Q = nullptr, *DKint = nullptr;
     shg::InstDesc *desc = nullptr;
-    int *active = nullptr;                 // device flags: the instance marches / sweeps
+    int *active = nullptr;                 // device flags: the instance marches
+    int *ids = nullptr;                    // device list of the instances the Jacobi launches sweep
+    int nids = 0;
+    std::vector<int> host_ids;
+    shg::Jacobi2Plan jp{};                 // the two-sweep plan of layout jp_v for nids instances
+    int jp_v = -2;
     double *jpart = nullptr, *jres = nullptr;
     double2 *ppart = nullptr, *pred = nullptr;
     int njb = 0;                           // residual partials per instance
@@ -77,6 +82,8 @@ struct shg_thermal_batch {
     std::vector<float> t_centre;
     std::vector<shg_thermal_stats> stats;
     int row_v = -1, col_v = -1;
+    int jac_v = -1;                        // Jacobi layout: a two-sweep plan, or single sweeps
+    shg::Jacobi2Plan jplan[shg::kJacobi2VariantCount] = {};
     hipGraphExec_t gmarch = nullptr;       // the batched march (reads the flags from `active`)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ran = false;
@@ -88,7 +95,7 @@ void tb_release(shg_thermal_batch *b) {
     if (!b) return;
     if (b->gmarch) (void)hipGraphExecDestroy(b->gmarch);
     for (void *ptr : {(void *)b->U, (void *)b->Uout, (void *)b->tw, (void *)b->G, (void *)b->NT, (void *)b->T[0],
-                      (void *)b->T[1], (void *)b->Q, (void *)b->DKint, (void *)b->desc, (void *)b->active,
+                      (void *)b->T[1], (void *)b->Q, (void *)b->DKint, (void *)b->desc, (void *)b->active, (void *)b->ids,
                       (void *)b->jpart, (void *)b->jres, (void *)b->ppart, (void *)b->pred})
         if (ptr) (void)hipFree(ptr);
     if (b->e0) (void)hipEventDestroy(b->e0);
@@ -99,6 +106,24 @@ void tb_release(shg_thermal_batch *b) {
 
 int set_active(shg_thermal_batch *b, const std::vector<int> &on) {
     TB_TRY(shg_internal_copy(b->active, on.data(), on.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    return SHG_OK;
+}
+
+// The instances the next Jacobi launches sweep (those with on[i] set), and the two-sweep
+// plan sized for that many: instances that have stopped take no workgroups.
+int set_sweeping(shg_thermal_batch *b, const std::vector<int> &on) {
+    std::vector<int> list;
+    for (int i = 0; i < b->count; i++)
+        if (on[i]) list.push_back(i);
+    if (list == b->host_ids && b->jp_v == b->jac_v && b->nids == (int)list.size()) return SHG_OK;
+    b->host_ids = list;
+    b->nids = (int)list.size();
+    b->jp_v = b->jac_v;
+    if (!b->nids) return SHG_OK;
+    TB_TRY(shg_internal_copy(b->ids, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    if (b->jac_v >= 0 && b->jac_v < shg::kJacobi2VariantCount &&
+        shg::jacobi2_plan(b->n, b->n, b->hz, b->jac_v, &b->jp, b->nids) != 0)
+        return shg_internal_fail(SHG_E_DEVICE, "shg_thermal_batch_run: Jacobi layout query failed");
     return SHG_OK;
 }
 
@@ -195,19 +220,73 @@ int tune(shg_thermal_batch *b) {
     return SHG_OK;
 }
 
-// One single sweep of the flagged instances; with `resid`, each instance's mean squared
-// update (FP64; zero for the others).
-int sweep(shg_thermal_batch *b, std::vector<double> *resid) {
-    float *Ti = b->T[b->tcur], *Tf = b->T[b->tcur ^ 1];
-    TB_TRY(shg::launch_jacobi_batch(Tf, Ti, b->Q, b->n, b->n, b->hz, b->count, b->active, b->jc,
-                                    resid ? b->jpart : nullptr, b->stream));
-    b->tcur ^= 1;
-    if (resid) {
-        TB_TRY(shg::launch_sum_partials_batch(b->jpart, b->njb, b->count, b->jres, b->stream));
-        resid->assign(b->count, 0.0);
-        TB_TRY(shg_internal_copy(resid->data(), b->jres, b->count * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-        for (double &r : *resid) r /= (double)b->n3;
+// Times every two-sweep layout (all instances flagged) and two single sweeps, each
+// replayed from a graph, and keeps the fastest.  The timed launches read the current
+// temperatures and write the other buffer only.
+int tune_jacobi(shg_thermal_batch *b) {
+    const char *nt = std::getenv("SHG_NO_TUNE");
+    if (nt && nt[0] == '1') {
+        b->jac_v = 0;
+        return SHG_OK;
     }
+    TB_RC(set_sweeping(b, std::vector<int>(b->count, 1)));
+    float *Ti = b->T[b->tcur], *Tf = b->T[b->tcur ^ 1];
+    float best = 1e30f;
+    int bv = 0;
+    for (int v = 0; v <= shg::kJacobi2VariantCount; v++) {
+        float ms = 0.f;
+        TB_RC(shg_internal_time_graph(b->stream, b->e0, b->e1, 4, [&] {
+            hipError_t e;
+            if (v < shg::kJacobi2VariantCount) {
+                e = shg::launch_jacobi2_batch(Tf, Ti, b->Q, b->n, b->n, b->hz, b->nids, b->ids, b->jc, b->stream,
+                                              b->jplan[v]);
+            } else {
+                e = shg::launch_jacobi_batch(Tf, Ti, b->Q, b->n, b->n, b->hz, b->nids, b->ids, b->jc, nullptr,
+                                             b->stream);
+                if (e == hipSuccess)
+                    e = shg::launch_jacobi_batch(Tf, Ti, b->Q, b->n, b->n, b->hz, b->nids, b->ids, b->jc, nullptr,
+                                                 b->stream);
+            }
+            return e == hipSuccess ? (int)SHG_OK
+                                   : shg_internal_fail(SHG_E_DEVICE, std::string("tuning Jacobi: ") + hipGetErrorString(e));
+        }, &ms));
+        if (ms < best) { best = ms; bv = v; }
+    }
+    b->jac_v = bv;
+    return SHG_OK;
+}
+
+// `nsweeps` sweeps of the instances set_sweeping chose, without the residual: two per
+// pass with a two-sweep layout, then a single one if the count is odd.
+int sweeps(shg_thermal_batch *b, int64_t nsweeps) {
+    if (!b->nids) return SHG_OK;
+    int64_t i = 0;
+    if (b->jac_v < shg::kJacobi2VariantCount)
+        for (; nsweeps - i >= 2; i += 2) {
+            TB_TRY(shg::launch_jacobi2_batch(b->T[b->tcur ^ 1], b->T[b->tcur], b->Q, b->n, b->n, b->hz, b->nids,
+                                             b->ids, b->jc, b->stream, b->jp));
+            b->tcur ^= 1;
+        }
+    for (; i < nsweeps; i++) {
+        TB_TRY(shg::launch_jacobi_batch(b->T[b->tcur ^ 1], b->T[b->tcur], b->Q, b->n, b->n, b->hz, b->nids, b->ids,
+                                        b->jc, nullptr, b->stream));
+        b->tcur ^= 1;
+    }
+    return SHG_OK;
+}
+
+// One single sweep of the instances set_sweeping chose, with each one's mean squared
+// update (FP64) in resid[instance] (zero for the others).
+int sweep_checked(shg_thermal_batch *b, std::vector<double> *resid) {
+    resid->assign(b->count, 0.0);
+    if (!b->nids) return SHG_OK;
+    float *Ti = b->T[b->tcur], *Tf = b->T[b->tcur ^ 1];
+    TB_TRY(shg::launch_jacobi_batch(Tf, Ti, b->Q, b->n, b->n, b->hz, b->nids, b->ids, b->jc, b->jpart, b->stream));
+    b->tcur ^= 1;
+    TB_TRY(shg::launch_sum_partials_batch(b->jpart, b->njb, b->nids, b->jres, b->stream));
+    std::vector<double> r(b->nids);
+    TB_TRY(shg_internal_copy(r.data(), b->jres, r.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    for (int j = 0; j < b->nids; j++) (*resid)[b->host_ids[j]] = r[j] / (double)b->n3;
     return SHG_OK;
 }
 
@@ -234,7 +313,8 @@ extern "C" {
 
 int shg_thermal_batch_create(const shg_params *params, int count, shg_thermal_batch **out) {
     return shg_internal_guard(__func__, [&]() -> int {
-        if (!params || !out || count < 1) return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_create: bad argument");
+        if (!params || !out || count < 1 || count > 65535)
+            return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_create: bad argument (count must be 1..65535)");
         *out = nullptr;
         const shg_params &p0 = params[0];
         if (!p0.thermal) return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_create: instances must be THERMAL");
@@ -257,6 +337,9 @@ int shg_thermal_batch_create(const shg_params *params, int count, shg_thermal_ba
         b->n2 = (size_t)b->n * b->n;
         b->n3 = b->n2 * b->hz;
         b->njb = shg::jacobi_blocks(b->n, b->n, b->hz);
+        for (int v = 0; v < shg::kJacobi2VariantCount; v++)
+            if (shg::jacobi2_plan(b->n, b->n, b->hz, v, &b->jplan[v], count) != 0)
+                return bad(shg_internal_fail(SHG_E_DEVICE, "shg_thermal_batch_create: Jacobi layout query failed"));
         const size_t C = (size_t)count;
         hipError_t e;
         if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -272,6 +355,7 @@ int shg_thermal_batch_create(const shg_params *params, int count, shg_thermal_ba
             hipMalloc(&b->Q, C * b->n3 * sizeof(float)) != hipSuccess ||
             hipMalloc(&b->DKint, C * b->n2 * sizeof(float)) != hipSuccess ||
             hipMalloc(&b->desc, C * sizeof(shg::InstDesc)) != hipSuccess || hipMalloc(&b->active, C * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->ids, C * sizeof(int)) != hipSuccess ||
             hipMalloc(&b->jpart, C * b->njb * sizeof(double)) != hipSuccess || hipMalloc(&b->jres, C * sizeof(double)) != hipSuccess ||
             hipMalloc(&b->ppart, 256 * sizeof(double2)) != hipSuccess || hipMalloc(&b->pred, sizeof(double2)) != hipSuccess)
             return bad(shg_internal_fail(SHG_E_NOMEM, "shg_thermal_batch_create: device allocation failed"));
@@ -344,6 +428,23 @@ int shg_thermal_batch_get_variants(const shg_thermal_batch *b, int *row_variant,
     });
 }
 
+int shg_thermal_batch_set_jacobi_layout(shg_thermal_batch *b, int layout) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || layout < 0 || layout > shg::kJacobi2VariantCount)
+            return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_set_jacobi_layout: layout out of range");
+        b->jac_v = layout;
+        return SHG_OK;
+    });
+}
+
+int shg_thermal_batch_get_jacobi_layout(const shg_thermal_batch *b, int *layout) {
+    return shg_internal_guard(__func__, [&]() -> int {
+        if (!b || !layout) return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_get_jacobi_layout: bad argument");
+        *layout = b->jac_v;
+        return SHG_OK;
+    });
+}
+
 int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats) {
     return shg_internal_guard(__func__, [&]() -> int {
         if (!b) return shg_internal_fail(SHG_E_INVALID, "shg_thermal_batch_run: null batch");
@@ -359,6 +460,7 @@ int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats) {
             }
         }
         TB_RC(init_temperatures(b));
+        if (b->jac_v < 0) TB_RC(tune_jacobi(b));
         const int check = p0.jacobi_check > 0 ? p0.jacobi_check : 5000;
         const int num_of_iter = p0.max_outer > 0 ? p0.max_outer : 5;
         const double tol = p0.jacobi_tol > 0 ? (double)(float)p0.jacobi_tol : (double)(float)(5e-4 / (double)b->n3);
@@ -385,15 +487,14 @@ int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats) {
             std::vector<int> going(C);
             std::vector<int64_t> counter_at_stop(C, 0);
             for (int i = 0; i < C; i++) going[i] = !done[i];
-            const int t_start = b->tcur;
             int64_t counter = 0;
             float ms = 0.f;
             TB_TRY(hipEventRecord(b->e0, b->stream));
             while (std::any_of(going.begin(), going.end(), [](int g) { return g != 0; })) {
-                TB_RC(set_active(b, going));
+                TB_RC(set_sweeping(b, going));
                 if (counter % check == 0) {
                     std::vector<double> r;
-                    TB_RC(sweep(b, &r));
+                    TB_RC(sweep_checked(b, &r));
                     for (int i = 0; i < C; i++) {
                         if (!going[i]) continue;
                         if (counter > 0) aux[i] = reduced[i];
@@ -406,9 +507,11 @@ int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats) {
                     const int64_t next_check = (counter / check + 1) * (int64_t)check;
                     int64_t burst = next_check - counter;
                     if (p0.jacobi_max > 0) burst = std::min<int64_t>(burst, p0.jacobi_max - counter);
-                    for (int64_t j = 0; j < burst; j++) TB_RC(sweep(b, nullptr));
+                    TB_RC(sweeps(b, burst));
                     counter += burst;
                 }
+                for (int i = 0; i < C; i++)
+                    if (going[i]) where[i] = b->tcur;
                 for (int i = 0; i < C; i++) {
                     if (!going[i]) continue;
                     const bool capped = p0.jacobi_max > 0 && counter >= p0.jacobi_max;
@@ -425,11 +528,8 @@ int shg_thermal_batch_run(shg_thermal_batch *b, shg_thermal_stats *stats) {
             // each instance of this iteration stopped after its own number of sweeps, so its
             // temperature sits in one buffer or the other: bring the marching ones to T[tcur]
             const int tnow = b->tcur;
-            for (int i = 0; i < C; i++) {
-                if (done[i]) continue;
-                where[i] = t_start ^ (int)(counter_at_stop[i] & 1);
-                TB_RC(bring_to_current(i));
-            }
+            for (int i = 0; i < C; i++)
+                if (!done[i]) TB_RC(bring_to_current(i));
             // fields and propagators of the marching instances at their grid-centre
             // temperature of slice 0 (Solver.h:329-331), then one batched march
             std::vector<int> marching(C);

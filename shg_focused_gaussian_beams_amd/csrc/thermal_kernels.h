@@ -72,19 +72,24 @@ hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int
 // Two sweeps in one pass (z-marching LDS tiles, temporal blocking), bit-exact with two
 // launch_jacobi sweeps; with `part` the second sweep's squared updates, one FP64 partial
 // per block (plan.blocks of them).  Variants: 3 tile shapes x 3 wave counts; a plan
-// (jacobi2_plan, which queries the current device) fixes the tiles and z-chunks.
+// (jacobi2_plan, which queries the current device) fixes the tiles and z-chunks for
+// `count` instances run by one launch.
 constexpr int kJacobi2VariantCount = 9;
 struct Jacobi2Plan { int shape, tiles_x, zc, nchunks, blocks; };
-int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan);   // 0 on success
+int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan, int count = 1);   // 0 on success
 hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                           double *part, int nparts, hipStream_t st, const Jacobi2Plan &plan);
+// The same two sweeps for instances ids[0 .. nids) of a batch (instance i at
+// T + i nx ny hz, likewise Tf and Q; plan made for nids instances), without the residual.
+hipError_t launch_jacobi2_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
+                                const int *ids, const JacobiConst &c, hipStream_t st, const Jacobi2Plan &plan);
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
 // Batched THERMAL instances of one heat grid (instance i at T + i nx ny hz, likewise Tf,
-// Q): one single sweep of every instance whose active[i] is set; with `part`, each
-// instance's residual partials (jacobi_blocks of them, instance after instance, zeros for
-// inactive instances), summed per instance into out[i] by launch_sum_partials_batch.
-hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int count,
-                               const int *active, const JacobiConst &c, double *part, hipStream_t st);
+// Q): one single sweep of instances ids[0 .. nids); with `part`, the residual partials of
+// each (jacobi_blocks of them, in the order of ids), summed per entry of ids into out[j]
+// by launch_sum_partials_batch.
+hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
+                               const int *ids, const JacobiConst &c, double *part, hipStream_t st);
 hipError_t launch_sum_partials_batch(const double *part, int per_instance, int count, double *out, hipStream_t st);
 // Per heat voxel tab = {n_p(T), n_s(T)} with the reference's typing (T is frozen during
 // one march, so the row pass reads them instead of evaluating the FP64 Sellmeier fits per

@@ -126,13 +126,16 @@ __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const fl
 // tile that needs them, with the same formulas, so the result is two single sweeps bit
 // for bit.  With RESID the FP64 sum of (T2 - T1)^2 over the block's voxels (the second
 // sweep's mean squared update) goes to part[block].
-// Block order: workgroups go to the 8 XCDs round-robin; the logical block (tile, chunk),
-// chunk fastest, is dealt out so that each XCD runs a contiguous range of them, whose
-// z-halo and x-y halo planes its L2 then shares.
+// Block order: workgroups go to the 8 XCDs round-robin; the logical block (instance,
+// tile, chunk), chunk fastest, is dealt out so that each XCD runs a contiguous range of
+// them, whose z-halo and x-y halo planes its L2 then shares.
+// Batched instances: slot j < gridDim.y runs instance ids[j] (at offset ids[j]*inst_voxels
+// of Tf, Ti, Q), its partials at j*gridDim.x; without `ids` the one slot is instance 0.
 template <int TX, int TY, bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                  const float *__restrict__ Q, int nx, int ny, int hz, int zc,
-                                                 int tiles_x, int nchunks, JacobiConst c, double *__restrict__ part) {
+                                                 int tiles_x, int nchunks, JacobiConst c, double *__restrict__ part,
+                                                 const int *__restrict__ ids, size_t inst_voxels) {
     constexpr int W2 = TX + 4, H2 = TY + 4, N2 = W2 * H2;
     constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
     constexpr int N0 = TX * TY;
@@ -141,9 +144,15 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
     __shared__ float Lq[3][N1];
     __shared__ float L1[3][N1];
     const int tid = threadIdx.x;
-    const int nb = gridDim.x, b = blockIdx.x;
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     const int per = nb >> 3, rem = nb & 7, xcd = b & 7;
-    const int lb = xcd * per + min(xcd, rem) + (b >> 3);
+    const int gb = xcd * per + min(xcd, rem) + (b >> 3);
+    const int slot = gb / (int)gridDim.x, lb = gb - slot * (int)gridDim.x;
+    const size_t inst = ids ? (size_t)ids[slot] : 0;
+    Tf += inst * inst_voxels;
+    Ti += inst * inst_voxels;
+    Q += inst * inst_voxels;
+    if (RESID) part += (size_t)slot * gridDim.x;
     const int tile = lb / nchunks, chunk = lb - tile * nchunks;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
     const int x0 = tx * TX, y0 = ty * TY;
@@ -293,19 +302,23 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
     if (threadIdx.x == 0) *out = s[0];
 }
 
-// Batched single sweep: blockIdx.z runs over the instances' planes (instance = z / hz);
-// the same voxel formulas as k_jacobi.  Inactive instances write zero partials.
+// Batched single sweep: slot blockIdx.y runs instance ids[blockIdx.y]; blockIdx.x is the
+// block of k_jacobi's grid flattened (x fastest, then y, then z), so that a slot's
+// partials (at blockIdx.y * gridDim.x) come in k_jacobi's order; the same voxel formulas.
 template <bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi_batch(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                       const float *__restrict__ Q, int nx, int ny, int hz,
-                                                      const int *__restrict__ active, JacobiConst c,
+                                                      const int *__restrict__ ids, JacobiConst c,
                                                       double *__restrict__ part) {
-    const int inst = blockIdx.z / hz, z = blockIdx.z - inst * hz;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const bool on = active[inst] != 0;
+    const int inst = ids[blockIdx.y];
+    const int gx = nx / 64, gy = (ny + 3) / 4;
+    const int bz = (int)blockIdx.x / (gx * gy), rxy = (int)blockIdx.x - bz * gx * gy;
+    const int by = rxy / gx, bx = rxy - by * gx;
+    const int z = bz;
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by * 4 + (threadIdx.x >> 6);
     double r = 0.0;
-    if (on && x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
+    if (x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
         const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
         const size_t i = (size_t)inst * sz * hz + (size_t)z * sz + (size_t)y * sy + x;
         const float out = jacobi_voxel(
@@ -323,8 +336,7 @@ __global__ __launch_bounds__(256) void k_jacobi_batch(float *__restrict__ Tf, co
         __shared__ double w[4];
         if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = r;
         __syncthreads();
-        if (threadIdx.x == 0)
-            part[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+        if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = w[0] + w[1] + w[2] + w[3];
     }
 }
 
@@ -637,8 +649,8 @@ static int j2_slots() {
     return cus * per_cu;
 }
 
-int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan) {
-    if (variant < 0 || variant >= kJacobi2VariantCount || !plan || nx < 2 || ny < 2 || hz < 2) return -1;
+int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan, int count) {
+    if (variant < 0 || variant >= kJacobi2VariantCount || !plan || nx < 2 || ny < 2 || hz < 2 || count < 1) return -1;
     const int shape = variant / 3, waves = kJ2Waves[variant % 3];
     const int tx = kJ2Shape[shape][0], ty = kJ2Shape[shape][1];
     const int slots = shape == 0 ? j2_slots<64, 16>() : shape == 1 ? j2_slots<64, 8>() : j2_slots<128, 8>();
@@ -647,7 +659,7 @@ int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan) {
     p.shape = shape;
     p.tiles_x = (nx + tx - 1) / tx;
     const int tiles = p.tiles_x * ((ny + ty - 1) / ty);
-    const int chunks = std::max(1, waves * slots / tiles);
+    const int chunks = std::max(1, waves * slots / (tiles * count));
     p.zc = std::max(4, (hz + chunks - 1) / chunks);
     p.nchunks = (hz + p.zc - 1) / p.zc;
     p.blocks = tiles * p.nchunks;
@@ -657,13 +669,14 @@ int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan) {
 
 template <int TX, int TY>
 static hipError_t j2_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const Jacobi2Plan &p,
-                            const JacobiConst &c, double *part, hipStream_t st) {
+                            const JacobiConst &c, double *part, hipStream_t st, int nids, const int *ids) {
+    const size_t stride = (size_t)nx * ny * hz;
     if (part)
-        hipLaunchKernelGGL((k_jacobi2<TX, TY, true>), dim3(p.blocks), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, p.zc,
-                           p.tiles_x, p.nchunks, c, part);
+        hipLaunchKernelGGL((k_jacobi2<TX, TY, true>), dim3(p.blocks, nids), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz,
+                           p.zc, p.tiles_x, p.nchunks, c, part, ids, stride);
     else
-        hipLaunchKernelGGL((k_jacobi2<TX, TY, false>), dim3(p.blocks), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, p.zc,
-                           p.tiles_x, p.nchunks, c, part);
+        hipLaunchKernelGGL((k_jacobi2<TX, TY, false>), dim3(p.blocks, nids), dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz,
+                           p.zc, p.tiles_x, p.nchunks, c, part, ids, stride);
     return hipGetLastError();
 }
 
@@ -673,21 +686,35 @@ hipError_t launch_jacobi2(float *Tf, const float *Ti, const float *Q, int nx, in
     if (p.nchunks * p.zc < hz || p.tiles_x * kJ2Shape[p.shape][0] < nx) return hipErrorInvalidValue;
     if (part && nparts < p.blocks) return hipErrorInvalidValue;
     switch (p.shape) {
-        case 0: return j2_launch<64, 16>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
-        case 1: return j2_launch<64, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
-        case 2: return j2_launch<128, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st);
+        case 0: return j2_launch<64, 16>(Tf, Ti, Q, nx, ny, hz, p, c, part, st, 1, nullptr);
+        case 1: return j2_launch<64, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st, 1, nullptr);
+        case 2: return j2_launch<128, 8>(Tf, Ti, Q, nx, ny, hz, p, c, part, st, 1, nullptr);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int count,
-                               const int *active, const JacobiConst &c, double *part, hipStream_t st) {
-    if (nx % 64 != 0 || hz < 2 || ny < 2 || count < 1 || !active) return hipErrorInvalidValue;
-    const dim3 grid(nx / 64, (ny + 3) / 4, (unsigned)(count * hz));
+hipError_t launch_jacobi2_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
+                                const int *ids, const JacobiConst &c, hipStream_t st, const Jacobi2Plan &p) {
+    if (p.blocks <= 0 || p.zc < 1 || hz < 2 || ny < 2 || nx < 2 || nids < 1 || nids > 65535 || !ids)
+        return hipErrorInvalidValue;
+    if (p.nchunks * p.zc < hz || p.tiles_x * kJ2Shape[p.shape][0] < nx) return hipErrorInvalidValue;
+    if ((int64_t)p.blocks * nids > INT32_MAX) return hipErrorInvalidValue;
+    switch (p.shape) {
+        case 0: return j2_launch<64, 16>(Tf, Ti, Q, nx, ny, hz, p, c, nullptr, st, nids, ids);
+        case 1: return j2_launch<64, 8>(Tf, Ti, Q, nx, ny, hz, p, c, nullptr, st, nids, ids);
+        case 2: return j2_launch<128, 8>(Tf, Ti, Q, nx, ny, hz, p, c, nullptr, st, nids, ids);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
+                               const int *ids, const JacobiConst &c, double *part, hipStream_t st) {
+    if (nx % 64 != 0 || hz < 2 || ny < 2 || nids < 1 || nids > 65535 || !ids) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)jacobi_blocks(nx, ny, hz), (unsigned)nids);
     if (part)
-        hipLaunchKernelGGL(k_jacobi_batch<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, active, c, part);
+        hipLaunchKernelGGL(k_jacobi_batch<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
     else
-        hipLaunchKernelGGL(k_jacobi_batch<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, active, c, part);
+        hipLaunchKernelGGL(k_jacobi_batch<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
     return hipGetLastError();
 }
 

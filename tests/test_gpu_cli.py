@@ -137,8 +137,9 @@ def test_sweep_program_matches_single_solves(gpu, tmp_path):
 
 @pytest.mark.gpu
 def test_thermal_sweep_concurrent_points_match_single_runs(gpu, tmp_path):
-    # THERMAL points run as concurrent contexts (one host thread and stream each); every
-    # point's planes equal a single SHG run of that point bit for bit (fixed layouts)
+    # THERMAL points run as one lockstep batch (default) or as concurrent contexts (one
+    # host thread and stream each); either way every point's planes equal a single SHG run
+    # of that point bit for bit (fixed layouts)
     if not os.path.exists(SWEEP):
         pytest.fail("SHG_sweep program not built")
     fixed = {**os.environ, "SHG_NO_TUNE": "1"}
@@ -147,16 +148,20 @@ def test_thermal_sweep_concurrent_points_match_single_runs(gpu, tmp_path):
     args = ["--power=20,30,40", "--waist=30", "--t1=27", "--workers=3", "--folder=" + str(tmp_path / "Sims")]
     r = subprocess.run([SWEEP] + args + COMMON + extra, capture_output=True, text=True, timeout=120, env=fixed)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    r = subprocess.run([SWEEP] + args[:-1] + ["--thermal_batch=0", "--folder=" + str(tmp_path / "Ctx")] + COMMON +
+                       extra, capture_output=True, text=True, timeout=120, env=fixed)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     for k, P in enumerate((20, 30, 40)):
-        d = tmp_path / "Sims" / f"sPPLT_P_{P}_W_T1_27_WAIST_30"
         single = tmp_path / f"single_{P}"
         single.mkdir()
         # the sweep seeds point k's SH noise with seed + k (default seed 1)
         _run(single, [str(P), "30", "27", "27", "1", "0", f"--seed={1 + k}"] + COMMON + extra, {"SHG_NO_TUNE": "1"})
-        for name in ("Pump_out", "Signal_out"):
-            a = np.fromfile(d / f"{name}.c64", np.complex64)
-            b = np.fromfile(single / f"{name}.c64", np.complex64)
-            assert a.size == 64 * 64 and np.array_equal(a, b), (P, name)
+        for folder in ("Sims", "Ctx"):
+            d = tmp_path / folder / f"sPPLT_P_{P}_W_T1_27_WAIST_30"
+            for name in ("Pump_out", "Signal_out"):
+                a = np.fromfile(d / f"{name}.c64", np.complex64)
+                b = np.fromfile(single / f"{name}.c64", np.complex64)
+                assert a.size == 64 * 64 and np.array_equal(a, b), (folder, P, name)
     # tuned once for the sweep and captured into graphs by three threads at once: the
     # layouts may differ from the fixed ones, so the planes agree to rounding
     r = subprocess.run([SWEEP] + args[:-1] + ["--folder=" + str(tmp_path / "Tuned")] + COMMON + extra,

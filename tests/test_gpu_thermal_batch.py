@@ -1,7 +1,8 @@
 """GPU checks of the batched THERMAL sweep (shg_thermal_batch_*).
 
 - Every instance of a batch equals shg_thermal_run on a context of its own with the same
-  parameters and the same row/column layouts, bit for bit: Pump_out/Signal_out, the
+  parameters and the same row/column layouts, bit for bit, whichever Jacobi layout
+  either side runs (two-sweep passes or single sweeps): Pump_out/Signal_out, the
   temperature, the powers and the sweep statistics.  The instances are chosen to stop at
   different sweep counts and outer iterations (a near-uniform one converges at its first
   check and takes the early-stop rule, one is held by the sweep cap), so the lockstep
@@ -36,18 +37,19 @@ def _signals(n, count, seed=11):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bc,cap,rv,cv", [(SHG_BC_TOP_OPEN, 0, 0, 0), (SHG_BC_OVEN_SURROUNDED, 0, 6, 8),
-                                          (SHG_BC_TOP_OPEN, 200, 4, 3)])
-def test_batch_instances_equal_their_own_thermal_runs(gpu, bc, cap, rv, cv):
+@pytest.mark.parametrize("bc,cap,rv,cv,jv", [(SHG_BC_TOP_OPEN, 0, 0, 0, 9), (SHG_BC_OVEN_SURROUNDED, 0, 6, 8, 4),
+                                             (SHG_BC_TOP_OPEN, 200, 4, 3, 0), (SHG_BC_TOP_OPEN, 0, 2, 5, 8)])
+def test_batch_instances_equal_their_own_thermal_runs(gpu, bc, cap, rv, cv, jv):
     ps = _instances(bc, cap)
     sig = _signals(64, len(ps))
-    print(f"\nthermal batch bc={bc} cap={cap} layouts=({rv},{cv}) count={len(ps)}", flush=True)
+    print(f"\nthermal batch bc={bc} cap={cap} layouts=({rv},{cv}) jacobi={jv} count={len(ps)}", flush=True)
     with gpu.ThermalBatch(ps) as b:
         for i, s in enumerate(sig):
             b.set_signal(i, s)
         b.set_variants(rv, cv)
+        b.set_jacobi_layout(jv)
         bst = b.run()
-        assert b.variants() == (rv, cv)
+        assert b.variants() == (rv, cv) and b.jacobi_layout() == jv
         bf = [b.get_fields(i) for i in range(len(ps))]
         bT = [b.get_temperature(i) for i in range(len(ps))]
         bpow = b.powers()

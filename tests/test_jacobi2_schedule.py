@@ -30,22 +30,32 @@ def voxel(ld, x, y, z, nx, ny, hz):
             v += w * ld(dx, dy, dz)
     return v
 
-def emulate(Ti, Q, TX, TY, zc, upd):
-    hz, ny, nx = Ti.shape
+def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
+    """Ti, Q: (count, hz, ny, nx); one launch whose slot j (gridDim.y = len(ids)) sweeps
+    instance ids[j]; the other instances keep Tf's prior contents (here: NaN)."""
+    count, hz, ny, nx = Ti.shape
+    ids = list(range(count)) if ids is None else ids
     W2, H2, N2 = TX+4, TY+4, (TX+4)*(TY+4)
     W1, H1, N1 = TX+2, TY+2, (TX+2)*(TY+2)
     tiles_x, tiles_y = -(-nx//TX), -(-ny//TY)
     nch = -(-hz//zc)
-    nb = tiles_x*tiles_y*nch
-    Tf = np.full_like(Ti, np.nan); Tf[:, 0, :] = Ti[:, 0, :]
-    for y in range(ny):
-        for x in range(nx):
-            if not upd(x, y, nx, ny): Tf[:, y, x] = Ti[:, y, x]
+    blocks = tiles_x*tiles_y*nch
+    nb = blocks*len(ids)
+    Tf_all = np.full_like(Ti, np.nan)
+    for i in ids:
+        for y in range(ny):
+            for x in range(nx):
+                if not upd(x, y, nx, ny): Tf_all[i, :, y, x] = Ti[i, :, y, x]
+    Ti_all, Q_all = Ti, Q
     seen = set()
-    for b in range(nb):
+    for b in range(nb):                       # b = blockIdx.y * gridDim.x + blockIdx.x
         per, rem, xcd = nb >> 3, nb & 7, b & 7
-        lb = xcd*per + min(xcd, rem) + (b >> 3)
-        assert 0 <= lb < nb and lb not in seen; seen.add(lb)
+        gb = xcd*per + min(xcd, rem) + (b >> 3)
+        assert 0 <= gb < nb and gb not in seen; seen.add(gb)
+        slot, lb = divmod(gb, blocks)
+        assert 0 <= slot < len(ids)
+        inst = ids[slot]
+        Ti, Q, Tf = Ti_all[inst], Q_all[inst], Tf_all[inst]
         tile, chunk = divmod(lb, nch)
         ty, tx = divmod(tile, tiles_x)
         x0, y0 = tx*TX, ty*TY
@@ -88,7 +98,7 @@ def emulate(Ti, Q, TX, TY, zc, upd):
                         q = Lq[(p2+3) % 3, c1]; assert not np.isnan(q)
                         Tf[p2, gy, gx] = voxel(ld, gx, gy, p2, nx, ny, hz) + q
     assert len(seen) == nb
-    return Tf
+    return Tf_all
 
 def _top(x, y, nx, ny):
     return y >= 1
@@ -106,6 +116,24 @@ def test_two_sweep_schedule_equals_two_sweeps(nx, ny, hz, TX, TY, zc, bc):
     Ti = rng.random((hz, ny, nx))
     Q = 0.01 * rng.random((hz, ny, nx))
     ref = sweep(sweep(Ti, Q, upd), Q, upd)
-    em = emulate(Ti, Q, TX, TY, zc, upd)
+    em = emulate(Ti[None], Q[None], TX, TY, zc, upd)[0]
     assert not np.isnan(em).any()
     assert np.array_equal(em, ref)
+
+
+@pytest.mark.parametrize("count,ids,nx,ny,hz,TX,TY,zc", [(3, [0, 2], 20, 13, 7, 8, 4, 3),
+                                                          (5, [1, 2, 4], 12, 9, 5, 8, 4, 2),
+                                                          (2, [0, 1], 16, 8, 6, 8, 8, 4)])
+def test_batched_two_sweep_schedule(count, ids, nx, ny, hz, TX, TY, zc):
+    # one launch over a list of instances: the XCD dealing runs over every listed
+    # instance's blocks (a bijection onto slot x block), each listed instance gets its two
+    # sweeps exactly, and the others are not written
+    rng = np.random.default_rng(count * 7 + nx)
+    Ti = rng.random((count, hz, ny, nx))
+    Q = 0.01 * rng.random((count, hz, ny, nx))
+    em = emulate(Ti, Q, TX, TY, zc, _top, ids)
+    for i in range(count):
+        if i in ids:
+            assert np.array_equal(em[i], sweep(sweep(Ti[i], Q[i], _top), Q[i], _top)), i
+        else:
+            assert np.isnan(em[i]).all(), i
