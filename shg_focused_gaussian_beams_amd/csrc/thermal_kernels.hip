@@ -113,19 +113,55 @@ __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const fl
     }
 }
 
+// Cell i of a W x H region whose inner cells form the rectangle [ax, bx) x [ay, by): the
+// rectangle's cells first, row by row, then the rows above it, the rows below it, and the
+// columns left and right of it.  A bijection of [0, W H) onto the region.
+__device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx, int ay, int by, int &row, int &col) {
+    const int wi = bx - ax, hi = by - ay, ni = wi * hi;
+    if (i < ni) {
+        row = ay + i / wi;
+        col = ax + i - (i / wi) * wi;
+        return;
+    }
+    i -= ni;
+    if (i < ay * W) {
+        row = i / W;
+        col = i - row * W;
+        return;
+    }
+    i -= ay * W;
+    if (i < (H - by) * W) {
+        row = by + i / W;
+        col = i - (row - by) * W;
+        return;
+    }
+    i -= (H - by) * W;
+    if (i < ax * hi) {
+        row = ay + i / ax;
+        col = i - (row - ay) * ax;
+        return;
+    }
+    i -= ax * hi;
+    const int wr = W - bx;
+    row = ay + i / wr;
+    col = bx + i - (row - ay) * wr;
+}
+
 // Two Jacobi sweeps per pass, Tf <- J(J(Ti, Q), Q), marching in z (temporal blocking).
 // Workgroup = one TX x TY tile of the x-y plane and one chunk of zc planes.  It keeps in
-// LDS a ring of three planes of Ti over the tile plus a halo of 2 (region R2), a ring of
-// three planes of Q over the tile plus a halo of 1 (R1), and a ring of three planes of
-// the first sweep's values T1 over R1.  Step k of the march stages Ti plane k and Q plane
-// k-1 (both loaded into registers one step ahead, so their latency overlaps the
-// previous step's arithmetic), computes T1 plane k-1 on R1, and then the second sweep's plane k-2
-// on the tile, which it writes.  T1 never leaves the chip: a pass reads Ti once over R2
-// (plus the z halo of its chunk) and Q once over R1, and writes Tf once, where two single
-// sweeps read and write everything twice.  The halo values of T1 are recomputed by every
-// tile that needs them, with the same formulas, so the result is two single sweeps bit
-// for bit.  With RESID the FP64 sum of (T2 - T1)^2 over the block's voxels (the second
-// sweep's mean squared update) goes to part[block].
+// LDS a ring of three planes of Ti over the tile plus a halo of 2 (region R2) and a ring
+// of three planes of the first sweep's values T1 over the tile plus a halo of 1 (R1).
+// Step k of the march stages Ti plane k, computes T1 plane k-1 on R1, and then the
+// second sweep's plane k-2 on the tile, which it writes.  Ti plane k+1 and the Q values
+// of the next step's cells (plane k over R1, plane k-1 over the tile, each thread its own
+// cells) are loaded into registers one step ahead, so that their latency overlaps this
+// step's arithmetic; Q never goes through LDS, which keeps the footprint to the two
+// rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
+// chunk) and Q over R1, and writes Tf once, where two single sweeps read and write
+// everything twice.  The halo values of T1 are recomputed by every tile that needs them,
+// with the same formulas, so the result is two single sweeps bit for bit.  With RESID the
+// FP64 sum of (T2 - T1)^2 over the block's voxels (the second sweep's mean squared
+// update) goes to part[block].
 // Block order: workgroups go to the 8 XCDs round-robin; the logical block (instance,
 // tile, chunk), chunk fastest, is dealt out so that each XCD runs a contiguous range of
 // them, whose z-halo and x-y halo planes its L2 then shares.
@@ -141,7 +177,6 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
     constexpr int N0 = TX * TY;
     constexpr int R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256, R0 = (N0 + 255) / 256;
     __shared__ float Li[3][N2];
-    __shared__ float Lq[3][N1];
     __shared__ float L1[3][N1];
     const int tid = threadIdx.x;
     const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
@@ -160,12 +195,15 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
     const size_t sz = (size_t)nx * ny;
 
     // Per thread, everything about its cells that does not change along the march: the
-    // plane offsets of its R2 (Ti staging) and R1 (Q staging) cells, or -1 outside the
-    // domain; for its R1 cells (first sweep) their place in the R2 ring planes and what
-    // the sweep does there; for its tile cells (second sweep) their place in the R1 ring
-    // planes, plane offset and what the sweep does there.  Cell kinds: 0 outside the
-    // domain, 1 an oven (kept), 2 updated; bits 2-3 the x class + 1, bit 4 below the top.
-    int off2[R2], off1[R1], c2r[R1], k1[R1], c1r[R0], g0[R0], k0[R0];
+    // plane offsets of its R2 cells (Ti staging, row by row), or -1 outside the domain;
+    // for its R1 cells (first sweep) their place in the R2 and R1 ring planes, plane
+    // offset (-1 outside) and what the sweep does there; for its tile cells (second
+    // sweep) their place in the R1 ring planes, plane offset and what the sweep does
+    // there.  Cell kinds: 0 outside the domain, 1 an oven (kept), 2 updated; bits 2-3 the
+    // x class + 1, bit 4 below the top.  The sweeps visit a region's cells inner-first
+    // (region_cell): the cells of the one formula then fill whole wavefronts, which run
+    // it without class dispatch.
+    int off2[R2], c2r[R1], q1[R1], o1[R1], k1[R1], c1r[R0], g0[R0], k0[R0];
 #pragma unroll
     for (int r = 0; r < R2; r++) {
         const int t = tid + 256 * r;
@@ -177,36 +215,74 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
         if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) return 0;
         return (jacobi_updates(c, gx, gy, nx, ny) ? 2 : 1) | ((jacobi_cls(gx, nx) + 1) << 2) | ((gy < ny - 1) << 4);
     };
+    constexpr int kInnerKind = 2 | (1 << 2) | (1 << 4);   // updated, x inside, below the top
+    // inner cells (x in [1, nx-2], y in [1, ny-2]) of R1 and of the tile, in region coordinates
+    const int a1x = max(0, 2 - x0), b1x = max(a1x, min(W1, nx - x0)), a1y = max(0, 2 - y0),
+              b1y = max(a1y, min(H1, ny - y0));
+    const int a0x = max(0, 1 - x0), b0x = max(a0x, min(TX, nx - 1 - x0)), a0y = max(0, 1 - y0),
+              b0y = max(a0y, min(TY, ny - 1 - y0));
 #pragma unroll
     for (int r = 0; r < R1; r++) {
         const int t = tid + 256 * r;
-        const int row = t / W1, col = t - row * W1;
+        int row = 0, col = 0;
+        if (t < N1) region_cell(t, W1, H1, a1x, b1x, a1y, b1y, row, col);
         const int gx = x0 - 1 + col, gy = y0 - 1 + row;
-        off1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
-        c2r[r] = (row + 1) * W2 + col + 1;                 // the same cell in R2
+        c2r[r] = (row + 1) * W2 + col + 1;                 // the cell in R2
+        q1[r] = row * W1 + col;                            // and in R1
+        o1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
         k1[r] = t < N1 ? kind(gx, gy) : 0;
     }
 #pragma unroll
     for (int r = 0; r < R0; r++) {
         const int t = tid + 256 * r;
-        const int row = t / TX, col = t - row * TX;
+        int row = 0, col = 0;
+        if (t < N0) region_cell(t, TX, TY, a0x, b0x, a0y, b0y, row, col);
         const int gx = x0 + col, gy = y0 + row;
-        c1r[r] = (row + 1) * W1 + col + 1;                 // the same cell in R1
+        c1r[r] = (row + 1) * W1 + col + 1;                 // the cell in R1
         g0[r] = gy * nx + gx;
         k0[r] = t < N0 ? kind(gx, gy) : 0;
     }
-    float pre[R2], qpre[R1];
-    auto fetch = [&](int k) {                        // Ti plane k over R2, Q plane k-1 over R1
+    // per slot, whether every lane of the wavefront holds an inner cell (wave-uniform: the
+    // sweeps then run the inner formula with no per-lane class branches)
+    bool f1[R1], f0[R0];
 #pragma unroll
-        for (int r = 0; r < R2; r++) pre[r] = (k >= 0 && k < hz && off2[r] >= 0) ? Ti[(size_t)k * sz + off2[r]] : 0.f;
+    for (int r = 0; r < R1; r++)
+        f1[r] = __builtin_amdgcn_ballot_w64(tid + 256 * r < N1 && k1[r] == kInnerKind) == ~0ull;
 #pragma unroll
-        for (int r = 0; r < R1; r++)
-            qpre[r] = (k >= 1 && k <= hz && off1[r] >= 0) ? Q[(size_t)(k - 1) * sz + off1[r]] : 0.f;
+    for (int r = 0; r < R0; r++)
+        f0[r] = __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
+
+    // Loads of step k (issued during step k-1): Ti plane k over R2, Q plane k-1 at the R1
+    // cells and Q plane k-2 at the tile cells.  Out-of-domain cells load from the plane
+    // origin and keep 0; planes outside [0, hz) are not loaded.
+    float pre[R2], qn1[R1], qn0[R0];
+    auto fetch = [&](int k) {
+        if (k >= 0 && k < hz) {
+            const float *pl = Ti + (size_t)k * sz;
+#pragma unroll
+            for (int r = 0; r < R2; r++) {
+                const float v = pl[max(off2[r], 0)];
+                pre[r] = off2[r] >= 0 ? v : 0.f;
+            }
+        }
+        if (k >= 1 && k <= hz) {
+            const float *pl = Q + (size_t)(k - 1) * sz;
+#pragma unroll
+            for (int r = 0; r < R1; r++) {
+                const float v = pl[max(o1[r], 0)];
+                qn1[r] = o1[r] >= 0 ? v : 0.f;
+            }
+        }
+        if (k >= 2 && k <= hz + 1) {
+            const float *pl = Q + (size_t)(k - 2) * sz;
+#pragma unroll
+            for (int r = 0; r < R0; r++) qn0[r] = pl[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
+        }
     };
     double res = 0.0;
     fetch(za - 2);
     for (int k = za - 2; k <= zb + 1; k++) {
-        // (a) stage Ti plane k and Q plane k-1, then load the next ones
+        // (a) stage Ti plane k, take this step's Q values, then load the next step's
         if (k >= 0 && k < hz) {
             float *dst = Li[(k + 3) % 3];
 #pragma unroll
@@ -215,41 +291,38 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
                 if (t < N2) dst[t] = pre[r];
             }
         }
-        if (k >= 1 && k <= hz) {
-            float *dst = Lq[(k + 2) % 3];
+        float q1c[R1], q0c[R0];
 #pragma unroll
-            for (int r = 0; r < R1; r++) {
-                const int t = tid + 256 * r;
-                if (t < N1) dst[t] = qpre[r];
-            }
-        }
+        for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
+#pragma unroll
+        for (int r = 0; r < R0; r++) q0c[r] = qn0[r];
         if (k + 1 <= zb + 1) fetch(k + 1);
         __syncthreads();
         // (b) first sweep, plane p1 = k - 1, over R1
         const int p1 = k - 1;
         if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
             const float *lm = Li[(p1 + 2) % 3], *l0 = Li[(p1 + 3) % 3], *lp = Li[(p1 + 4) % 3];
-            const float *lq = Lq[(p1 + 3) % 3];
             float *dst = L1[(p1 + 3) % 3];
             const int zc1 = jacobi_cls(p1, hz);
 #pragma unroll
             for (int r = 0; r < R1; r++) {
                 const int t = tid + 256 * r;
-                if (t < N1) {
-                    const int c2 = c2r[r], kd = k1[r];
+                const int c2 = c2r[r], kd = k1[r], p = q1[r];
+                auto ld = [&](int dx, int dy, int dz) {
+                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                    return pl[c2 + dy * W2 + dx];
+                };
+                auto lqv = [&] { return q1c[r]; };
+                if (zc1 == 0 && f1[r]) {                                 // a wavefront of inner cells
+                    dst[p] = jacobi_voxel(c, 0, true, 0, ld, lqv);
+                } else if (t < N1) {
                     float v = 0.f;
                     if ((kd & 3) == 2) {
-                        v = jacobi_voxel(
-                            c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc1,
-                            [&](int dx, int dy, int dz) {
-                                const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
-                                return pl[c2 + dy * W2 + dx];
-                            },
-                            [&] { return lq[t]; });
+                        v = jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc1, ld, lqv);
                     } else if ((kd & 3) == 1) {
                         v = l0[c2];                                      // ovens keep their value
                     }
-                    dst[t] = v;
+                    dst[p] = v;
                 }
             }
         }
@@ -258,19 +331,19 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
         const int p2 = k - 2;
         if (p2 >= za && p2 < zb) {
             const float *lm = L1[(p2 + 2) % 3], *l0 = L1[(p2 + 3) % 3], *lp = L1[(p2 + 4) % 3];
-            const float *lq = Lq[(p2 + 3) % 3];
             const int zc2 = jacobi_cls(p2, hz);
 #pragma unroll
             for (int r = 0; r < R0; r++) {
                 const int c1 = c1r[r], kd = k0[r];
-                if ((kd & 3) == 2) {
-                    const float out = jacobi_voxel(
-                        c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2,
-                        [&](int dx, int dy, int dz) {
-                            const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
-                            return pl[c1 + dy * W1 + dx];
-                        },
-                        [&] { return lq[c1]; });
+                auto ld = [&](int dx, int dy, int dz) {
+                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                    return pl[c1 + dy * W1 + dx];
+                };
+                auto lqv = [&] { return q0c[r]; };
+                const bool fast = zc2 == 0 && f0[r];                     // a wavefront of inner cells
+                if (fast || (kd & 3) == 2) {
+                    const float out = fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
+                                           : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
                     Tf[(size_t)p2 * sz + g0[r]] = out;
                     if (RESID) {
                         const double d = (double)(out - l0[c1]);

@@ -1,8 +1,9 @@
 """CPU check of the two-sweep Jacobi pass's schedule (k_jacobi2 in
 shg_focused_gaussian_beams_amd/csrc/thermal_kernels.hip), restated step for step: the
 XCD-dealt block order, the tile / z-chunk decomposition, the LDS rings of three planes of
-Ti (halo 2) and of the first sweep (halo 1), the staging at march step k, the first sweep
-of plane k-1 and the second of plane k-2.  A toy stencil that reads every in-domain
+Ti (halo 2) and of the first sweep (halo 1), the staging at march step k, the Q values
+loaded into registers one step ahead, the first sweep of plane k-1 and the second of
+plane k-2.  A toy stencil that reads every in-domain
 neighbour stands in for the Jacobi formulas (which the kernel shares with the
 single-sweep kernel); the schedule must give two plain sweeps exactly, read only staged
 ring entries, and stay inside its rings — on grids that are not multiples of the tile,
@@ -60,16 +61,33 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
         ty, tx = divmod(tile, tiles_x)
         x0, y0 = tx*TX, ty*TY
         za, zb = chunk*zc, min(hz, chunk*zc + zc)
-        Li = np.full((3, N2), np.nan); L1 = np.full((3, N1), np.nan); Lq = np.full((3, N1), np.nan)
-        for k in range(za-2, zb+2):
+        Li = np.full((3, N2), np.nan); L1 = np.full((3, N1), np.nan)
+        # registers: Ti plane k over R2, Q plane k-1 at the R1 cells, Q plane k-2 at the
+        # tile cells, each loaded one step ahead (NaN: not loaded)
+        nxt = {}
+
+        def fetch(k):
+            pre = np.full(N2, np.nan); qn1 = np.full(N1, np.nan); qn0 = np.full(TX*TY, np.nan)
             if 0 <= k < hz:
                 for t in range(N2):
                     row, col = divmod(t, W2); gx, gy = x0-2+col, y0-2+row
-                    Li[(k+3) % 3, t] = Ti[k, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+                    pre[t] = Ti[k, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
             if 1 <= k <= hz:
                 for t in range(N1):
                     row, col = divmod(t, W1); gx, gy = x0-1+col, y0-1+row
-                    Lq[(k+2) % 3, t] = Q[k-1, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+                    qn1[t] = Q[k-1, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+            if 2 <= k <= hz + 1:
+                for t in range(TX*TY):
+                    row, col = divmod(t, TX); gx, gy = x0+col, y0+row
+                    qn0[t] = Q[k-2, gy, gx] if (gx < nx and gy < ny) else np.nan
+            nxt.update(pre=pre, qn1=qn1, qn0=qn0)
+        fetch(za-2)
+        for k in range(za-2, zb+2):
+            if 0 <= k < hz:
+                Li[(k+3) % 3] = nxt["pre"]
+            q1c, q0c = nxt["qn1"], nxt["qn0"]
+            if k+1 <= zb+1:
+                fetch(k+1)
             p1 = k-1
             if max(za-1, 0) <= p1 <= min(zb, hz-1):
                 for t in range(N1):
@@ -81,7 +99,7 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
                             def ld(dx, dy, dz):
                                 i = c2 + dy*W2 + dx; assert 0 <= i < N2
                                 val = Li[(p1+dz+3) % 3, i]; assert not np.isnan(val), (b, k, gx, gy, dx, dy, dz); return val
-                            q = Lq[(p1+3) % 3, t]; assert not np.isnan(q)
+                            q = q1c[t]; assert not np.isnan(q)
                             v = voxel(ld, gx, gy, p1, nx, ny, hz) + q
                         else:
                             v = Li[(p1+3) % 3, c2]
@@ -95,7 +113,7 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
                         def ld(dx, dy, dz):
                             i = c1 + dy*W1 + dx; assert 0 <= i < N1
                             val = L1[(p2+dz+3) % 3, i]; assert not np.isnan(val); return val
-                        q = Lq[(p2+3) % 3, c1]; assert not np.isnan(q)
+                        q = q0c[t]; assert not np.isnan(q)
                         Tf[p2, gy, gx] = voxel(ld, gx, gy, p2, nx, ny, hz) + q
     assert len(seen) == nb
     return Tf_all
@@ -137,3 +155,46 @@ def test_batched_two_sweep_schedule(count, ids, nx, ny, hz, TX, TY, zc):
             assert np.array_equal(em[i], sweep(sweep(Ti[i], Q[i], _top), Q[i], _top)), i
         else:
             assert np.isnan(em[i]).all(), i
+
+
+def region_cell(i, W, H, ax, bx, ay, by):
+    """region_cell of thermal_kernels.hip: cell i of a W x H region, inner rectangle first."""
+    wi, hi = bx - ax, by - ay
+    ni = wi * hi
+    if i < ni:
+        return ay + i // wi, ax + i % wi
+    i -= ni
+    if i < ay * W:
+        return i // W, i % W
+    i -= ay * W
+    if i < (H - by) * W:
+        return by + i // W, i % W
+    i -= (H - by) * W
+    if i < ax * hi:
+        return ay + i // ax, i % ax
+    i -= ax * hi
+    wr = W - bx
+    return ay + i // wr, bx + i % wr
+
+
+@pytest.mark.parametrize("nx,ny,TX,TY", [(256, 256, 64, 8), (64, 64, 64, 16), (512, 512, 128, 8), (128, 96, 64, 8),
+                                         (64, 64, 128, 8)])
+def test_inner_first_cell_order(nx, ny, TX, TY):
+    # the kernel's cell order over R1 (halo 1) and over the tile, at every tile position:
+    # a bijection onto the region whose first cells are exactly the inner cells
+    # (x in [1, nx-2], y in [1, ny-2]), so whole wavefronts share one formula
+    W1, H1 = TX + 2, TY + 2
+    for y0 in range(0, ny, TY):
+        for x0 in range(0, nx, TX):
+            for (W, H, ox, oy, ax, bx, ay, by) in (
+                    (W1, H1, x0 - 1, y0 - 1, max(0, 2 - x0), None, max(0, 2 - y0), None),
+                    (TX, TY, x0, y0, max(0, 1 - x0), None, max(0, 1 - y0), None)):
+                if W == W1:
+                    bx, by = max(ax, min(W1, nx - x0)), max(ay, min(H1, ny - y0))
+                else:
+                    bx, by = max(ax, min(TX, nx - 1 - x0)), max(ay, min(TY, ny - 1 - y0))
+                cells = [region_cell(i, W, H, ax, bx, ay, by) for i in range(W * H)]
+                assert sorted(cells) == [(r, c) for r in range(H) for c in range(W)]
+                inner = [1 <= ox + c <= nx - 2 and 1 <= oy + r <= ny - 2 for r, c in cells]
+                ni = sum(inner)
+                assert all(inner[:ni]) and not any(inner[ni:]), (x0, y0, W)
