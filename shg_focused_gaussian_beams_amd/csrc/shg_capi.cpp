@@ -235,9 +235,11 @@ int enqueue_march_thermal(shg_ctx *c, float4 *snaps = nullptr) {
     const float2 *NT = c->ntab;
     const int tv = c->row_v < 0 ? 0 : c->row_v;
     HIP_TRY(shg::launch_thermal_row_pass(c->n, c->U, c->Uout, c->tw, NT, c->Q, c->DKint, a, c->stream, tv));
+    int w = 0;                                      // DKint holds the prefix of step w
     for (int s = 0; s < nz; s++) {
         HIP_TRY(shg::launch_col_pass(c->n, c->d_desc, 1, c->tw, c->stream, c->col_v < 0 ? 0 : c->col_v));
         a.prev = therm_step(c, s);
+        a.dk_pending = s - w;
         float4 *snap = c->Uout;
         if (s < nz - 1) {
             a.cur = therm_step(c, s + 1);
@@ -246,6 +248,11 @@ int enqueue_march_thermal(shg_ctx *c, float4 *snaps = nullptr) {
             // steps of a slice overwrite the earlier ones (heat grid coarser than the march)
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_NCUR | shg::TROW_BWD;
             if (a.prev.slice != a.cur.slice) a.flags |= shg::TROW_QPREV;
+            // DKint is stored where the heat slice changes (and at the last step); within
+            // a slice the next pass re-adds the slice's dk to the stored prefix
+            if (a.prev.slice == a.cur.slice && s + 1 < nz - 1 && s + 1 - w < shg::kMaxDkPending)
+                a.flags |= shg::TROW_DKKEEP;
+            else w = s + 1;
             if (s + 1 == nz - 1 || snaps) a.flags |= shg::TROW_SNAP;
             if (snaps) snap = snaps + (size_t)(s + 1) * n2;     // the state before step s+1
         } else {

@@ -170,13 +170,18 @@ int enqueue_march(shg_thermal_batch *b) {
     a.flags = shg::TROW_FIRST | shg::TROW_NCUR | shg::TROW_BWD;
     const int rv = b->row_v < 0 ? 0 : b->row_v, cv = b->col_v < 0 ? 0 : b->col_v;
     TB_TRY(shg::launch_thermal_row_pass(b->n, b->U, b->Uout, b->tw, b->NT, b->Q, b->DKint, a, b->stream, rv, b->count));
+    int w = 0;                                      // DKint holds the prefix of step w
     for (int s = 0; s < b->nz; s++) {
         TB_TRY(shg::launch_col_pass(b->n, b->desc, b->count, b->tw, b->stream, cv));
         a.prev = shg_internal_therm_step_of(p, b->hz, s);
+        a.dk_pending = s - w;
         if (s < b->nz - 1) {
             a.cur = shg_internal_therm_step_of(p, b->hz, s + 1);
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_NCUR | shg::TROW_BWD;
             if (a.prev.slice != a.cur.slice) a.flags |= shg::TROW_QPREV;
+            if (a.prev.slice == a.cur.slice && s + 1 < b->nz - 1 && s + 1 - w < shg::kMaxDkPending)
+                a.flags |= shg::TROW_DKKEEP;
+            else w = s + 1;
             if (s + 1 == b->nz - 1) a.flags |= shg::TROW_SNAP;
         } else {
             a.flags = shg::TROW_FWD | shg::TROW_NPREV | shg::TROW_QPREV;

@@ -46,16 +46,23 @@ struct ThermStep {
 
 enum ThermRowFlags : int {
     TROW_FWD = 1, TROW_NPREV = 2, TROW_QPREV = 4, TROW_SNAP = 8, TROW_NCUR = 16, TROW_BWD = 32, TROW_FIRST = 64,
-    TROW_TAPE = 128
+    TROW_TAPE = 128, TROW_DKKEEP = 256
 };
 
 // With TROW_TAPE the pass also records, for the adjoint, the fields before the second RK4
 // of step prev.step into tape_a2[prev.step - tape_s0] and DKint of step cur.step into
 // tape_dk[cur.step - tape_s0] (n*n entries per step).
+// The DKint array may lag: it holds the prefix of an earlier step w, and dk_pending =
+// prev.step - w further additions of dk(T(slice(prev))) (every step in (w, prev.step]
+// maps to that slice) give DKint_{prev}.  With TROW_DKKEEP the pass does not store
+// DKint_cur (the next pass derives it the same way); stored or derived, the float
+// additions are the same, in the same order.
+constexpr int kMaxDkPending = 8;                  // a stored prefix lags at most this many steps
 struct ThermRowArgs {
     ThermStep prev, cur;
     ThermConst k;
     int flags;
+    int dk_pending;
     int tape_s0;
     float4 *tape_a2;
     float *tape_dk;

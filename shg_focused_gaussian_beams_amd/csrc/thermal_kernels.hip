@@ -461,6 +461,7 @@ __device__ __forceinline__ bool therm_batch_skip(float4 *__restrict__ &U, float4
 //   QPREV heat source of slice(s-1) from the fields after step s-1
 //   SNAP  copy of the fields after step s-1 to Uout (the reference's Pump_out state)
 //   NCUR  DKint_s = DKint_{s-1} + dk(T(slice(s))), then the first RK4 of step s
+//         (stored unless DKKEEP; a lagging stored prefix is brought up by dk_pending adds)
 //   BWD   R_bwd (start of step s's diffraction)
 // DKint holds the running float prefix sum of dk over the executed steps: the reference
 // recomputes the same float sum from z' = 0 twice per step (src/headers/PhaseMatching.h:
@@ -476,6 +477,9 @@ __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx,
     CellPhys cp{};
     if (a.flags & (TROW_NPREV | TROW_QPREV)) {
         cp = cell_phys(k, nnp);
+#pragma unroll
+        for (int i = 0; i < kMaxDkPending; i++)                         // the steps since the stored prefix
+            if (i < a.dk_pending) dkprev = dkprev + cp.dk;
         if ((a.flags & (TROW_TAPE | TROW_NPREV)) == (TROW_TAPE | TROW_NPREV))
             a.tape_a2[(size_t)(a.prev.step - a.tape_s0) * n2 + idx] = make_float4(P.x, P.y, S.x, S.y);
         if (a.flags & TROW_NPREV) rk4_thermal(P, S, cp, dkprev, a.prev, k);
@@ -485,7 +489,7 @@ __device__ __forceinline__ void therm_cell_pre(float2 &P, float2 &S, size_t idx,
     if (a.flags & TROW_NCUR) {
         const CellPhys cc = same ? cp : cell_phys(k, nnc);
         const float dk = dkprev + cc.dk;
-        DKint[idx] = dk;
+        if (!(a.flags & TROW_DKKEEP)) DKint[idx] = dk;
         if (a.flags & TROW_TAPE) a.tape_dk[(size_t)(a.cur.step - a.tape_s0) * n2 + idx] = dk;
         rk4_thermal(P, S, cc, dk, a.cur, k);
     }
