@@ -47,7 +47,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per THERMAL z-step (row + column pass, 256x256, the tuned layouts) from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the factors read for a copy
 # of known size.
-TRAFFIC_THERMAL_STEP = 5554 * 1024
+TRAFFIC_THERMAL_STEP = 5369 * 1024
 B_PLAIN, B_THERMAL, B_FWDADJ, B_JACOBI = 32, 56, 80, 12
 # VALU lane-operations per plain cell-step of the batch's passes (rocprofv3 SQ_INSTS_VALU
 # x 64 per cell: 393 row pass + 141 column pass at 512^2) and the chip's FP32 issue rate
@@ -424,6 +424,39 @@ def bench_thermal_sweep(shg, args):
                 eta_min=float(eta_b.min()), eta_max=float(eta_b.max()), t1_of_eta_max=t1s[int(np.argmax(eta_b))])
 
 
+def bench_jacobi_layouts(shg, args):
+    """Every Jacobi layout (the nine two-sweep passes and single sweeps) on four heat
+    grids: the reference's default run (64x64x200), the THERMAL workload (256x256x100),
+    heat grid = field grid (256x256x400) and the inverse design's (512x512x200); us per
+    sweep over 200 sweeps each, and the tuner's pick."""
+    import torch
+    grids = [("64x64x200", shg.default_params(thermal=1)), ("256x256x100", shg.thermal256_params(heat_nz=100)),
+             ("256x256x400", shg.thermal256_params(heat_nz=0)), ("512x512x200", shg.inverse512_params())]
+    out = {}
+    for name, p in grids:
+        if args.jacobi_grid and name != args.jacobi_grid:
+            continue
+        n = 200
+        with shg.Context(p) as ctx:
+            per = []
+            for layout in range(shg.SHG_JACOBI_LAYOUTS):
+                ctx.set_jacobi_layout(layout)
+                ctx.jacobi(4, residual=False)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ctx.jacobi(n, residual=False)
+                torch.cuda.synchronize()
+                per.append((time.perf_counter() - t0) / n * 1e6)
+        with shg.Context(p) as ctx:
+            ctx.jacobi(2, residual=False)
+            tuned = ctx.jacobi_layout()
+        out[name] = {"us_per_sweep": per, "single_sweep_us": per[-1], "best_two_sweep_us": min(per[:-1]),
+                     "best_two_sweep_layout": per.index(min(per[:-1])), "tuned_layout": tuned,
+                     "two_sweep_speedup": per[-1] / min(per[:-1])}
+        log(f"jacobi {name}: single {per[-1]:.2f} us, best two-sweep {min(per[:-1]):.2f} us, tuned {tuned}")
+    return out
+
+
 def cpu_small_workloads(shg, default_run_stats):
     """oracle32 -O3 on one pinned core: the plain 128x128, nz 200 workload in full, the
     plain forward of the gradient workload (256x256, nz 400) in full, the Jacobi sweep on
@@ -614,8 +647,10 @@ def main():
     ap.add_argument("--inverse-instances", type=int, default=8)
     ap.add_argument("--inverse-iterations", type=int, default=2)
     ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse", "inverse-full",
-                                          "thermal-sweep"], default="all",
+                                          "thermal-sweep", "jacobi"], default="all",
                     help="run one extra workload alone and print its figures (for profiler runs)")
+    ap.add_argument("--jacobi-grid", default="",
+                    help="with --section jacobi: one heat grid only (64x64x200, 256x256x100, 256x256x400, 512x512x200)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB 5 times (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -649,6 +684,8 @@ def main():
             r = bench_inverse_full(shg, args)
         elif args.section == "thermal-sweep":
             r = bench_thermal_sweep(shg, args)
+        elif args.section == "jacobi":
+            r = bench_jacobi_layouts(shg, args)
         else:
             r = bench_adjoint(shg, args)
             r["res"] = {k: v for k, v in r["res"].items()}
