@@ -207,9 +207,9 @@ int shg_jacobi(shg_ctx *ctx, int64_t nsweeps, double *mean_sq_update);
  * `mean_sq_update` (a device double) is not NULL it receives the last sweep's mean
  * squared update, as shg_jacobi returns it.  Work is enqueued on `stream` (a
  * hipStream_t; NULL: the null stream) and the call returns without waiting.  It calls no
- * allocating routine: the residual's partial sums live in a fixed 2 MiB array of the
+ * allocating routine: the residual's partial sums live in a fixed 256 KiB array of the
  * library on each device (resolved on the device's first call), which limits the heat
- * grid to 2^18 single-sweep partials, 512 x 512 x 256 voxels (larger grids with a
+ * grid to 2^15 single-sweep partials, 512 x 512 x 256 voxels (larger grids with a
  * residual are rejected with SHG_E_INVALID).  Every call on a device shares that array:
  * calls that overlap on one GPU (other threads, other streams) race. */
 int shg_jacobi_noalloc(const shg_params *p, float *T, float *T_work, const float *Q, int64_t nsweeps, int layout,

@@ -15,9 +15,10 @@
 namespace {
 
 // Residual partials (one FP64 sum per workgroup of the sweep that reports the residual)
-// and, in the last slot, their total: the single-sweep kernel writes one per 64 x 4 x 1
-// block, so 2^18 partials cover every heat grid up to 512 x 512 x 256.
-constexpr int kScratchParts = 1 << 18;
+// and, in the last slot, their total: the single-sweep kernel writes one per workgroup
+// of 64 x 4 columns of jacobi_planes planes (8 on a 512 x 512 grid), so 2^15 partials
+// cover every heat grid up to 512 x 512 x 256.
+constexpr int kScratchParts = 1 << 15;
 constexpr int kMaxDevices = 64;
 
 static __device__ __align__(256) double scratch[kScratchParts + 1];
@@ -88,7 +89,7 @@ extern "C" int shg_jacobi_noalloc(const shg_params *p, float *T, float *T_work, 
         const int last_parts = (pairs && nsweeps % 2 == 0) ? plan.blocks : shg::jacobi_blocks(n, n, hz);
         if (mean_sq_update && nsweeps > 0 && last_parts > kScratchParts)
             return shg_internal_fail(SHG_E_INVALID, "shg_jacobi_noalloc: the heat grid is larger than the residual "
-                                                    "scratch (at most 2^18 partials: 512 x 512 x 256)");
+                                                    "scratch (at most 2^15 partials: 512 x 512 x 256)");
         double *scr = nullptr;
         if (int rc = scratch_on_device(&scr)) return rc;
         hipStream_t st = (hipStream_t)stream;

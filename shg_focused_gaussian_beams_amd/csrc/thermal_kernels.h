@@ -109,7 +109,9 @@ constexpr int kThermRowVariantCount = 7;
 hipError_t launch_thermal_row_pass(int n, float4 *U, float4 *Uout, const float2 *tw, const float2 *NT, float *Q,
                                    float *DKint, const ThermRowArgs &a, hipStream_t st, int variant = 0,
                                    int count = 1);
-// Grid size the Jacobi launcher uses (number of residual partials it writes).
+// Planes per thread of the single sweep on a heat grid (1, 2, 4 or 8), and the grid size
+// the single-sweep launcher uses (number of residual partials it writes).
+int jacobi_planes(int nx, int ny, int hz);
 int jacobi_blocks(int nx, int ny, int hz);
 
 }  // namespace shg

@@ -78,31 +78,49 @@ __device__ __forceinline__ bool jacobi_updates(const JacobiConst &c, int x, int 
     return y >= 1;
 }
 
-// One Jacobi sweep Tf <- J(Ti, Q), thread per voxel (64 x 4 voxels per workgroup, one z
-// plane).  Voxels the sweep does not update are never written.  With RESID the FP64 sum
-// of (Tf - Ti)^2 over the block goes to part[block].  Used for single sweeps (the
-// residual checks of the reference's loop, odd sweep counts).
-template <bool RESID>
+// One Jacobi sweep of the voxels (x, y, z0 .. z0+ZB-1) of one thread (a column of ZB
+// planes: the planes' z neighbours are shared, so a block reads each of its planes once
+// and its two halo planes again, within its own lifetime); returns the FP64 sum of
+// (Tf - Ti)^2 over them with RESID.  Voxels the sweep does not update are never written.
+template <int ZB, bool RESID>
+__device__ __forceinline__ double jacobi_column(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                const float *__restrict__ Q, int nx, int ny, int hz, int x, int y,
+                                                int z0, const JacobiConst &c) {
+    double r = 0.0;
+    if (x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
+        const ptrdiff_t sy = nx, sz = (ptrdiff_t)nx * ny;
+        const int xc = jacobi_cls(x, nx);
+        const bool yi = y < ny - 1;
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = z0 + j;
+            if (z < hz) {
+                const ptrdiff_t i = z * sz + y * sy + x;
+                const float out = jacobi_voxel(
+                    c, xc, yi, jacobi_cls(z, hz), [&](int dx, int dy, int dz) { return Ti[i + dx + dy * sy + dz * sz]; },
+                    [&] { return Q[i]; });
+                Tf[i] = out;
+                if (RESID) {
+                    const double d = (double)(out - Ti[i]);
+                    r += d * d;
+                }
+            }
+        }
+    }
+    return r;
+}
+
+// One Jacobi sweep Tf <- J(Ti, Q): a workgroup of 64 x 4 threads, each a column of ZB
+// planes (jacobi_column; ZB = jacobi_planes of the grid).  With RESID the FP64 sum of
+// (Tf - Ti)^2 over the block goes to part[block].  Used for single sweeps (the residual
+// checks of the reference's loop, odd sweep counts) and as a layout of the sweep bursts.
+template <int ZB, bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                 const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
                                                 double *__restrict__ part) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
-    double r = 0.0;
-    if (x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
-        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-        const size_t i = (size_t)z * sz + (size_t)y * sy + x;
-        const float out = jacobi_voxel(
-            c, jacobi_cls(x, nx), y < ny - 1, jacobi_cls(z, hz),
-            [&](int dx, int dy, int dz) { return Ti[(ptrdiff_t)i + dx + (ptrdiff_t)dy * (ptrdiff_t)sy + (ptrdiff_t)dz * (ptrdiff_t)sz]; },
-            [&] { return Q[i]; });
-        Tf[i] = out;
-        if (RESID) {
-            const double d = (double)(out - Ti[i]);
-            r = d * d;
-        }
-    }
+    double r = jacobi_column<ZB, RESID>(Tf, Ti, Q, nx, ny, hz, x, y, blockIdx.z * ZB, c);
     if (RESID) {
         for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
         __shared__ double w[4];
@@ -377,35 +395,21 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
 
 // Batched single sweep: slot blockIdx.y runs instance ids[blockIdx.y]; blockIdx.x is the
 // block of k_jacobi's grid flattened (x fastest, then y, then z), so that a slot's
-// partials (at blockIdx.y * gridDim.x) come in k_jacobi's order; the same voxel formulas.
-template <bool RESID>
+// partials (at blockIdx.y * gridDim.x) come in k_jacobi's order; the same columns.
+template <int ZB, bool RESID>
 __global__ __launch_bounds__(256) void k_jacobi_batch(float *__restrict__ Tf, const float *__restrict__ Ti,
                                                       const float *__restrict__ Q, int nx, int ny, int hz,
                                                       const int *__restrict__ ids, JacobiConst c,
                                                       double *__restrict__ part) {
-    const int inst = ids[blockIdx.y];
+    const size_t inst = (size_t)ids[blockIdx.y];
     const int gx = nx / 64, gy = (ny + 3) / 4;
     const int bz = (int)blockIdx.x / (gx * gy), rxy = (int)blockIdx.x - bz * gx * gy;
     const int by = rxy / gx, bx = rxy - by * gx;
-    const int z = bz;
-    const int x = bx * 64 + (threadIdx.x & 63);
-    const int y = by * 4 + (threadIdx.x >> 6);
-    double r = 0.0;
-    if (x < nx && y < ny && jacobi_updates(c, x, y, nx, ny)) {
-        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-        const size_t i = (size_t)inst * sz * hz + (size_t)z * sz + (size_t)y * sy + x;
-        const float out = jacobi_voxel(
-            c, jacobi_cls(x, nx), y < ny - 1, jacobi_cls(z, hz),
-            [&](int dx, int dy, int dz) { return Ti[(ptrdiff_t)i + dx + (ptrdiff_t)dy * (ptrdiff_t)sy + (ptrdiff_t)dz * (ptrdiff_t)sz]; },
-            [&] { return Q[i]; });
-        Tf[i] = out;
-        if (RESID) {
-            const double d = (double)(out - Ti[i]);
-            r = d * d;
-        }
-    }
+    const size_t off = inst * (size_t)nx * ny * hz;
+    double r = jacobi_column<ZB, RESID>(Tf + off, Ti + off, Q + off, nx, ny, hz, bx * 64 + (threadIdx.x & 63),
+                                        by * 4 + (threadIdx.x >> 6), bz * ZB, c);
     if (RESID) {
-        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        for (int o = 32; o > 0; o >>= 1) r += __shfl_down(r, o, 64);
         __shared__ double w[4];
         if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = r;
         __syncthreads();
@@ -692,17 +696,42 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-int jacobi_blocks(int nx, int ny, int hz) { return (nx / 64) * ((ny + 3) / 4) * hz; }
+// Planes per thread of the single sweep: the most of {8, 4, 2} that still gives the grid
+// 4096 workgroups (16 per CU), else 1.  A rule of the grid alone (not timed), so every
+// context and batch of one grid sums its residual partials alike.  Measured per sweep
+// (planes 1 / 2 / 4 / 8): 64x64x200 5.3 / 5.4 / 6.0 / 8.0 us, 256x256x100 22.1 / 21.7 /
+// 20.6 / 21.4 us, 512x512x200 214 / 201 / 189 / 184 us.
+int jacobi_planes(int nx, int ny, int hz) {
+    const long cols = (long)(nx / 64) * ((ny + 3) / 4);
+    for (int zb = 8; zb > 1; zb >>= 1)
+        if (cols * ((hz + zb - 1) / zb) >= 4096) return zb;
+    return 1;
+}
+
+int jacobi_blocks(int nx, int ny, int hz) {
+    const int zb = jacobi_planes(nx, ny, hz);
+    return (nx / 64) * ((ny + 3) / 4) * ((hz + zb - 1) / zb);
+}
+
+template <int ZB>
+static void jacobi_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
+                          double *part, hipStream_t st) {
+    const dim3 grid(nx / 64, (ny + 3) / 4, (hz + ZB - 1) / ZB);
+    if (part)
+        hipLaunchKernelGGL((k_jacobi<ZB, true>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    else
+        hipLaunchKernelGGL((k_jacobi<ZB, false>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+}
 
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                          double *part, int nparts, hipStream_t st) {
     if (nx % 64 != 0 || hz < 2 || ny < 2) return hipErrorInvalidValue;
-    const dim3 grid(nx / 64, (ny + 3) / 4, hz);
-    if (part) {
-        if (nparts < jacobi_blocks(nx, ny, hz)) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
-    } else {
-        hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    if (part && nparts < jacobi_blocks(nx, ny, hz)) return hipErrorInvalidValue;
+    switch (jacobi_planes(nx, ny, hz)) {
+        case 8: jacobi_launch<8>(Tf, Ti, Q, nx, ny, hz, c, part, st); break;
+        case 4: jacobi_launch<4>(Tf, Ti, Q, nx, ny, hz, c, part, st); break;
+        case 2: jacobi_launch<2>(Tf, Ti, Q, nx, ny, hz, c, part, st); break;
+        default: jacobi_launch<1>(Tf, Ti, Q, nx, ny, hz, c, part, st); break;
     }
     return hipGetLastError();
 }
@@ -784,14 +813,25 @@ hipError_t launch_jacobi2_batch(float *Tf, const float *Ti, const float *Q, int 
     }
 }
 
+template <int ZB>
+static void jacobi_batch_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
+                                const int *ids, const JacobiConst &c, double *part, hipStream_t st) {
+    const dim3 grid((unsigned)jacobi_blocks(nx, ny, hz), (unsigned)nids);
+    if (part)
+        hipLaunchKernelGGL((k_jacobi_batch<ZB, true>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
+    else
+        hipLaunchKernelGGL((k_jacobi_batch<ZB, false>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
+}
+
 hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
                                const int *ids, const JacobiConst &c, double *part, hipStream_t st) {
     if (nx % 64 != 0 || hz < 2 || ny < 2 || nids < 1 || nids > 65535 || !ids) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)jacobi_blocks(nx, ny, hz), (unsigned)nids);
-    if (part)
-        hipLaunchKernelGGL(k_jacobi_batch<true>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
-    else
-        hipLaunchKernelGGL(k_jacobi_batch<false>, grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
+    switch (jacobi_planes(nx, ny, hz)) {
+        case 8: jacobi_batch_launch<8>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
+        case 4: jacobi_batch_launch<4>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
+        case 2: jacobi_batch_launch<2>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
+        default: jacobi_batch_launch<1>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
+    }
     return hipGetLastError();
 }
 

@@ -36,7 +36,11 @@ def _rand_T(p, seed, lo=26.0, hi=29.0):
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx,nz,hz,bc", [(64, 12, 0, SHG_BC_TOP_OPEN), (128, 9, 0, SHG_BC_TOP_OPEN),
                                          (64, 24, 8, SHG_BC_TOP_OPEN), (64, 12, 0, SHG_BC_OVEN_SURROUNDED),
-                                         (128, 20, 6, SHG_BC_OVEN_SURROUNDED)])
+                                         (128, 20, 6, SHG_BC_OVEN_SURROUNDED),
+                                         # grids where a single-sweep thread takes 2, 4 or 8 planes and
+                                         # the last block of planes is partial
+                                         (256, 37, 0, SHG_BC_TOP_OPEN), (128, 301, 0, SHG_BC_OVEN_SURROUNDED),
+                                         (512, 29, 0, SHG_BC_TOP_OPEN)])
 def test_jacobi_sweeps_bit_exact(gpu, nx, nz, hz, bc):
     p = _thermal_params(nx=nx, ny=nx, nz=nz, heat_nz=hz, thermal_bc=bc)
     hzz = hz if hz > 0 else nz
