@@ -93,8 +93,8 @@ hipError_t launch_jacobi2_batch(float *Tf, const float *Ti, const float *Q, int 
 hipError_t launch_sum_partials(const double *part, int nparts, double *out, hipStream_t st);
 // Batched THERMAL instances of one heat grid (instance i at T + i nx ny hz, likewise Tf,
 // Q): one single sweep of instances ids[0 .. nids); with `part`, the residual partials of
-// each (jacobi_blocks of them, in the order of ids), summed per entry of ids into out[j]
-// by launch_sum_partials_batch.
+// each (jacobi_blocks of them, in the order of ids, summed as a single instance's), summed
+// per entry of ids into out[j] by launch_sum_partials_batch.
 hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
                                const int *ids, const JacobiConst &c, double *part, hipStream_t st);
 hipError_t launch_sum_partials_batch(const double *part, int per_instance, int count, double *out, hipStream_t st);

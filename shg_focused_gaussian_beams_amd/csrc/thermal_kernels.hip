@@ -701,12 +701,13 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
 // context and batch of one grid sums its residual partials alike.  Measured per sweep
 // (planes 1 / 2 / 4 / 8): 64x64x200 5.3 / 5.4 / 6.0 / 8.0 us, 256x256x100 22.1 / 21.7 /
 // 20.6 / 21.4 us, 512x512x200 214 / 201 / 189 / 184 us.
-int jacobi_planes(int nx, int ny, int hz) {
-    const long cols = (long)(nx / 64) * ((ny + 3) / 4);
+static int planes_for(long cols, int hz) {
     for (int zb = 8; zb > 1; zb >>= 1)
         if (cols * ((hz + zb - 1) / zb) >= 4096) return zb;
     return 1;
 }
+
+int jacobi_planes(int nx, int ny, int hz) { return planes_for((long)(nx / 64) * ((ny + 3) / 4), hz); }
 
 int jacobi_blocks(int nx, int ny, int hz) {
     const int zb = jacobi_planes(nx, ny, hz);
@@ -816,7 +817,7 @@ hipError_t launch_jacobi2_batch(float *Tf, const float *Ti, const float *Q, int 
 template <int ZB>
 static void jacobi_batch_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
                                 const int *ids, const JacobiConst &c, double *part, hipStream_t st) {
-    const dim3 grid((unsigned)jacobi_blocks(nx, ny, hz), (unsigned)nids);
+    const dim3 grid((unsigned)((nx / 64) * ((ny + 3) / 4) * ((hz + ZB - 1) / ZB)), (unsigned)nids);
     if (part)
         hipLaunchKernelGGL((k_jacobi_batch<ZB, true>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, ids, c, part);
     else
@@ -826,7 +827,11 @@ static void jacobi_batch_launch(float *Tf, const float *Ti, const float *Q, int 
 hipError_t launch_jacobi_batch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, int nids,
                                const int *ids, const JacobiConst &c, double *part, hipStream_t st) {
     if (nx % 64 != 0 || hz < 2 || ny < 2 || nids < 1 || nids > 65535 || !ids) return hipErrorInvalidValue;
-    switch (jacobi_planes(nx, ny, hz)) {
+    // a sweep with residual partials takes a single instance's plane count, so that each
+    // instance's partials come as its own context's; the others count every instance's
+    // workgroups (the swept temperatures are the same whatever the count)
+    const long cols = (long)(nx / 64) * ((ny + 3) / 4);
+    switch (part ? jacobi_planes(nx, ny, hz) : planes_for(cols * nids, hz)) {
         case 8: jacobi_batch_launch<8>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
         case 4: jacobi_batch_launch<4>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
         case 2: jacobi_batch_launch<2>(Tf, Ti, Q, nx, ny, hz, nids, ids, c, part, st); break;
