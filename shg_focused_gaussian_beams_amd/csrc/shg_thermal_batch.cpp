@@ -42,12 +42,17 @@ namespace {
         if (rc_) return rc_;   \
     } while (0)
 
-// The fields that may differ between the instances of a batch: the beam and the ovens.
-bool same_build(shg_params a, shg_params b) {
-    for (shg_params *q : {&a, &b}) {
-        q->power = q->waist = q->focus = q->temp = q->t_peltier1 = q->t_peltier2 = 0.0;
-    }
-    return std::memcmp(&a, &b, sizeof(shg_params)) == 0;
+// Whether two instances share everything but the beam and the ovens (power, waist,
+// focus, temp, t_peltier1, t_peltier2), field by field (the struct's padding bytes are
+// not compared: callers need not clear them).
+bool same_build(const shg_params &a, const shg_params &b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.crystal == b.crystal && a.thermal == b.thermal &&
+           a.scheme == b.scheme && a.lx == b.lx && a.ly == b.ly && a.lz == b.lz && a.lambda_p == b.lambda_p &&
+           a.deff == b.deff && a.qpm_period == b.qpm_period && a.alpha_p == b.alpha_p && a.alpha_s == b.alpha_s &&
+           a.beta_s == b.beta_s && a.rho_p == b.rho_p && a.rho_s == b.rho_s && a.k_th == b.k_th &&
+           a.h_conv == b.h_conv && a.t_inf == b.t_inf && a.heat_nz == b.heat_nz && a.jacobi_check == b.jacobi_check &&
+           a.max_outer == b.max_outer && a.jacobi_max == b.jacobi_max && a.jacobi_tol == b.jacobi_tol &&
+           a.thermal_bc == b.thermal_bc;
 }
 
 }  // namespace
