@@ -237,8 +237,11 @@ typedef struct shg_adjoint_result {
 } shg_adjoint_result;
 
 /* Discrete adjoint of the z-march (Solver::CWES, src/headers/Solver.h:245-276) for
- * J = eta.  Checkpoints every `checkpoint_interval` steps (<= 0: ceil(sqrt(nz-1))), with
- * per-segment recomputation.  d_eta_d_fh (nx*ny complex64, may be null) receives the
+ * J = eta.  Checkpoints every `checkpoint_interval` steps, with per-segment recomputation;
+ * <= 0 tapes the whole march once (no recomputation) when that tape, about
+ * (nz-1)*nx*ny*32 bytes (36 in THERMAL), takes at most a quarter of the free device memory,
+ * and otherwise checkpoints every ceil(sqrt(nz-1)) steps; res->checkpoint_interval reports
+ * the choice.  d_eta_d_fh (nx*ny complex64, may be null) receives the
  * gradient with respect to the z = 0 pump field, dJ = Re sum conj(g) dA.  The focus and
  * the pump part of the temperature gradient assume the z = 0 pump is setPump at the
  * context's power, waist, focus and temperature (plus additive noise).

@@ -312,14 +312,41 @@ int enqueue_thermal_segment(shg_ctx *c, int s0, int s1, const shg::TapeArgs *tp,
 // uniform shift of the field; per voxel in AdjointBuffers::Ap).  The heat source is not
 // touched.
 // sens = false: the field gradient and eta only (no focus, temperature or dJ/dT map).
+// Checkpoint interval of the adjoint for `checkpoint_interval` <= 0: the whole march as one
+// taped segment (no recomputation) when its tape (the fields before every second RK4 and
+// the column spectra, plus DKint per step in THERMAL) takes at most a quarter of the free
+// device memory, else ceil(sqrt(M)) with per-segment recomputation.  A context that
+// already holds a whole-march tape keeps it.
+int auto_interval(const shg_ctx *c, int M, size_t n2, bool thermal) {
+    if (c->adj.k == M && c->adj.nseg == 1) return M;
+    const double tape = (double)M * (double)n2 * (2.0 * sizeof(float4) + (thermal ? sizeof(float) : 0.0));
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && tape <= 0.25 * (double)free_b) return M;
+    return (int)std::ceil(std::sqrt((double)M));
+}
+
+// The tape buffers for interval K (and the THERMAL ones); an automatic whole-march tape that
+// does not fit after all (other contexts allocated meanwhile) falls back to sqrt(M).
+int adjoint_buffers(shg_ctx *c, int M, bool auto_k, bool thermal, int *K, int *nseg) {
+    *K = std::max(1, std::min(*K, M));
+    *nseg = (M + *K - 1) / *K;
+    int rc = ensure_buffers(c, *K, *nseg);
+    if (!rc && thermal) rc = ensure_thermal_buffers(c, *K, *nseg);
+    if (rc == SHG_E_NOMEM && auto_k && *K == M && M > 1) {
+        (void)hipGetLastError();  // the failed hipMalloc's status must not reach the next launch check
+        *K = std::max(1, std::min((int)std::ceil(std::sqrt((double)M)), M));
+        *nseg = (M + *K - 1) / *K;
+        rc = ensure_buffers(c, *K, *nseg);
+        if (!rc && thermal) rc = ensure_thermal_buffers(c, *K, *nseg);
+    }
+    return rc;
+}
+
 int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res, bool sens) {
     const int M = c->p.nz - 1, n = c->n;
     const size_t n2 = (size_t)n * n, n3 = n2 * c->hz;
-    int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
-    K = std::max(1, std::min(K, M));
-    const int nseg = (M + K - 1) / K;
-    ARC_TRY(ensure_buffers(c, K, nseg));
-    ARC_TRY(ensure_thermal_buffers(c, K, nseg));
+    int K = checkpoint_interval > 0 ? checkpoint_interval : auto_interval(c, M, n2, true), nseg = 0;
+    ARC_TRY(adjoint_buffers(c, M, checkpoint_interval <= 0, true, &K, &nseg));
     AdjointBuffers &b = c->adj;
     b.map_valid = false;
     hipStream_t st = c->stream;
@@ -455,10 +482,8 @@ int thermal_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_
 int plain_adjoint(shg_ctx *c, int checkpoint_interval, float *d_eta_d_fh, shg_adjoint_result *res, bool sens) {
     const int M = c->steps, n = c->n;
     const size_t n2 = (size_t)n * n;
-    int K = checkpoint_interval > 0 ? checkpoint_interval : (int)std::ceil(std::sqrt((double)M));
-    K = std::max(1, std::min(K, M));
-    const int nseg = (M + K - 1) / K;
-    ARC_TRY(ensure_buffers(c, K, nseg));
+    int K = checkpoint_interval > 0 ? checkpoint_interval : auto_interval(c, M, n2, false), nseg = 0;
+    ARC_TRY(adjoint_buffers(c, M, checkpoint_interval <= 0, false, &K, &nseg));
     AdjointBuffers &b = c->adj;
     hipStream_t st = c->stream;
     const int rb = shg::adj_row_parts(n), cb = shg::adj_col_parts(n);

@@ -181,6 +181,14 @@ def test_checkpoint_interval_does_not_change_gradients(gpu):
             res, g = ctx.adjoint(checkpoint_interval=k)
             assert res["checkpoint_interval"] == k and res["segments"] == -(-24 // k)
             out.append((res, g))
+        # the default tapes the whole march when the tape fits (it does on this grid), and
+        # after it the checkpointed layout is rebuilt on request
+        res, g = ctx.adjoint()
+        assert res["checkpoint_interval"] == 24 and res["segments"] == 1
+        out.append((res, g))
+        res, g = ctx.adjoint(checkpoint_interval=5)
+        assert res["segments"] == 5
+        out.append((res, g))
     r0, g0 = out[0]
     for r, g in out[1:]:
         assert abs(r["eta"] - r0["eta"]) <= 1e-6 * r0["eta"]
