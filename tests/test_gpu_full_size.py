@@ -344,3 +344,67 @@ def test_gradient_workload_temperature_and_field_vs_oracle64_fd(gpu):
     assert abs(fdT - res["d_eta_d_temp"]) <= 2e-2 * abs(fdT)
     assert abs(fdf - res["d_eta_d_focus"]) <= 2e-2 * abs(fdf)
     assert rel.max() <= 2e-2
+
+
+@pytest.mark.gpu
+def test_gradient_workload_64_joint_directions(gpu):
+    # 64 seeded directions (seed 4) in the joint parameters (A0, focus, T) of the gradient
+    # workload, each block's FD step 1e-3 relative to its norm (|A0|, the focus, T).  The
+    # adjoint's directional derivative Re<g, dA> + d_focus df + d_T dT is compared with
+    # central FD: the (A0, focus) part on the GPU forward (float; pump rebuilt at each
+    # focus), the T part from central FD of the FP64 restatement at 26.913 C, where the
+    # float and double phase mismatch agree (the float model's eta(T) is a staircase on
+    # the scale of a 1e-3 relative T step, so T differences run on the smooth FP64 model).
+    # The focus FD carries an O(h^2) curvature term (the beam sits near its optimal focus),
+    # so every direction is also Richardson-extrapolated from steps 1e-3 and 2e-3.
+    p = _grad_params(temp=26.913)
+    pump, noise = _grad_inputs(gpu, p)
+    a0 = (pump + noise).astype(np.complex64)
+    sh0 = np.zeros_like(a0)
+    rng = np.random.default_rng(4)
+    ny, nx = a0.shape
+    y, x = np.meshgrid(np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    dirs = []
+    for _ in range(64):
+        m = np.zeros(a0.shape, np.complex128)
+        for ky in range(-2, 3):
+            for kx in range(-2, 3):
+                m += (rng.standard_normal() + 1j * rng.standard_normal()) * np.exp(2j * np.pi * (kx * x + ky * y))
+        dA = a0 * m
+        dA *= np.linalg.norm(a0) / np.linalg.norm(dA)
+        dirs.append((dA, p.focus * rng.standard_normal(), p.temp * rng.standard_normal()))
+    e = 1e-3
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(a0, sh0)
+        res, g = ctx.adjoint()
+        g = g.astype(np.complex128)
+
+        def eta_gpu(s, k):
+            dA, df, _ = dirs[k]
+            f = (gpu.make_pump(p.copy(focus=p.focus + s * df)) + noise + s * dA).astype(np.complex64)
+            ctx.set_fields(f, sh0)
+            ctx.forward()
+            return ctx.powers()[2]
+        fd1 = np.array([(eta_gpu(e, k) - eta_gpu(-e, k)) / (2 * e) for k in range(64)])
+        fd2 = np.array([(eta_gpu(2 * e, k) - eta_gpu(-2 * e, k)) / (4 * e) for k in range(64)])
+    rich = (4 * fd1 - fd2) / 3
+    noise64 = a0.astype(np.complex128) - orc.pump(p, 64)
+    hT = e * p.temp
+
+    def eta_T(t):
+        q = p.copy(temp=t)
+        return _eta_orc(q, orc.pump(q, 64) + noise64, 64, t)
+    print("\n  oracle64 FD in T: 2 forwards at 256x256, nz 400 ...", flush=True)
+    rT = _parallel([(eta_T, p.temp + hT), (eta_T, p.temp - hT)], workers=2)
+    fdT = (rT[0] - rT[1]) / (2 * hT)
+    ad_af = np.array([float(np.real(np.sum(np.conj(g) * dA))) + res["d_eta_d_focus"] * df for dA, df, _ in dirs])
+    dT = np.array([d[2] for d in dirs])
+    ad = ad_af + res["d_eta_d_temp"] * dT
+    ref1, refr = fd1 + fdT * dT, rich + fdT * dT
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print(f"  64 joint directions (A0, focus, T): relL2 adjoint vs FD step 1e-3 {rel(ad, ref1):.2e}, "
+          f"vs Richardson {rel(ad, refr):.2e}; (A0, focus) part: step 1e-3 {rel(ad_af, fd1):.2e}, "
+          f"Richardson {rel(ad_af, rich):.2e}; T part: adjoint {res['d_eta_d_temp']:+.6e} oracle64 FD {fdT:+.6e}")
+    assert rel(ad, refr) <= 5e-3
+    assert rel(ad_af, rich) <= 5e-3
+    assert rel(ad, ref1) <= 1e-2
