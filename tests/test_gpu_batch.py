@@ -1,6 +1,7 @@
 """Batched sweeps on the GPU: every instance of a batch must equal its own plain solve
 (same kernels, same inputs: bit-identical), and small instances are checked against the
-oracle directly."""
+oracle directly (the distance to the all-FP64 restatement, oracle64, is printed beside
+it, with oracle32's own distance to oracle64)."""
 import numpy as np
 import pytest
 
@@ -57,8 +58,11 @@ def test_batch_small_instances_match_oracle(gpu):
             fh0, sh0 = b.get_input(i)
             fh, sh = b.get_fields(i)
             rfh, rsh = orc.cwes(p, fh0, sh0, prec=32)
+            dfh, dsh = orc.cwes(p, fh0, sh0, prec=64)
             e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
-            print(f"  instance {i}: nz={p.nz} relL2 FH {e[0]:.2e} SH {e[1]:.2e}")
+            print(f"  instance {i}: nz={p.nz} relL2 FH {e[0]:.2e} SH {e[1]:.2e}; vs oracle64 FH "
+                  f"{orc.rel_l2(fh, dfh):.2e} SH {orc.rel_l2(sh, dsh):.2e} (oracle32 vs oracle64 SH "
+                  f"{orc.rel_l2(rsh, dsh):.2e})")
             assert max(e) <= 1e-4, (i, e)
 
 
@@ -114,6 +118,8 @@ def test_full_sweep_of_256_instances(gpu, monkeypatch):
         assert np.array_equal(fh, outs[i][0]) and np.array_equal(sh, outs[i][1]), (i, p.nx)
         if p.nx == 64:
             rfh, rsh = orc.cwes(p, ins[i][0], ins[i][1], prec=32)
+            dfh, dsh = orc.cwes(p, ins[i][0], ins[i][1], prec=64)
             e = (orc.rel_l2(fh, rfh), orc.rel_l2(sh, rsh))
-            print(f"\n  instance {i}: 64^2 nz={p.nz} relL2 FH {e[0]:.2e} SH {e[1]:.2e}", flush=True)
+            print(f"\n  instance {i}: 64^2 nz={p.nz} relL2 FH {e[0]:.2e} SH {e[1]:.2e}; vs oracle64 FH "
+                  f"{orc.rel_l2(fh, dfh):.2e} SH {orc.rel_l2(sh, dsh):.2e}", flush=True)
             assert max(e) <= 1e-4, (i, e)
