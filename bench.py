@@ -525,7 +525,7 @@ def cpu_small_workloads(shg, default_run_stats):
 def bench_inverse(shg, args):
     """Inverse-design sample: THERMAL 512x512, nz = 1000, heat 512x512x200, P = 50 W; the
     temperature field from a bounded THERMAL outer loop, then adjoint ascent of a seeded
-    +-0.1 rad pump phase mask with the field held fixed, judged by a plain forward.
+    +-0.1 rad pump phase mask with the field held fixed, checked by a plain forward.
     `inverse_instances` instances ascend at once (one context and host thread each) for
     `inverse_iterations` iterations; instance 0 is also timed alone.  Both are projected
     to 32 instances x 100 iterations.  The adjoint keeps a full tape (checkpoint interval
@@ -568,12 +568,12 @@ def bench_inverse(shg, args):
     for i, (mask, etas) in enumerate(res):
         ctxs[i].close()
         p = ctxs[i].params
-        judge = shg.Context(p.copy(thermal=0))
+        check = shg.Context(p.copy(thermal=0))
         fld = (pumps[i] * np.exp(1j * mask)).astype(np.complex64)
-        judge.set_fields(fld, np.zeros_like(fld))
-        judge.forward()
-        out.append(dict(**th[i], eta_first=etas[0], eta_last=etas[-1], eta_plain_judge=judge.powers()[2]))
-        judge.close()
+        check.set_fields(fld, np.zeros_like(fld))
+        check.forward()
+        out.append(dict(**th[i], eta_first=etas[0], eta_last=etas[-1], eta_plain_check=check.powers()[2]))
+        check.close()
     th_s = float(np.mean([o["thermal_s"] for o in out]))
     cells = 512 * 512 * 999
     return dict(instances_concurrent=n, iterations_timed=it, checkpoint_interval=ck, per_instance=out,
@@ -588,7 +588,7 @@ def bench_inverse_full(shg, args):
     """The large batched inverse design end to end: 32 instances of THERMAL 512x512,
     nz = 1000, heat grid 512x512x200, P = 50 W, each with its temperature field from a
     bounded THERMAL outer loop and its seeded +-0.1 rad pump phase mask, 100 ascent
-    iterations each, judged by a plain forward.  The instances run `inverse_instances` at
+    iterations each, checked by a plain forward.  The instances run `inverse_instances` at
     a time (one context and host thread each, full tapes), wave after wave."""
     import numpy as np
     import torch
@@ -613,12 +613,12 @@ def bench_inverse_full(shg, args):
         res = ascend_phase_masks(ctxs, pumps, masks, it, checkpoint_interval=999)
         for i, c, pump, (mask, etas) in zip(ids, ctxs, pumps, res):
             c.close()
-            judge = shg.Context(shg.inverse512_params(thermal=0))
+            check = shg.Context(shg.inverse512_params(thermal=0))
             fld = (pump * np.exp(1j * mask)).astype(np.complex64)
-            judge.set_fields(fld, np.zeros_like(fld))
-            judge.forward()
-            out.append(dict(instance=i, eta_first=etas[0], eta_last=etas[-1], eta_plain_judge=judge.powers()[2]))
-            judge.close()
+            check.set_fields(fld, np.zeros_like(fld))
+            check.forward()
+            out.append(dict(instance=i, eta_first=etas[0], eta_last=etas[-1], eta_plain_check=check.powers()[2]))
+            check.close()
         torch.cuda.synchronize()
         log(f"inverse design: instances {ids[0]}..{ids[-1]} done at {time.perf_counter() - t_all:.1f} s")
     wall = time.perf_counter() - t_all
@@ -629,7 +629,7 @@ def bench_inverse_full(shg, args):
                 eta_first_mean=float(np.mean([o["eta_first"] for o in out])),
                 eta_last_mean=float(np.mean([o["eta_last"] for o in out])),
                 eta_gain_min=float(min(gains)), eta_gain_max=float(max(gains)),
-                judge_vs_last_max_rel=float(max(abs(o["eta_plain_judge"] - o["eta_last"]) / o["eta_last"] for o in out)),
+                check_vs_last_max_rel=float(max(abs(o["eta_plain_check"] - o["eta_last"]) / o["eta_last"] for o in out)),
                 per_instance=out)
 
 

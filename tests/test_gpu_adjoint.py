@@ -2,7 +2,7 @@
 central finite differences.
 
 - 64 seeded random field directions, central FD of the GPU forward with a step of 1e-3
-  relative to |A0| (as specified for the adjoint check), judged as a 64-vector (relative
+  relative to |A0| (as specified for the adjoint check), compared as a 64-vector (relative
   L2 within 5e-3, and correlation); with a 1e-2 step the first 8 directions are checked
   one by one.  The directions are smooth modulations of the pump (random 2D Fourier modes
   |k| <= 2), along which eta moves at O(eta); white-noise directions mostly miss the beam,
