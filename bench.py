@@ -119,11 +119,12 @@ def cpu_baseline_thermal(p, T, fh0, sh0, budget_s=12.0, budget_o0_s=4.0):
             "sample_O0": f"the same at -O0: first {k0} z-steps, {dt0:.1f} s"}
 
 
-def cpu_batch_projection(ps, budget_s=1.5):
+def cpu_batch_projection(ps, budget_s=1.5, per_size=2):
     """The sweep as the reference's SHG.sh runs it: one serial solve per point.  oracle32's
-    plain march is timed on one core (pinned) for one instance of every grid size in the
-    sweep, over as many z-steps as fit in about budget_s each, and the per-size rate is
-    extrapolated to every instance's cells x steps."""
+    plain march is timed on one core (pinned) for the first `per_size` instances of every
+    grid size in the sweep, over as many z-steps as fit in about budget_s each, and the
+    per-size rate (cells x steps over the summed time) is extrapolated to every instance's
+    cells x steps."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
@@ -132,22 +133,25 @@ def cpu_batch_projection(ps, budget_s=1.5):
     rates = {}
     try:
         for n in sorted({p.nx for p in ps}):
-            q = next(p for p in ps if p.nx == n)
-            fh = orc.pump(q, 32)
-            sh = np.zeros_like(fh)
-            t0 = time.perf_counter()
-            orc.cwes(q, fh, sh, prec=32, nsteps=1)
-            one = time.perf_counter() - t0
-            k = max(1, min(q.nz - 1, int(budget_s / max(one, 1e-6))))
-            t0 = time.perf_counter()
-            orc.cwes(q, fh, sh, prec=32, nsteps=k)
-            rates[n] = n * n * k / (time.perf_counter() - t0)
+            work, secs = 0.0, 0.0
+            for q in [p for p in ps if p.nx == n][:per_size]:
+                fh = orc.pump(q, 32)
+                sh = np.zeros_like(fh)
+                t0 = time.perf_counter()
+                orc.cwes(q, fh, sh, prec=32, nsteps=1)
+                one = time.perf_counter() - t0
+                k = max(1, min(q.nz - 1, int(budget_s / max(one, 1e-6))))
+                t0 = time.perf_counter()
+                orc.cwes(q, fh, sh, prec=32, nsteps=k)
+                secs += time.perf_counter() - t0
+                work += n * n * k
+            rates[n] = work / secs
     finally:
         os.sched_setaffinity(0, affinity)
     serial_s = sum(p.nx * p.ny * (p.nz - 1) / rates[p.nx] for p in ps)
     return {"serial_s_projected": serial_s, "cell_steps_per_s_by_size": {str(n): r for n, r in rates.items()},
             "cores": 1, "kind": "port",
-            "sample": "oracle32 -O3 plain march, one instance per grid size for about "
+            "sample": f"oracle32 -O3 plain march, {per_size} instances per grid size for about "
                       f"{budget_s:g} s each, pinned to one core, extrapolated by cells x steps"}
 
 
