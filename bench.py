@@ -530,7 +530,7 @@ def bench_inverse(shg, args):
     """Inverse-design sample: THERMAL 512x512, nz = 1000, heat 512x512x200, P = 50 W; the
     temperature field from a bounded THERMAL outer loop, then adjoint ascent of a seeded
     +-0.1 rad pump phase mask with the field held fixed, checked by a plain forward.
-    `inverse_instances` instances ascend at once (one context and host thread each) for
+    `inverse_instances` instances (16 by default) ascend at once (one context and host thread each) for
     `inverse_iterations` iterations; instance 0 is also timed alone.  Both are projected
     to 32 instances x 100 iterations.  The adjoint keeps a full tape (checkpoint interval
     nz - 1: ~9.4 GB per 512x512, nz 1000 instance) instead of recomputing sqrt(nz)
@@ -651,7 +651,7 @@ def main():
     ap.add_argument("--plain-variants", default="", help="ROW,COL: the same for the plain 256x256 march")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--single-sample", type=int, default=32)
-    ap.add_argument("--inverse-instances", type=int, default=8)
+    ap.add_argument("--inverse-instances", type=int, default=16)
     ap.add_argument("--inverse-iterations", type=int, default=2)
     ap.add_argument("--section", choices=["all", "plain", "batch", "adjoint", "inverse", "inverse-full",
                                           "thermal-sweep", "jacobi"], default="all",
