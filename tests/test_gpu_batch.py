@@ -48,6 +48,26 @@ def test_batch_equals_single_instance_solves(gpu, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_batch_of_shortest_marches_equals_single_solves(gpu, monkeypatch):
+    # instances of 1-4 z-steps (nz 2..5) of every grid size: the instances still marching
+    # drop out of the launches after their last step; each equals its own solve bit for bit
+    monkeypatch.setenv("SHG_NO_TUNE", "1")
+    ps = _mixed(10, seed=15, nz_range=(2, 5))
+    print("\nbatch sizes", [p.nx for p in ps], "nz", [p.nz for p in ps], flush=True)
+    with gpu.Batch(ps) as b:
+        b.init_fields(noise_rel=1e-6, seed=6)
+        b.run()
+        outs = [b.get_fields(i) for i in range(len(ps))]
+        ins = [b.get_input(i) for i in range(len(ps))]
+    for i, p in enumerate(ps):
+        with gpu.Context(p) as ctx:
+            ctx.set_fields(*ins[i])
+            ctx.forward()
+            fh, sh = ctx.get_fields()
+        assert np.array_equal(fh, outs[i][0]) and np.array_equal(sh, outs[i][1]), (i, p.nx, p.nz)
+
+
+@pytest.mark.gpu
 def test_batch_small_instances_match_oracle(gpu):
     ps = _mixed(6, seed=12, nz_range=(10, 40))
     ps = [p.copy(nx=64, ny=64) for p in ps]

@@ -50,6 +50,10 @@ CASES = {
     "walkoff_n128_nz40": lambda: plain128_params(nz=40, power=20.0, lx=400.0, ly=400.0, rho_p=2e-3, rho_s=1e-3),
     # MgO:PPLN (the crystal model the reference's README names but does not implement)
     "ppln_n128_nz40": lambda: plain128_params(nz=40, crystal=SHG_CRYSTAL_PPLN, qpm_period=6.92, temp=40.0),
+    # the shortest marches: nz = 2 is one z-step (first row pass, column pass, last row
+    # pass), nz = 3 two steps
+    "edge_n64_nz2": lambda: plain128_params(nx=64, ny=64, nz=2, power=50.0, waist=20.0, lx=160.0, ly=160.0),
+    "edge_n512_nz3": lambda: plain128_params(nx=512, ny=512, nz=3, power=50.0, lx=500.0, ly=500.0),
 }
 
 

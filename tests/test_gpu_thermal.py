@@ -67,7 +67,10 @@ def test_jacobi_sweeps_bit_exact(gpu, nx, nz, hz, bc):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx,nz,hz,crystal", [(64, 16, 0, SHG_CRYSTAL_SPPLT), (128, 12, 0, SHG_CRYSTAL_SPPLT),
-                                              (64, 20, 5, SHG_CRYSTAL_SPPLT), (64, 16, 4, SHG_CRYSTAL_PPLN)])
+                                              (64, 20, 5, SHG_CRYSTAL_SPPLT), (64, 16, 4, SHG_CRYSTAL_PPLN),
+                                              # the shortest marches: one step (nz 2), two steps on a
+                                              # two-slice heat grid
+                                              (64, 2, 0, SHG_CRYSTAL_SPPLT), (256, 3, 2, SHG_CRYSTAL_SPPLT)])
 def test_thermal_march_matches_oracle(gpu, nx, nz, hz, crystal):
     extra = dict(crystal=crystal, qpm_period=6.92) if crystal == SHG_CRYSTAL_PPLN else {}
     p = _thermal_params(nx=nx, ny=nx, nz=nz, heat_nz=hz, power=40.0, waist=30.0, lx=240.0, ly=240.0, **extra)
