@@ -162,3 +162,32 @@ def test_forward_snapshots_match_step_by_step_march(gpu):
     assert np.array_equal(out_fh, sfh[-1]) and np.array_equal(out_sh, ssh[-1])
     rfh, rsh = orc.cwes(p, fh0, sh0, prec=32)
     assert orc.rel_l2(sfh[-1], rfh) <= TOL_FIELD and orc.rel_l2(ssh[-1], rsh) <= TOL_FIELD
+
+
+@pytest.mark.gpu
+def test_zero_fields_stay_zero_and_nonfinite_inputs_are_rejected(gpu, monkeypatch):
+    # a zero pump and SH generate nothing (every operation of the step maps 0 to 0): the
+    # output is exactly zero and the efficiency 0; a NaN or Inf anywhere in the input is
+    # refused with SHG_E_NONFINITE and the context stays usable
+    monkeypatch.setenv("SHG_NO_TUNE", "1")                   # both contexts run the same layouts
+    p = plain128_params(nx=64, ny=64, nz=12)
+    zero = np.zeros((64, 64), np.complex64)
+    fh0, sh0 = _inputs(p)
+    with gpu.Context(p) as ctx:
+        ctx.set_fields(zero, zero)
+        ctx.forward()
+        fh, sh = ctx.get_fields()
+        assert not np.any(fh) and not np.any(sh)
+        assert ctx.powers() == (0.0, 0.0, 0.0)
+        for bad in (np.nan, np.inf):
+            for which in (0, 1):
+                f = [fh0.copy(), sh0.copy()]
+                f[which][17, 40] = bad
+                with pytest.raises(gpu.ShgError) as err:
+                    ctx.set_fields(*f)
+                assert err.value.code == -5, err.value
+        ctx.set_fields(fh0, sh0)
+        ctx.forward()
+        ok_fh, ok_sh = ctx.get_fields()
+    ref_fh, ref_sh, _ = _run_gpu(gpu, p, fh0, sh0)
+    assert np.array_equal(ok_fh, ref_fh) and np.array_equal(ok_sh, ref_sh)
