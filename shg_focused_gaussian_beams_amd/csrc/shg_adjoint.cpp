@@ -187,10 +187,13 @@ int ensure_pump_fields(shg_ctx *c, double np, double T) {
     AdjointBuffers &b = c->adj;
     const int n = c->n;
     const size_t n2 = (size_t)n * n;
-    if (b.pump_df && b.pump_T == T) return SHG_OK;
-    if (!b.pump_df && (hipMalloc(&b.pump_df, n2 * sizeof(double2)) != hipSuccess ||
-                       hipMalloc(&b.pump_dn, n2 * sizeof(double2)) != hipSuccess))
-        return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+    if (b.pump_df && b.pump_dn && b.pump_T == T) return SHG_OK;
+    b.pump_T = -1e300;                         // valid again only once both fields hold T's values
+    for (double2 **ptr : {&b.pump_df, &b.pump_dn})
+        if (!*ptr && hipMalloc(ptr, n2 * sizeof(double2)) != hipSuccess) {
+            *ptr = nullptr;
+            return shg_internal_fail(SHG_E_NOMEM, "shg_adjoint: device allocation failed");
+        }
     const shg::phys::Consts k(c->p);
     const double eps0 = 8.8541878128e-6, cl = 299.792458;
     const double w0 = c->p.waist, f = c->p.focus, lam0 = (double)k.lp;
