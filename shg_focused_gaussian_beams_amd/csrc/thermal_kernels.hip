@@ -185,33 +185,23 @@ __device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx,
 // them, whose z-halo and x-y halo planes its L2 then shares.
 // Batched instances: slot j < gridDim.y runs instance ids[j] (at offset ids[j]*inst_voxels
 // of Tf, Ti, Q), its partials at j*gridDim.x; without `ids` the one slot is instance 0.
-template <int TX, int TY, bool RESID>
-__global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const float *__restrict__ Ti,
-                                                 const float *__restrict__ Q, int nx, int ny, int hz, int zc,
-                                                 int tiles_x, int nchunks, JacobiConst c, double *__restrict__ part,
-                                                 const int *__restrict__ ids, size_t inst_voxels) {
+// The march of one workgroup over its tile and z-chunk (k_jacobi2).  INNER: every cell of
+// the tile and of its one-cell halo is an updated inner cell in x and y (the tile's
+// two-cell halo lies inside the grid and off the ovens), so the sweeps need no per-cell
+// class: the inner formula, or on the grid's first and last planes the z-face formula,
+// for every cell.  The general form dispatches per cell class.  Both run the same
+// formulas in the same cell order, so a tile's result does not depend on which one ran.
+template <int TX, int TY, bool RESID, bool INNER>
+__device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                              const float *__restrict__ Q, int nx, int ny, int hz,
+                                              const JacobiConst &c, double *__restrict__ part, int lb, int x0,
+                                              int y0, int za, int zb, size_t sz, float (*Li)[(TX + 4) * (TY + 4)],
+                                              float (*L1)[(TX + 2) * (TY + 2)]) {
     constexpr int W2 = TX + 4, H2 = TY + 4, N2 = W2 * H2;
     constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
     constexpr int N0 = TX * TY;
     constexpr int R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256, R0 = (N0 + 255) / 256;
-    __shared__ float Li[3][N2];
-    __shared__ float L1[3][N1];
     const int tid = threadIdx.x;
-    const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
-    const int per = nb >> 3, rem = nb & 7, xcd = b & 7;
-    const int gb = xcd * per + min(xcd, rem) + (b >> 3);
-    const int slot = gb / (int)gridDim.x, lb = gb - slot * (int)gridDim.x;
-    const size_t inst = ids ? (size_t)ids[slot] : 0;
-    Tf += inst * inst_voxels;
-    Ti += inst * inst_voxels;
-    Q += inst * inst_voxels;
-    if (RESID) part += (size_t)slot * gridDim.x;
-    const int tile = lb / nchunks, chunk = lb - tile * nchunks;
-    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int x0 = tx * TX, y0 = ty * TY;
-    const int za = chunk * zc, zb = min(hz, za + zc);
-    const size_t sz = (size_t)nx * ny;
-
     // Per thread, everything about its cells that does not change along the march: the
     // plane offsets of its R2 cells (Ti staging, row by row), or -1 outside the domain;
     // for its R1 cells (first sweep) their place in the R2 and R1 ring planes, plane
@@ -248,7 +238,7 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
         c2r[r] = (row + 1) * W2 + col + 1;                 // the cell in R2
         q1[r] = row * W1 + col;                            // and in R1
         o1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
-        k1[r] = t < N1 ? kind(gx, gy) : 0;
+        k1[r] = INNER ? 0 : (t < N1 ? kind(gx, gy) : 0);
     }
 #pragma unroll
     for (int r = 0; r < R0; r++) {
@@ -258,17 +248,17 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
         const int gx = x0 + col, gy = y0 + row;
         c1r[r] = (row + 1) * W1 + col + 1;                 // the cell in R1
         g0[r] = gy * nx + gx;
-        k0[r] = t < N0 ? kind(gx, gy) : 0;
+        k0[r] = INNER ? (t < N0 ? 1 : 0) : (t < N0 ? kind(gx, gy) : 0);
     }
     // per slot, whether every lane of the wavefront holds an inner cell (wave-uniform: the
     // sweeps then run the inner formula with no per-lane class branches)
     bool f1[R1], f0[R0];
 #pragma unroll
     for (int r = 0; r < R1; r++)
-        f1[r] = __builtin_amdgcn_ballot_w64(tid + 256 * r < N1 && k1[r] == kInnerKind) == ~0ull;
+        f1[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N1 && k1[r] == kInnerKind) == ~0ull;
 #pragma unroll
     for (int r = 0; r < R0; r++)
-        f0[r] = __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
+        f0[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
 
     // Loads of step k (issued during step k-1): Ti plane k over R2, Q plane k-1 at the R1
     // cells and Q plane k-2 at the tile cells.  Out-of-domain cells load from the plane
@@ -331,7 +321,9 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
                     return pl[c2 + dy * W2 + dx];
                 };
                 auto lqv = [&] { return q1c[r]; };
-                if (zc1 == 0 && f1[r]) {                                 // a wavefront of inner cells
+                if (INNER) {                                             // inner cells only
+                    if (t < N1) dst[p] = jacobi_voxel(c, 0, true, zc1, ld, lqv);
+                } else if (zc1 == 0 && f1[r]) {                          // a wavefront of inner cells
                     dst[p] = jacobi_voxel(c, 0, true, 0, ld, lqv);
                 } else if (t < N1) {
                     float v = 0.f;
@@ -358,10 +350,11 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
                     return pl[c1 + dy * W1 + dx];
                 };
                 auto lqv = [&] { return q0c[r]; };
-                const bool fast = zc2 == 0 && f0[r];                     // a wavefront of inner cells
-                if (fast || (kd & 3) == 2) {
-                    const float out = fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
-                                           : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
+                const bool fast = !INNER && zc2 == 0 && f0[r];           // a wavefront of inner cells
+                if (INNER ? tid + 256 * r < N0 : (fast || (kd & 3) == 2)) {
+                    const float out = INNER  ? jacobi_voxel(c, 0, true, zc2, ld, lqv)
+                                      : fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
+                                             : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
                     Tf[(size_t)p2 * sz + g0[r]] = out;
                     if (RESID) {
                         const double d = (double)(out - l0[c1]);
@@ -378,6 +371,38 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
         __syncthreads();
         if (tid == 0) part[lb] = w[0] + w[1] + w[2] + w[3];
     }
+}
+
+template <int TX, int TY, bool RESID>
+__global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                 const float *__restrict__ Q, int nx, int ny, int hz, int zc,
+                                                 int tiles_x, int nchunks, JacobiConst c, double *__restrict__ part,
+                                                 const int *__restrict__ ids, size_t inst_voxels) {
+    constexpr int W2 = TX + 4, H2 = TY + 4, N2 = W2 * H2;
+    constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
+    constexpr int N0 = TX * TY;
+    constexpr int R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256, R0 = (N0 + 255) / 256;
+    __shared__ float Li[3][N2];
+    __shared__ float L1[3][N1];
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int per = nb >> 3, rem = nb & 7, xcd = b & 7;
+    const int gb = xcd * per + min(xcd, rem) + (b >> 3);
+    const int slot = gb / (int)gridDim.x, lb = gb - slot * (int)gridDim.x;
+    const size_t inst = ids ? (size_t)ids[slot] : 0;
+    Tf += inst * inst_voxels;
+    Ti += inst * inst_voxels;
+    Q += inst * inst_voxels;
+    if (RESID) part += (size_t)slot * gridDim.x;
+    const int tile = lb / nchunks, chunk = lb - tile * nchunks;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int za = chunk * zc, zb = min(hz, za + zc);
+    const size_t sz = (size_t)nx * ny;
+
+    if (x0 >= 2 && x0 + TX <= nx - 2 && y0 >= 2 && y0 + TY <= ny - 2)
+        jacobi2_march<TX, TY, RESID, true>(Tf, Ti, Q, nx, ny, hz, c, part, lb, x0, y0, za, zb, sz, Li, L1);
+    else
+        jacobi2_march<TX, TY, RESID, false>(Tf, Ti, Q, nx, ny, hz, c, part, lb, x0, y0, za, zb, sz, Li, L1);
 }
 
 __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ part, int n, double *out) {
