@@ -167,14 +167,15 @@ __device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx,
 
 // Two Jacobi sweeps per pass, Tf <- J(J(Ti, Q), Q), marching in z (temporal blocking).
 // Workgroup = one TX x TY tile of the x-y plane and one chunk of zc planes.  It keeps in
-// LDS a ring of three planes of Ti over the tile plus a halo of 2 (region R2) and a ring
-// of three planes of the first sweep's values T1 over the tile plus a halo of 1 (R1).
-// Step k of the march stages Ti plane k, computes T1 plane k-1 on R1, and then the
-// second sweep's plane k-2 on the tile, which it writes.  Ti plane k+1 and the Q values
-// of the next step's cells (plane k over R1, plane k-1 over the tile, each thread its own
-// cells) are loaded into registers one step ahead, so that their latency overlaps this
-// step's arithmetic; Q never goes through LDS, which keeps the footprint to the two
-// rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
+// LDS a ring of four planes of Ti over the tile plus a halo of 2 (region R2) and a ring
+// of four planes of the first sweep's values T1 over the tile plus a halo of 1 (R1).
+// Step k of the march stages Ti plane k, computes T1 plane k-2 on R1 (from Ti planes
+// staged by earlier steps), and the second sweep's plane k-4 on the tile (from T1 planes
+// of earlier steps), which it writes: no ring slot is written and read in the same step,
+// so one barrier per step suffices.  Ti plane k+1 and the Q values of the next step's
+// cells (plane k-1 over R1, plane k-3 over the tile, each thread its own cells) are
+// loaded into registers one step ahead, so that their latency overlaps this step's
+// arithmetic; Q never goes through LDS, which keeps the footprint to the two rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
 // chunk) and Q over R1, and writes Tf once, where two single sweeps read and write
 // everything twice.  The halo values of T1 are recomputed by every tile that needs them,
 // with the same formulas, so the result is two single sweeps bit for bit.  With RESID the
@@ -260,8 +261,8 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
     for (int r = 0; r < R0; r++)
         f0[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
 
-    // Loads of step k (issued during step k-1): Ti plane k over R2, Q plane k-1 at the R1
-    // cells and Q plane k-2 at the tile cells.  Out-of-domain cells load from the plane
+    // Loads of step k (issued during step k-1): Ti plane k over R2, Q plane k-2 at the R1
+    // cells and Q plane k-4 at the tile cells.  Out-of-domain cells load from the plane
     // origin and keep 0; planes outside [0, hz) are not loaded.
     float pre[R2], qn1[R1], qn0[R0];
     auto fetch = [&](int k) {
@@ -273,26 +274,30 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
                 pre[r] = off2[r] >= 0 ? v : 0.f;
             }
         }
-        if (k >= 1 && k <= hz) {
-            const float *pl = Q + (size_t)(k - 1) * sz;
+        if (k >= 2 && k <= hz + 1) {
+            const float *pl = Q + (size_t)(k - 2) * sz;
 #pragma unroll
             for (int r = 0; r < R1; r++) {
                 const float v = pl[max(o1[r], 0)];
                 qn1[r] = o1[r] >= 0 ? v : 0.f;
             }
         }
-        if (k >= 2 && k <= hz + 1) {
-            const float *pl = Q + (size_t)(k - 2) * sz;
+        if (k >= 4 && k <= hz + 3) {
+            const float *pl = Q + (size_t)(k - 4) * sz;
 #pragma unroll
             for (int r = 0; r < R0; r++) qn0[r] = pl[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
         }
     };
+    // Step k stages Ti plane k, runs the first sweep on plane k-2 (its Ti planes k-3..k-1
+    // were staged by earlier steps) and the second on plane k-4 (its first-sweep planes
+    // k-5..k-3 come from earlier steps): within a step no ring slot is both written and
+    // read (rings of four planes), so one barrier per step, at its end, orders everything.
     double res = 0.0;
     fetch(za - 2);
-    for (int k = za - 2; k <= zb + 1; k++) {
+    for (int k = za - 2; k <= zb + 3; k++) {
         // (a) stage Ti plane k, take this step's Q values, then load the next step's
         if (k >= 0 && k < hz) {
-            float *dst = Li[(k + 3) % 3];
+            float *dst = Li[(k + 4) & 3];
 #pragma unroll
             for (int r = 0; r < R2; r++) {
                 const int t = tid + 256 * r;
@@ -304,13 +309,12 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
         for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
 #pragma unroll
         for (int r = 0; r < R0; r++) q0c[r] = qn0[r];
-        if (k + 1 <= zb + 1) fetch(k + 1);
-        __syncthreads();
-        // (b) first sweep, plane p1 = k - 1, over R1
-        const int p1 = k - 1;
+        if (k + 1 <= zb + 3) fetch(k + 1);
+        // (b) first sweep, plane p1 = k - 2, over R1
+        const int p1 = k - 2;
         if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
-            const float *lm = Li[(p1 + 2) % 3], *l0 = Li[(p1 + 3) % 3], *lp = Li[(p1 + 4) % 3];
-            float *dst = L1[(p1 + 3) % 3];
+            const float *lm = Li[(p1 + 3) & 3], *l0 = Li[(p1 + 4) & 3], *lp = Li[(p1 + 5) & 3];
+            float *dst = L1[(p1 + 4) & 3];
             const int zc1 = jacobi_cls(p1, hz);
 #pragma unroll
             for (int r = 0; r < R1; r++) {
@@ -336,11 +340,10 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
                 }
             }
         }
-        __syncthreads();
-        // (c) second sweep, plane p2 = k - 2, over the tile
-        const int p2 = k - 2;
+        // (c) second sweep, plane p2 = k - 4, over the tile
+        const int p2 = k - 4;
         if (p2 >= za && p2 < zb) {
-            const float *lm = L1[(p2 + 2) % 3], *l0 = L1[(p2 + 3) % 3], *lp = L1[(p2 + 4) % 3];
+            const float *lm = L1[(p2 + 3) & 3], *l0 = L1[(p2 + 4) & 3], *lp = L1[(p2 + 5) & 3];
             const int zc2 = jacobi_cls(p2, hz);
 #pragma unroll
             for (int r = 0; r < R0; r++) {
@@ -363,6 +366,7 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
                 }
             }
         }
+        __syncthreads();
     }
     if (RESID) {
         for (int off = 32; off > 0; off >>= 1) res += __shfl_down(res, off, 64);
@@ -382,8 +386,8 @@ __global__ __launch_bounds__(256) void k_jacobi2(float *__restrict__ Tf, const f
     constexpr int W1 = TX + 2, H1 = TY + 2, N1 = W1 * H1;
     constexpr int N0 = TX * TY;
     constexpr int R2 = (N2 + 255) / 256, R1 = (N1 + 255) / 256, R0 = (N0 + 255) / 256;
-    __shared__ float Li[3][N2];
-    __shared__ float L1[3][N1];
+    __shared__ float Li[4][N2];
+    __shared__ float L1[4][N1];
     const int nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
     const int per = nb >> 3, rem = nb & 7, xcd = b & 7;
     const int gb = xcd * per + min(xcd, rem) + (b >> 3);

@@ -1,9 +1,10 @@
 """CPU check of the two-sweep Jacobi pass's schedule (k_jacobi2 in
 shg_focused_gaussian_beams_amd/csrc/thermal_kernels.hip), restated step for step: the
-XCD-dealt block order, the tile / z-chunk decomposition, the LDS rings of three planes of
+XCD-dealt block order, the tile / z-chunk decomposition, the LDS rings of four planes of
 Ti (halo 2) and of the first sweep (halo 1), the staging at march step k, the Q values
-loaded into registers one step ahead, the first sweep of plane k-1 and the second of
-plane k-2.  A toy stencil that reads every in-domain
+loaded into registers one step ahead, the first sweep of plane k-2 and the second of
+plane k-4, with one barrier per step: every ring entry a step reads was written by an
+earlier step (never by the same one).  A toy stencil that reads every in-domain
 neighbour stands in for the Jacobi formulas (which the kernel shares with the
 single-sweep kernel); the schedule must give two plain sweeps exactly, read only staged
 ring entries, and stay inside its rings — on grids that are not multiples of the tile,
@@ -61,8 +62,9 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
         ty, tx = divmod(tile, tiles_x)
         x0, y0 = tx*TX, ty*TY
         za, zb = chunk*zc, min(hz, chunk*zc + zc)
-        Li = np.full((3, N2), np.nan); L1 = np.full((3, N1), np.nan)
-        # registers: Ti plane k over R2, Q plane k-1 at the R1 cells, Q plane k-2 at the
+        Li = np.full((4, N2), np.nan); L1 = np.full((4, N1), np.nan)
+        Li_at = [(None, None)] * 4; L1_at = [(None, None)] * 4   # (plane, step) of each ring slot
+        # registers: Ti plane k over R2, Q plane k-2 at the R1 cells, Q plane k-4 at the
         # tile cells, each loaded one step ahead (NaN: not loaded)
         nxt = {}
 
@@ -72,23 +74,23 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
                 for t in range(N2):
                     row, col = divmod(t, W2); gx, gy = x0-2+col, y0-2+row
                     pre[t] = Ti[k, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
-            if 1 <= k <= hz:
+            if 2 <= k <= hz + 1:
                 for t in range(N1):
                     row, col = divmod(t, W1); gx, gy = x0-1+col, y0-1+row
-                    qn1[t] = Q[k-1, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
-            if 2 <= k <= hz + 1:
+                    qn1[t] = Q[k-2, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0
+            if 4 <= k <= hz + 3:
                 for t in range(TX*TY):
                     row, col = divmod(t, TX); gx, gy = x0+col, y0+row
-                    qn0[t] = Q[k-2, gy, gx] if (gx < nx and gy < ny) else np.nan
+                    qn0[t] = Q[k-4, gy, gx] if (gx < nx and gy < ny) else np.nan
             nxt.update(pre=pre, qn1=qn1, qn0=qn0)
         fetch(za-2)
-        for k in range(za-2, zb+2):
+        for k in range(za-2, zb+4):
             if 0 <= k < hz:
-                Li[(k+3) % 3] = nxt["pre"]
+                Li[(k+4) % 4] = nxt["pre"]; Li_at[(k+4) % 4] = (k, k)
             q1c, q0c = nxt["qn1"], nxt["qn0"]
-            if k+1 <= zb+1:
+            if k+1 <= zb+3:
                 fetch(k+1)
-            p1 = k-1
+            p1 = k-2
             if max(za-1, 0) <= p1 <= min(zb, hz-1):
                 for t in range(N1):
                     row, col = divmod(t, W1); gx, gy = x0-1+col, y0-1+row
@@ -98,13 +100,18 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
                         if upd(gx, gy, nx, ny):
                             def ld(dx, dy, dz):
                                 i = c2 + dy*W2 + dx; assert 0 <= i < N2
-                                val = Li[(p1+dz+3) % 3, i]; assert not np.isnan(val), (b, k, gx, gy, dx, dy, dz); return val
+                                slot = (p1+dz+4) % 4
+                                assert Li_at[slot][0] == p1+dz and Li_at[slot][1] < k, (b, k, slot, Li_at[slot])
+                                val = Li[slot, i]; assert not np.isnan(val), (b, k, gx, gy, dx, dy, dz); return val
                             q = q1c[t]; assert not np.isnan(q)
                             v = voxel(ld, gx, gy, p1, nx, ny, hz) + q
                         else:
-                            v = Li[(p1+3) % 3, c2]
-                    L1[(p1+3) % 3, t] = v
-            p2 = k-2
+                            slot = (p1+4) % 4
+                            assert Li_at[slot][0] == p1 and Li_at[slot][1] < k
+                            v = Li[slot, c2]
+                    L1[(p1+4) % 4, t] = v
+                L1_at[(p1+4) % 4] = (p1, k)
+            p2 = k-4
             if za <= p2 < zb:
                 for t in range(TX*TY):
                     row, col = divmod(t, TX); gx, gy = x0+col, y0+row
@@ -112,7 +119,9 @@ def emulate(Ti, Q, TX, TY, zc, upd, ids=None):
                     if gx < nx and gy < ny and upd(gx, gy, nx, ny):
                         def ld(dx, dy, dz):
                             i = c1 + dy*W1 + dx; assert 0 <= i < N1
-                            val = L1[(p2+dz+3) % 3, i]; assert not np.isnan(val); return val
+                            slot = (p2+dz+4) % 4
+                            assert L1_at[slot][0] == p2+dz and L1_at[slot][1] < k, (b, k, slot, L1_at[slot])
+                            val = L1[slot, i]; assert not np.isnan(val); return val
                         q = q0c[t]; assert not np.isnan(q)
                         Tf[p2, gy, gx] = voxel(ld, gx, gy, p2, nx, ny, hz) + q
     assert len(seen) == nb
