@@ -172,10 +172,11 @@ __device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx,
 // Step k of the march stages Ti plane k, computes T1 plane k-2 on R1 (from Ti planes
 // staged by earlier steps), and the second sweep's plane k-4 on the tile (from T1 planes
 // of earlier steps), which it writes: no ring slot is written and read in the same step,
-// so one barrier per step suffices.  Ti plane k+1 and the Q values of the next step's
-// cells (plane k-1 over R1, plane k-3 over the tile, each thread its own cells) are
-// loaded into registers one step ahead, so that their latency overlaps this step's
-// arithmetic; Q never goes through LDS, which keeps the footprint to the two rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
+// so one barrier per step suffices.  Ti plane k+1 and Q plane k-1 (over R1 and over the
+// tile, each thread its own cells) are loaded into registers one step ahead, so that their
+// latency overlaps this step's arithmetic; the tile's values wait two more steps in
+// registers for the second sweep.  Q never goes through LDS, which keeps the footprint to
+// the two rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
 // chunk) and Q over R1, and writes Tf once, where two single sweeps read and write
 // everything twice.  The halo values of T1 are recomputed by every tile that needs them,
 // with the same formulas, so the result is two single sweeps bit for bit.  With RESID the
@@ -261,10 +262,12 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
     for (int r = 0; r < R0; r++)
         f0[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
 
-    // Loads of step k (issued during step k-1): Ti plane k over R2, Q plane k-2 at the R1
-    // cells and Q plane k-4 at the tile cells.  Out-of-domain cells load from the plane
-    // origin and keep 0; planes outside [0, hz) are not loaded.
-    float pre[R2], qn1[R1], qn0[R0];
+    // Loads of step k (issued during step k-1): Ti plane k over R2, and Q plane k-2 at the
+    // R1 cells and at the tile cells: the first sweep uses the R1 values at step k, the
+    // second the tile values at step k+2 (held in registers meanwhile, so that both reads
+    // of a Q plane come together and the second hits the cache).  Out-of-domain cells load
+    // from the plane origin and keep 0; planes outside [0, hz) are not loaded.
+    float pre[R2], qn1[R1], qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
     auto fetch = [&](int k) {
         if (k >= 0 && k < hz) {
             const float *pl = Ti + (size_t)k * sz;
@@ -282,8 +285,8 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
                 qn1[r] = o1[r] >= 0 ? v : 0.f;
             }
         }
-        if (k >= 4 && k <= hz + 3) {
-            const float *pl = Q + (size_t)(k - 4) * sz;
+        if (k >= 2 && k <= hz + 1) {
+            const float *pl = Q + (size_t)(k - 2) * sz;
 #pragma unroll
             for (int r = 0; r < R0; r++) qn0[r] = pl[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
         }
@@ -308,7 +311,11 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
 #pragma unroll
         for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
 #pragma unroll
-        for (int r = 0; r < R0; r++) q0c[r] = qn0[r];
+        for (int r = 0; r < R0; r++) {                 // tile Q: plane k-4 now, k-3 and k-2 held
+            q0c[r] = qo0[r];
+            qo0[r] = qm0[r];
+            qm0[r] = qn0[r];
+        }
         if (k + 1 <= zb + 3) fetch(k + 1);
         // (b) first sweep, plane p1 = k - 2, over R1
         const int p1 = k - 2;
