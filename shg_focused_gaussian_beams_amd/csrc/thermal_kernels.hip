@@ -165,6 +165,29 @@ __device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx,
     col = bx + i - (row - ay) * wr;
 }
 
+// Cell i of the (TX+2) x (TY+2) region around a TX x TY tile, tile first: i < TX TY is
+// tile cell i (row by row, at region row 1 + i / TX, column 1 + i % TX), then the
+// region's top row, its bottom row, its left column and its right column (without their
+// corners).  A bijection of [0, (TX+2)(TY+2)) onto the region whose first TX TY entries
+// are the tile in its own row-major order.
+__device__ __forceinline__ void ring_cell_tile_first(int i, int TX, int TY, int &row, int &col) {
+    const int W = TX + 2;
+    if (i < TX * TY) {
+        row = 1 + i / TX;
+        col = 1 + i - (i / TX) * TX;
+        return;
+    }
+    i -= TX * TY;
+    if (i < 2 * W) {
+        row = i < W ? 0 : TY + 1;
+        col = i < W ? i : i - W;
+        return;
+    }
+    i -= 2 * W;
+    row = 1 + (i < TY ? i : i - TY);
+    col = i < TY ? 0 : W - 1;
+}
+
 // Two Jacobi sweeps per pass, Tf <- J(J(Ti, Q), Q), marching in z (temporal blocking).
 // Workgroup = one TX x TY tile of the x-y plane and one chunk of zc planes.  It keeps in
 // LDS a ring of four planes of Ti over the tile plus a halo of 2 (region R2) and a ring
@@ -191,8 +214,11 @@ __device__ __forceinline__ void region_cell(int i, int W, int H, int ax, int bx,
 // the tile and of its one-cell halo is an updated inner cell in x and y (the tile's
 // two-cell halo lies inside the grid and off the ovens), so the sweeps need no per-cell
 // class: the inner formula, or on the grid's first and last planes the z-face formula,
-// for every cell.  The general form dispatches per cell class.  Both run the same
-// formulas in the same cell order, so a tile's result does not depend on which one ran.
+// for every cell.  The first sweep then visits R1 tile first (ring_cell_tile_first), so
+// that a thread's first R0 slots of R1 are its tile cells of the second sweep and their
+// Q values serve both sweeps (one Q load per R1 cell and plane).  The general form
+// dispatches per cell class.  Each cell gets the same formula either way, so a tile's
+// result does not depend on which form ran.
 template <int TX, int TY, bool RESID, bool INNER>
 __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const float *__restrict__ Ti,
                                               const float *__restrict__ Q, int nx, int ny, int hz,
@@ -235,7 +261,12 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
     for (int r = 0; r < R1; r++) {
         const int t = tid + 256 * r;
         int row = 0, col = 0;
-        if (t < N1) region_cell(t, W1, H1, a1x, b1x, a1y, b1y, row, col);
+        if (t < N1) {
+            if (INNER)
+                ring_cell_tile_first(t, TX, TY, row, col);
+            else
+                region_cell(t, W1, H1, a1x, b1x, a1y, b1y, row, col);
+        }
         const int gx = x0 - 1 + col, gy = y0 - 1 + row;
         c2r[r] = (row + 1) * W2 + col + 1;                 // the cell in R2
         q1[r] = row * W1 + col;                            // and in R1
@@ -263,32 +294,25 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
         f0[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
 
     // Loads of step k (issued during step k-1): Ti plane k over R2, and Q plane k-2 at the
-    // R1 cells and at the tile cells: the first sweep uses the R1 values at step k, the
-    // second the tile values at step k+2 (held in registers meanwhile, so that both reads
-    // of a Q plane come together and the second hits the cache).  Out-of-domain cells load
-    // from the plane origin and keep 0; planes outside [0, hz) are not loaded.
-    float pre[R2], qn1[R1], qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
+    // R1 cells and (general form) at the tile cells: the first sweep uses the R1 values at
+    // step k, the second the tile values at step k+2 (held in registers meanwhile; the two
+    // reads of a Q plane come together, so the second hits the cache, and an INNER tile
+    // takes its tile values from its R1 values).  Every step loads (a plane index outside
+    // [0, hz) is clamped to the grid, its values unused) and out-of-domain cells load from
+    // the plane origin, set to 0 where the values are used: a select, or a kept old value,
+    // at the load would make the wave wait for the load right there, leaving no step of
+    // latency hidden.
+    float pre[R2], qn1[R1] = {}, qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
     auto fetch = [&](int k) {
-        if (k >= 0 && k < hz) {
-            const float *pl = Ti + (size_t)k * sz;
+        const float *pt = Ti + (size_t)min(max(k, 0), hz - 1) * sz;
 #pragma unroll
-            for (int r = 0; r < R2; r++) {
-                const float v = pl[max(off2[r], 0)];
-                pre[r] = off2[r] >= 0 ? v : 0.f;
-            }
-        }
-        if (k >= 2 && k <= hz + 1) {
-            const float *pl = Q + (size_t)(k - 2) * sz;
+        for (int r = 0; r < R2; r++) pre[r] = pt[max(off2[r], 0)];
+        const float *pq = Q + (size_t)min(max(k - 2, 0), hz - 1) * sz;
 #pragma unroll
-            for (int r = 0; r < R1; r++) {
-                const float v = pl[max(o1[r], 0)];
-                qn1[r] = o1[r] >= 0 ? v : 0.f;
-            }
-        }
-        if (k >= 2 && k <= hz + 1) {
-            const float *pl = Q + (size_t)(k - 2) * sz;
+        for (int r = 0; r < R1; r++) qn1[r] = pq[max(o1[r], 0)];
+        if (!INNER) {                                   // INNER: the tile's values are qn1[0 .. R0)
 #pragma unroll
-            for (int r = 0; r < R0; r++) qn0[r] = pl[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
+            for (int r = 0; r < R0; r++) qn0[r] = pq[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
         }
     };
     // Step k stages Ti plane k, runs the first sweep on plane k-2 (its Ti planes k-3..k-1
@@ -304,17 +328,17 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
 #pragma unroll
             for (int r = 0; r < R2; r++) {
                 const int t = tid + 256 * r;
-                if (t < N2) dst[t] = pre[r];
+                if (t < N2) dst[t] = (INNER || off2[r] >= 0) ? pre[r] : 0.f;
             }
         }
         float q1c[R1], q0c[R0];
 #pragma unroll
-        for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
+        for (int r = 0; r < R1; r++) q1c[r] = (INNER || o1[r] >= 0) ? qn1[r] : 0.f;
 #pragma unroll
         for (int r = 0; r < R0; r++) {                 // tile Q: plane k-4 now, k-3 and k-2 held
             q0c[r] = qo0[r];
             qo0[r] = qm0[r];
-            qm0[r] = qn0[r];
+            qm0[r] = INNER ? q1c[r] : qn0[r];
         }
         if (k + 1 <= zb + 3) fetch(k + 1);
         // (b) first sweep, plane p1 = k - 2, over R1

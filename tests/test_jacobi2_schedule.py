@@ -207,3 +207,32 @@ def test_inner_first_cell_order(nx, ny, TX, TY):
                 inner = [1 <= ox + c <= nx - 2 and 1 <= oy + r <= ny - 2 for r, c in cells]
                 ni = sum(inner)
                 assert all(inner[:ni]) and not any(inner[ni:]), (x0, y0, W)
+
+
+def ring_cell_tile_first(i, TX, TY):
+    """ring_cell_tile_first of thermal_kernels.hip, restated."""
+    W = TX + 2
+    if i < TX * TY:
+        return 1 + i // TX, 1 + i % TX
+    i -= TX * TY
+    if i < 2 * W:
+        return (0, i) if i < W else (TY + 1, i - W)
+    i -= 2 * W
+    return (1 + i, 0) if i < TY else (1 + i - TY, W - 1)
+
+
+@pytest.mark.parametrize("TX,TY", [(64, 16), (64, 8), (128, 8)])
+def test_interior_tile_first_order(TX, TY):
+    # interior tiles visit R1 tile first: a bijection onto R1 whose first TX*TY entries are
+    # the tile in the second sweep's (row-major) order, so slot r < R0 of a thread's R1
+    # cells is its slot r of the tile and one Q value serves both sweeps
+    W1, H1 = TX + 2, TY + 2
+    cells = [ring_cell_tile_first(i, TX, TY) for i in range(W1 * H1)]
+    assert sorted(cells) == [(r, c) for r in range(H1) for c in range(W1)]
+    n0 = TX * TY
+    assert n0 % 256 == 0                                   # the tile fills whole thread slots
+    for tid in range(256):
+        for r in range(n0 // 256):
+            t = tid + 256 * r
+            tile = region_cell(t, TX, TY, 0, TX, 0, TY)    # an interior tile's order
+            assert cells[t] == (tile[0] + 1, tile[1] + 1)
