@@ -188,6 +188,21 @@ __device__ __forceinline__ void ring_cell_tile_first(int i, int TX, int TY, int 
     col = i < TY ? 0 : W - 1;
 }
 
+// A raw buffer descriptor over `bytes` bytes at `base` (wave-uniform).  Its loads and
+// stores take a per-lane byte offset, with no per-lane address arithmetic, and the
+// hardware range check makes a load at an offset past `bytes` return 0 and a store there
+// do nothing (the descriptor word 3 of CDNA: 32-bit data format, raw addressing).
+constexpr int kPastBuffer = 0x7ffffff0;               // a byte offset beyond any plane
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float *base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
 // Two Jacobi sweeps per pass, Tf <- J(J(Ti, Q), Q), marching in z (temporal blocking).
 // Workgroup = one TX x TY tile of the x-y plane and one chunk of zc planes.  It keeps in
 // LDS a ring of four planes of Ti over the tile plus a halo of 2 (region R2) and a ring
@@ -245,7 +260,7 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
         const int t = tid + 256 * r;
         const int row = t / W2, col = t - row * W2;
         const int gx = x0 - 2 + col, gy = y0 - 2 + row;
-        off2[r] = (t < N2 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+        off2[r] = (t < N2 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? 4 * (gy * nx + gx) : kPastBuffer;
     }
     auto kind = [&](int gx, int gy) {
         if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) return 0;
@@ -270,7 +285,7 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
         const int gx = x0 - 1 + col, gy = y0 - 1 + row;
         c2r[r] = (row + 1) * W2 + col + 1;                 // the cell in R2
         q1[r] = row * W1 + col;                            // and in R1
-        o1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+        o1[r] = (t < N1 && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? 4 * (gy * nx + gx) : kPastBuffer;
         k1[r] = INNER ? 0 : (t < N1 ? kind(gx, gy) : 0);
     }
 #pragma unroll
@@ -280,8 +295,9 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
         if (t < N0) region_cell(t, TX, TY, a0x, b0x, a0y, b0y, row, col);
         const int gx = x0 + col, gy = y0 + row;
         c1r[r] = (row + 1) * W1 + col + 1;                 // the cell in R1
-        g0[r] = gy * nx + gx;
+        g0[r] = 4 * (gy * nx + gx);
         k0[r] = INNER ? (t < N0 ? 1 : 0) : (t < N0 ? kind(gx, gy) : 0);
+        if (!k0[r]) g0[r] = kPastBuffer;                 // outside the grid: never stored, Q reads 0
     }
     // per slot, whether every lane of the wavefront holds an inner cell (wave-uniform: the
     // sweeps then run the inner formula with no per-lane class branches)
@@ -298,21 +314,22 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
     // step k, the second the tile values at step k+2 (held in registers meanwhile; the two
     // reads of a Q plane come together, so the second hits the cache, and an INNER tile
     // takes its tile values from its R1 values).  Every step loads (a plane index outside
-    // [0, hz) is clamped to the grid, its values unused) and out-of-domain cells load from
-    // the plane origin, set to 0 where the values are used: a select, or a kept old value,
-    // at the load would make the wave wait for the load right there, leaving no step of
-    // latency hidden.
+    // [0, hz) is clamped to the grid, its values unused): a kept old value, or a select at
+    // the load, would make the wave wait for the load right there, leaving no step of
+    // latency hidden.  Out-of-domain cells load at an offset past the plane, which the
+    // buffer range check turns into 0.
     float pre[R2], qn1[R1] = {}, qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
+    const int pbytes = (int)(4 * sz);
     auto fetch = [&](int k) {
-        const float *pt = Ti + (size_t)min(max(k, 0), hz - 1) * sz;
+        const auto rt = plane_rsrc(Ti + (size_t)min(max(k, 0), hz - 1) * sz, pbytes);
 #pragma unroll
-        for (int r = 0; r < R2; r++) pre[r] = pt[max(off2[r], 0)];
-        const float *pq = Q + (size_t)min(max(k - 2, 0), hz - 1) * sz;
+        for (int r = 0; r < R2; r++) pre[r] = buf_load(rt, off2[r]);
+        const auto rq = plane_rsrc(Q + (size_t)min(max(k - 2, 0), hz - 1) * sz, pbytes);
 #pragma unroll
-        for (int r = 0; r < R1; r++) qn1[r] = pq[max(o1[r], 0)];
+        for (int r = 0; r < R1; r++) qn1[r] = buf_load(rq, o1[r]);
         if (!INNER) {                                   // INNER: the tile's values are qn1[0 .. R0)
 #pragma unroll
-            for (int r = 0; r < R0; r++) qn0[r] = pq[k0[r] ? g0[r] : 0];   // tile cells past the grid edge: unused
+            for (int r = 0; r < R0; r++) qn0[r] = buf_load(rq, g0[r]);
         }
     };
     // Step k stages Ti plane k, runs the first sweep on plane k-2 (its Ti planes k-3..k-1
@@ -328,12 +345,12 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
 #pragma unroll
             for (int r = 0; r < R2; r++) {
                 const int t = tid + 256 * r;
-                if (t < N2) dst[t] = (INNER || off2[r] >= 0) ? pre[r] : 0.f;
+                if (t < N2) dst[t] = pre[r];
             }
         }
         float q1c[R1], q0c[R0];
 #pragma unroll
-        for (int r = 0; r < R1; r++) q1c[r] = (INNER || o1[r] >= 0) ? qn1[r] : 0.f;
+        for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
 #pragma unroll
         for (int r = 0; r < R0; r++) {                 // tile Q: plane k-4 now, k-3 and k-2 held
             q0c[r] = qo0[r];
@@ -341,7 +358,36 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
             qm0[r] = INNER ? q1c[r] : qn0[r];
         }
         if (k + 1 <= zb + 3) fetch(k + 1);
-        // (b) first sweep, plane p1 = k - 2, over R1
+        // (b) second sweep, plane p2 = k - 4, over the tile: its stores go out before the
+        // first sweep, which gives them that sweep's time to complete before the next step
+        // waits for its loads
+        const int p2 = k - 4;
+        if (p2 >= za && p2 < zb) {
+            const float *lm = L1[(p2 + 3) & 3], *l0 = L1[(p2 + 4) & 3], *lp = L1[(p2 + 5) & 3];
+            const int zc2 = jacobi_cls(p2, hz);
+            const auto rtf = plane_rsrc(Tf + (size_t)p2 * sz, pbytes);
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const int c1 = c1r[r], kd = k0[r];
+                auto ld = [&](int dx, int dy, int dz) {
+                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                    return pl[c1 + dy * W1 + dx];
+                };
+                auto lqv = [&] { return q0c[r]; };
+                const bool fast = !INNER && zc2 == 0 && f0[r];           // a wavefront of inner cells
+                if (INNER ? tid + 256 * r < N0 : (fast || (kd & 3) == 2)) {
+                    const float out = INNER  ? jacobi_voxel(c, 0, true, zc2, ld, lqv)
+                                      : fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
+                                             : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
+                    buf_store(rtf, g0[r], out);
+                    if (RESID) {
+                        const double d = (double)(out - l0[c1]);
+                        res += d * d;
+                    }
+                }
+            }
+        }
+        // (c) first sweep, plane p1 = k - 2, over R1
         const int p1 = k - 2;
         if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
             const float *lm = Li[(p1 + 3) & 3], *l0 = Li[(p1 + 4) & 3], *lp = Li[(p1 + 5) & 3];
@@ -368,32 +414,6 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
                         v = l0[c2];                                      // ovens keep their value
                     }
                     dst[p] = v;
-                }
-            }
-        }
-        // (c) second sweep, plane p2 = k - 4, over the tile
-        const int p2 = k - 4;
-        if (p2 >= za && p2 < zb) {
-            const float *lm = L1[(p2 + 3) & 3], *l0 = L1[(p2 + 4) & 3], *lp = L1[(p2 + 5) & 3];
-            const int zc2 = jacobi_cls(p2, hz);
-#pragma unroll
-            for (int r = 0; r < R0; r++) {
-                const int c1 = c1r[r], kd = k0[r];
-                auto ld = [&](int dx, int dy, int dz) {
-                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
-                    return pl[c1 + dy * W1 + dx];
-                };
-                auto lqv = [&] { return q0c[r]; };
-                const bool fast = !INNER && zc2 == 0 && f0[r];           // a wavefront of inner cells
-                if (INNER ? tid + 256 * r < N0 : (fast || (kd & 3) == 2)) {
-                    const float out = INNER  ? jacobi_voxel(c, 0, true, zc2, ld, lqv)
-                                      : fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
-                                             : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
-                    Tf[(size_t)p2 * sz + g0[r]] = out;
-                    if (RESID) {
-                        const double d = (double)(out - l0[c1]);
-                        res += d * d;
-                    }
                 }
             }
         }
