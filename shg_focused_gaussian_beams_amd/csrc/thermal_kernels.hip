@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "fft_lds.h"
 #include "nl_step.h"
@@ -309,115 +310,201 @@ __device__ __forceinline__ void jacobi2_march(float *__restrict__ Tf, const floa
     for (int r = 0; r < R0; r++)
         f0[r] = INNER || __builtin_amdgcn_ballot_w64(tid + 256 * r < N0 && k0[r] == kInnerKind) == ~0ull;
 
-    // Loads of step k (issued during step k-1): Ti plane k over R2, and Q plane k-2 at the
-    // R1 cells and (general form) at the tile cells: the first sweep uses the R1 values at
-    // step k, the second the tile values at step k+2 (held in registers meanwhile; the two
-    // reads of a Q plane come together, so the second hits the cache, and an INNER tile
-    // takes its tile values from its R1 values).  Every step loads (a plane index outside
-    // [0, hz) is clamped to the grid, its values unused): a kept old value, or a select at
-    // the load, would make the wave wait for the load right there, leaving no step of
-    // latency hidden.  Out-of-domain cells load at an offset past the plane, which the
-    // buffer range check turns into 0.
-    float pre[R2], qn1[R1] = {}, qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
     const int pbytes = (int)(4 * sz);
-    auto fetch = [&](int k) {
-        const auto rt = plane_rsrc(Ti + (size_t)min(max(k, 0), hz - 1) * sz, pbytes);
+    double res = 0.0;
+    if constexpr (INNER) {
+        // The march, step k = za-4 .. zb+3 (the first two steps only load).  Step k stages Ti
+        // plane k into ring slot k&3, runs the first sweep on plane k-2 (Ti planes k-3..k-1,
+        // staged by earlier steps; first-sweep slot (k-2)&3), loads Ti plane k+2 and Q plane k
+        // (register set k&1, whose previous contents, Ti plane k and Q plane k-2, this step has
+        // just used), and runs the second sweep on plane k-4 (first-sweep planes k-5..k-3, of
+        // earlier steps).  No ring slot is written and read in one step (rings of four planes),
+        // so one barrier per step, at its end, orders everything, and a load has a step and a
+        // half to arrive.  The loop is unrolled by four so that every ring slot and register
+        // set is fixed at compile time (no per-cell slot arithmetic, no register copies: a copy
+        // of a loaded register, or a select at the load, makes the wave wait for the load right
+        // there).  Q over R1 serves the first sweep, and its first R0 slots (the tile cells,
+        // ring_cell_tile_first) are held two steps in qh[slot] for the second.  Plane indices
+        // outside [0, hz) are clamped to the grid (values unused).  The general form below
+        // keeps one step of loads in flight: unrolled, its per-class code outgrew the
+        // instruction cache and its registers.
+        float pre[2][R2] = {}, qn1[2][R1] = {}, qh[4][R0];
+        auto step = [&](auto slot, int k) {
+            constexpr int S = decltype(slot)::value, P = S & 1;
+            if (k < za - 4 || k > zb + 3) return;
+            const bool live = k >= za - 2;                   // the first two steps only load
+            // (a) stage Ti plane k
+            if (live && k >= 0 && k < hz) {
 #pragma unroll
-        for (int r = 0; r < R2; r++) pre[r] = buf_load(rt, off2[r]);
-        const auto rq = plane_rsrc(Q + (size_t)min(max(k - 2, 0), hz - 1) * sz, pbytes);
+                for (int r = 0; r < R2; r++) {
+                    const int t = tid + 256 * r;
+                    if (t < N2) Li[S][t] = pre[P][r];
+                }
+            }
+            // (b) first sweep, plane p1 = k - 2, over R1
+            const int p1 = k - 2;
+            if (live && p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
+                const float *lm = Li[(S + 1) & 3], *l0 = Li[(S + 2) & 3], *lp = Li[(S + 3) & 3];
+                float *dst = L1[(S + 2) & 3];
+                const int zc1 = jacobi_cls(p1, hz);
 #pragma unroll
-        for (int r = 0; r < R1; r++) qn1[r] = buf_load(rq, o1[r]);
-        if (!INNER) {                                   // INNER: the tile's values are qn1[0 .. R0)
+                for (int r = 0; r < R1; r++) {
+                    const int t = tid + 256 * r;
+                    const int c2 = c2r[r], p = q1[r];
+                    auto ld = [&](int dx, int dy, int dz) {
+                        const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                        return pl[c2 + dy * W2 + dx];
+                    };
+                    auto lqv = [&] { return qn1[P][r]; };
+                    if (t < N1) dst[p] = jacobi_voxel(c, 0, true, zc1, ld, lqv);
+                }
+            }
+            // (c) hold the tile's Q of plane k-2 for the second sweep at step k+2, then load
+#pragma unroll
+            for (int r = 0; r < R0; r++) qh[S][r] = qn1[P][r];
+            if (k + 2 <= zb + 3) {
+                const auto rt = plane_rsrc(Ti + (size_t)min(max(k + 2, 0), hz - 1) * sz, pbytes);
+#pragma unroll
+                for (int r = 0; r < R2; r++) pre[P][r] = buf_load(rt, off2[r]);
+                const auto rq = plane_rsrc(Q + (size_t)min(max(k, 0), hz - 1) * sz, pbytes);
+#pragma unroll
+                for (int r = 0; r < R1; r++) qn1[P][r] = buf_load(rq, o1[r]);
+            }
+            // (d) second sweep, plane p2 = k - 4, over the tile
+            const int p2 = k - 4;
+            if (live && p2 >= za && p2 < zb) {
+                const float *lm = L1[(S + 3) & 3], *l0 = L1[S], *lp = L1[(S + 1) & 3];
+                const int zc2 = jacobi_cls(p2, hz);
+                const auto rtf = plane_rsrc(Tf + (size_t)p2 * sz, pbytes);
+#pragma unroll
+                for (int r = 0; r < R0; r++) {
+                    const int c1 = c1r[r];
+                    auto ld = [&](int dx, int dy, int dz) {
+                        const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                        return pl[c1 + dy * W1 + dx];
+                    };
+                    auto lqv = [&] { return qh[(S + 2) & 3][r]; };
+                    if (tid + 256 * r < N0) {
+                        const float out = jacobi_voxel(c, 0, true, zc2, ld, lqv);
+                        buf_store(rtf, g0[r], out);
+                        if (RESID) {
+                            const double d = (double)(out - l0[c1]);
+                            res += d * d;
+                        }
+                    }
+                }
+            }
+            if (live) __syncthreads();
+        };
+        for (int kb = (za - 4) & ~3; kb <= zb + 3; kb += 4) {
+            step(std::integral_constant<int, 0>{}, kb);
+            step(std::integral_constant<int, 1>{}, kb + 1);
+            step(std::integral_constant<int, 2>{}, kb + 2);
+            step(std::integral_constant<int, 3>{}, kb + 3);
+        }
+    } else {
+        // Loads of step k (issued during step k-1): Ti plane k over R2, and Q plane k-2 at the
+        // R1 cells and at the tile cells: the first sweep uses the R1 values at step k, the
+        // second the tile values at step k+2 (held in registers meanwhile; the two reads of a
+        // Q plane come together, so the second hits the cache).  Every step loads (a plane index outside
+        // [0, hz) is clamped to the grid, its values unused): a kept old value, or a select at
+        // the load, would make the wave wait for the load right there, leaving no step of
+        // latency hidden.  Out-of-domain cells load at an offset past the plane, which the
+        // buffer range check turns into 0.
+        float pre[R2], qn1[R1] = {}, qn0[R0] = {}, qm0[R0] = {}, qo0[R0] = {};
+        auto fetch = [&](int k) {
+            const auto rt = plane_rsrc(Ti + (size_t)min(max(k, 0), hz - 1) * sz, pbytes);
+#pragma unroll
+            for (int r = 0; r < R2; r++) pre[r] = buf_load(rt, off2[r]);
+            const auto rq = plane_rsrc(Q + (size_t)min(max(k - 2, 0), hz - 1) * sz, pbytes);
+#pragma unroll
+            for (int r = 0; r < R1; r++) qn1[r] = buf_load(rq, o1[r]);
 #pragma unroll
             for (int r = 0; r < R0; r++) qn0[r] = buf_load(rq, g0[r]);
-        }
-    };
-    // Step k stages Ti plane k, runs the first sweep on plane k-2 (its Ti planes k-3..k-1
-    // were staged by earlier steps) and the second on plane k-4 (its first-sweep planes
-    // k-5..k-3 come from earlier steps): within a step no ring slot is both written and
-    // read (rings of four planes), so one barrier per step, at its end, orders everything.
-    double res = 0.0;
-    fetch(za - 2);
-    for (int k = za - 2; k <= zb + 3; k++) {
-        // (a) stage Ti plane k, take this step's Q values, then load the next step's
-        if (k >= 0 && k < hz) {
-            float *dst = Li[(k + 4) & 3];
+        };
+        // Step k stages Ti plane k, runs the first sweep on plane k-2 (its Ti planes k-3..k-1
+        // were staged by earlier steps) and the second on plane k-4 (its first-sweep planes
+        // k-5..k-3 come from earlier steps): within a step no ring slot is both written and
+        // read (rings of four planes), so one barrier per step, at its end, orders everything.
+        fetch(za - 2);
+        for (int k = za - 2; k <= zb + 3; k++) {
+            // (a) stage Ti plane k, take this step's Q values, then load the next step's
+            if (k >= 0 && k < hz) {
+                float *dst = Li[(k + 4) & 3];
 #pragma unroll
-            for (int r = 0; r < R2; r++) {
-                const int t = tid + 256 * r;
-                if (t < N2) dst[t] = pre[r];
+                for (int r = 0; r < R2; r++) {
+                    const int t = tid + 256 * r;
+                    if (t < N2) dst[t] = pre[r];
+                }
             }
-        }
-        float q1c[R1], q0c[R0];
+            float q1c[R1], q0c[R0];
 #pragma unroll
-        for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
+            for (int r = 0; r < R1; r++) q1c[r] = qn1[r];
 #pragma unroll
-        for (int r = 0; r < R0; r++) {                 // tile Q: plane k-4 now, k-3 and k-2 held
-            q0c[r] = qo0[r];
-            qo0[r] = qm0[r];
-            qm0[r] = INNER ? q1c[r] : qn0[r];
-        }
-        if (k + 1 <= zb + 3) fetch(k + 1);
-        // (b) second sweep, plane p2 = k - 4, over the tile: its stores go out before the
-        // first sweep, which gives them that sweep's time to complete before the next step
-        // waits for its loads
-        const int p2 = k - 4;
-        if (p2 >= za && p2 < zb) {
-            const float *lm = L1[(p2 + 3) & 3], *l0 = L1[(p2 + 4) & 3], *lp = L1[(p2 + 5) & 3];
-            const int zc2 = jacobi_cls(p2, hz);
-            const auto rtf = plane_rsrc(Tf + (size_t)p2 * sz, pbytes);
+            for (int r = 0; r < R0; r++) {                 // tile Q: plane k-4 now, k-3 and k-2 held
+                q0c[r] = qo0[r];
+                qo0[r] = qm0[r];
+                qm0[r] = qn0[r];
+            }
+            if (k + 1 <= zb + 3) fetch(k + 1);
+            // (b) second sweep, plane p2 = k - 4, over the tile: its stores go out before the
+            // first sweep, which gives them that sweep's time to complete before the next step
+            // waits for its loads
+            const int p2 = k - 4;
+            if (p2 >= za && p2 < zb) {
+                const float *lm = L1[(p2 + 3) & 3], *l0 = L1[(p2 + 4) & 3], *lp = L1[(p2 + 5) & 3];
+                const int zc2 = jacobi_cls(p2, hz);
+                const auto rtf = plane_rsrc(Tf + (size_t)p2 * sz, pbytes);
 #pragma unroll
-            for (int r = 0; r < R0; r++) {
-                const int c1 = c1r[r], kd = k0[r];
-                auto ld = [&](int dx, int dy, int dz) {
-                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
-                    return pl[c1 + dy * W1 + dx];
-                };
-                auto lqv = [&] { return q0c[r]; };
-                const bool fast = !INNER && zc2 == 0 && f0[r];           // a wavefront of inner cells
-                if (INNER ? tid + 256 * r < N0 : (fast || (kd & 3) == 2)) {
-                    const float out = INNER  ? jacobi_voxel(c, 0, true, zc2, ld, lqv)
-                                      : fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
-                                             : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
-                    buf_store(rtf, g0[r], out);
-                    if (RESID) {
-                        const double d = (double)(out - l0[c1]);
-                        res += d * d;
+                for (int r = 0; r < R0; r++) {
+                    const int c1 = c1r[r], kd = k0[r];
+                    auto ld = [&](int dx, int dy, int dz) {
+                        const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                        return pl[c1 + dy * W1 + dx];
+                    };
+                    auto lqv = [&] { return q0c[r]; };
+                    const bool fast = zc2 == 0 && f0[r];                     // a wavefront of inner cells
+                    if (fast || (kd & 3) == 2) {
+                        const float out = fast ? jacobi_voxel(c, 0, true, 0, ld, lqv)
+                                               : jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc2, ld, lqv);
+                        buf_store(rtf, g0[r], out);
+                        if (RESID) {
+                            const double d = (double)(out - l0[c1]);
+                            res += d * d;
+                        }
                     }
                 }
             }
-        }
-        // (c) first sweep, plane p1 = k - 2, over R1
-        const int p1 = k - 2;
-        if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
-            const float *lm = Li[(p1 + 3) & 3], *l0 = Li[(p1 + 4) & 3], *lp = Li[(p1 + 5) & 3];
-            float *dst = L1[(p1 + 4) & 3];
-            const int zc1 = jacobi_cls(p1, hz);
+            // (c) first sweep, plane p1 = k - 2, over R1
+            const int p1 = k - 2;
+            if (p1 >= max(za - 1, 0) && p1 <= min(zb, hz - 1)) {
+                const float *lm = Li[(p1 + 3) & 3], *l0 = Li[(p1 + 4) & 3], *lp = Li[(p1 + 5) & 3];
+                float *dst = L1[(p1 + 4) & 3];
+                const int zc1 = jacobi_cls(p1, hz);
 #pragma unroll
-            for (int r = 0; r < R1; r++) {
-                const int t = tid + 256 * r;
-                const int c2 = c2r[r], kd = k1[r], p = q1[r];
-                auto ld = [&](int dx, int dy, int dz) {
-                    const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
-                    return pl[c2 + dy * W2 + dx];
-                };
-                auto lqv = [&] { return q1c[r]; };
-                if (INNER) {                                             // inner cells only
-                    if (t < N1) dst[p] = jacobi_voxel(c, 0, true, zc1, ld, lqv);
-                } else if (zc1 == 0 && f1[r]) {                          // a wavefront of inner cells
-                    dst[p] = jacobi_voxel(c, 0, true, 0, ld, lqv);
-                } else if (t < N1) {
-                    float v = 0.f;
-                    if ((kd & 3) == 2) {
-                        v = jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc1, ld, lqv);
-                    } else if ((kd & 3) == 1) {
-                        v = l0[c2];                                      // ovens keep their value
+                for (int r = 0; r < R1; r++) {
+                    const int t = tid + 256 * r;
+                    const int c2 = c2r[r], kd = k1[r], p = q1[r];
+                    auto ld = [&](int dx, int dy, int dz) {
+                        const float *pl = dz < 0 ? lm : (dz > 0 ? lp : l0);
+                        return pl[c2 + dy * W2 + dx];
+                    };
+                    auto lqv = [&] { return q1c[r]; };
+                    if (zc1 == 0 && f1[r]) {                          // a wavefront of inner cells
+                        dst[p] = jacobi_voxel(c, 0, true, 0, ld, lqv);
+                    } else if (t < N1) {
+                        float v = 0.f;
+                        if ((kd & 3) == 2) {
+                            v = jacobi_voxel(c, ((kd >> 2) & 3) - 1, (kd >> 4) & 1, zc1, ld, lqv);
+                        } else if ((kd & 3) == 1) {
+                            v = l0[c2];                                      // ovens keep their value
+                        }
+                        dst[p] = v;
                     }
-                    dst[p] = v;
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (RESID) {
         for (int off = 32; off > 0; off >>= 1) res += __shfl_down(res, off, 64);
