@@ -236,3 +236,91 @@ def test_interior_tile_first_order(TX, TY):
             t = tid + 256 * r
             tile = region_cell(t, TX, TY, 0, TX, 0, TY)    # an interior tile's order
             assert cells[t] == (tile[0] + 1, tile[1] + 1)
+
+
+def emulate_interior_march(Ti, Q, TX, TY, zc, upd):
+    """The interior tiles' march (jacobi2_march<..., INNER>), restated for one instance with
+    every tile marched that way (the schedule does not depend on the cell formulas): steps
+    k = za-4 .. zb+3, unrolled by four in the kernel so that ring slots (plane & 3) and
+    register sets (k & 1) are fixed; step k stages Ti plane k from set k&1, runs the first
+    sweep on plane k-2 with Q plane k-2 from set k&1, holds that plane's tile values in
+    qh[k&3], loads Ti plane k+2 and Q plane k into set k&1, and runs the second sweep on
+    plane k-4 with qh[(k+2)&3].  Registers and ring slots carry (plane, step written) tags,
+    so every read is checked to see the plane it needs, written at an earlier step (ring
+    slots) or at least one step earlier (loads)."""
+    hz, ny, nx = Ti.shape
+    W2, N2, W1, N1 = TX+4, (TX+4)*(TY+4), TX+2, (TX+2)*(TY+2)
+    tiles_x, tiles_y, nch = -(-nx//TX), -(-ny//TY), -(-hz//zc)
+    Tf = np.where(np.array([[upd(x, y, nx, ny) for x in range(nx)] for y in range(ny)])[None], np.nan, Ti)
+    for lb in range(tiles_x*tiles_y*nch):
+        tile, chunk = divmod(lb, nch)
+        ty, tx = divmod(tile, tiles_x)
+        x0, y0 = tx*TX, ty*TY
+        za, zb = chunk*zc, min(hz, chunk*zc + zc)
+        Li = np.full((4, N2), np.nan); L1 = np.full((4, N1), np.nan)
+        Li_at = [(None, None)] * 4; L1_at = [(None, None)] * 4
+        pre = [(None, None, None)] * 2; qn1 = [(None, None, None)] * 2; qh = [(None, None)] * 4
+        def plane_vals(A, k, xs, ys):
+            kk = min(max(k, 0), hz-1)
+            return np.array([A[kk, gy, gx] if (0 <= gx < nx and 0 <= gy < ny) else 0.0 for gx, gy in zip(xs, ys)])
+        r2 = [divmod(t, W2) for t in range(N2)]
+        r1 = [divmod(t, W1) for t in range(N1)]
+        kb = (za - 4) & ~3
+        for k in range(kb, zb + 4):
+            if k < za - 4:
+                continue
+            S, P, live = k & 3, k & 1, k >= za - 2
+            if live and 0 <= k < hz:
+                vals, plane, at = pre[P]
+                assert plane == k and at < k, (k, plane, at)
+                Li[S] = vals; Li_at[S] = (k, k)
+            p1 = k - 2
+            if live and max(za-1, 0) <= p1 <= min(zb, hz-1):
+                qv, qplane, qat = qn1[P]
+                assert qplane == p1 and qat < k
+                for t, (row, col) in enumerate(r1):
+                    gx, gy = x0-1+col, y0-1+row
+                    c2 = (row+1)*W2 + col+1
+                    v = 0.0
+                    if 0 <= gx < nx and 0 <= gy < ny:
+                        def ld(dx, dy, dz):
+                            slot = (p1+dz) & 3
+                            assert Li_at[slot][0] == p1+dz and Li_at[slot][1] < k, (k, slot, Li_at[slot])
+                            return Li[slot, c2 + dy*W2 + dx]
+                        v = voxel(ld, gx, gy, p1, nx, ny, hz) + qv[t] if upd(gx, gy, nx, ny) else ld(0, 0, 0)
+                    L1[p1 & 3, t] = v
+                L1_at[p1 & 3] = (p1, k)
+            assert S == (p1 + 2) & 3
+            qh[S] = (qn1[P][0], qn1[P][1])                       # the tile's Q of plane k-2
+            if k + 2 <= zb + 3:
+                pre[P] = (plane_vals(Ti, k+2, [x0-2+c for _, c in r2], [y0-2+r for r, _ in r2]), k+2, k)
+                qn1[P] = (plane_vals(Q, k, [x0-1+c for _, c in r1], [y0-1+r for r, _ in r1]), k, k)
+            p2 = k - 4
+            if live and za <= p2 < zb:
+                qv, qplane = qh[(S+2) & 3]
+                assert qplane == p2
+                for row in range(TY):
+                    for col in range(TX):
+                        gx, gy = x0+col, y0+row
+                        if not (gx < nx and gy < ny and upd(gx, gy, nx, ny)):
+                            continue
+                        c1 = (row+1)*W1 + col+1
+                        def ld(dx, dy, dz):
+                            slot = (p2+dz) & 3
+                            assert L1_at[slot][0] == p2+dz and L1_at[slot][1] < k, (k, slot, L1_at[slot])
+                            return L1[slot, c1 + dy*W1 + dx]
+                        Tf[p2, gy, gx] = voxel(ld, gx, gy, p2, nx, ny, hz) + qv[c1]
+    return Tf
+
+
+@pytest.mark.parametrize("nx,ny,hz,TX,TY,zc,bc", [(20, 13, 11, 8, 4, 3, "top"), (16, 9, 7, 8, 4, 4, "oven"),
+                                                   (12, 12, 2, 16, 8, 5, "top"), (24, 10, 9, 8, 4, 2, "top")])
+def test_interior_march_schedule_equals_two_sweeps(nx, ny, hz, TX, TY, zc, bc):
+    upd = _top if bc == "top" else _oven
+    rng = np.random.default_rng(nx * 10 + hz)
+    Ti = rng.random((hz, ny, nx))
+    Q = 0.01 * rng.random((hz, ny, nx))
+    ref = sweep(sweep(Ti, Q, upd), Q, upd)
+    em = emulate_interior_march(Ti, Q, TX, TY, zc, upd)
+    assert not np.isnan(em).any()
+    assert np.array_equal(em, ref)
