@@ -757,23 +757,31 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_wide(float4 *__restrict__ 
     float2 nnp[CPT], nnc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
     const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
+    // The row (transform lanes) and the twiddles are loaded and the twiddles staged first,
+    // the table entries loaded after them, so that the table loads are in flight behind
+    // the forward transform (a wait issued after them would also wait for them: loads
+    // complete in issue order).
+    float4 uv[E];
+    if (tid < TT) {
+#pragma unroll
+        for (int j = 0; j < E; j++) uv[j] = U[base + tid + j * TT];
+    }
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const size_t idx = base + tid + q * THREADS;
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
+        nnc[q] = (cu && !same) ? NT[scur + idx] : make_float2(0.f, 0.f);   // unused when same
     }
-    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     __syncthreads();
     auto sync = [] { wave_sync(); };
     if (tid < TT) {
         float2 fa[E], fb[E];
 #pragma unroll
         for (int j = 0; j < E; j++) {
-            const float4 v = U[base + tid + j * TT];
-            fa[j] = make_float2(v.x, v.y);
-            fb[j] = make_float2(v.z, v.w);
+            fa[j] = make_float2(uv[j].x, uv[j].y);
+            fb[j] = make_float2(uv[j].z, uv[j].w);
         }
         if (a.flags & TROW_FWD) fft2fe<N, E, -1>(fa, fb, lds[0], lds[1], tid, 1, tw, sync);
         wave_sync();
@@ -826,19 +834,22 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_split(float4 *__restrict__
     float2 nnp[CPT], nnc[CPT];
     const bool pv = a.flags & (TROW_NPREV | TROW_QPREV), cu = a.flags & TROW_NCUR;
     const bool same = pv && a.prev.slice == a.cur.slice;      // consecutive steps share a heat slice
+    // The row and the twiddles are staged first and the table entries loaded after them,
+    // so that the table loads are in flight behind the forward transform (a wait for the
+    // row issued after them would also wait for them: loads complete in issue order).
+    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
+    for (int c = tid; c < N; c += THREADS) {
+        const float4 v = U[base + c];
+        lds[0][lds_pad(c)] = make_float2(v.x, v.y);
+        lds[1][lds_pad(c)] = make_float2(v.z, v.w);
+    }
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
         const int c = tid + q * THREADS;
         const size_t idx = base + (c < N ? c : 0);
         dkp[q] = (a.flags & TROW_FIRST) ? 0.0f : DKint[idx];
         nnp[q] = pv ? NT[sprev + idx] : make_float2(0.f, 0.f);
-        nnc[q] = !cu ? make_float2(0.f, 0.f) : same ? nnp[q] : NT[scur + idx];
-    }
-    for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
-    for (int c = tid; c < N; c += THREADS) {
-        const float4 v = U[base + c];
-        lds[0][lds_pad(c)] = make_float2(v.x, v.y);
-        lds[1][lds_pad(c)] = make_float2(v.z, v.w);
+        nnc[q] = (cu && !same) ? NT[scur + idx] : make_float2(0.f, 0.f);   // unused when same
     }
     __syncthreads();
     if ((a.flags & TROW_FWD) && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
