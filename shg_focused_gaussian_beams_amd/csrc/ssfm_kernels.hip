@@ -43,8 +43,8 @@ __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict_
     const int row = blockIdx.x * C::ROWS + rr;
     const float4 c0 = gp(d.coef)[step];
     const float4 c1 = gp(d.coef)[mode == ROW_MID ? step + 1 : step];
-    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
-    __syncthreads();
+    // the row's loads go out before the twiddles are staged, so that the two latencies
+    // overlap instead of adding up across the barrier
     const auto Ur = gp(d.U) + (size_t)row * N;
     float2 fa[E], fb[E];
 #pragma unroll
@@ -53,6 +53,8 @@ __global__ __launch_bounds__(64 * W) void k_row_pass(const InstDesc *__restrict_
         fa[j] = make_float2(v.x, v.y);
         fb[j] = make_float2(v.z, v.w);
     }
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
     auto sync = [] { block_sync(); };
     if (mode != ROW_START) {
         fft2fe<N, E, -1>(fa, fb, lds[0][rr], lds[1][rr], t, 1, tw, sync);
@@ -88,8 +90,9 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
     const int tid = threadIdx.x;
     const int c = tid % CW, t = tid / CW;
     const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
-    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
-    __syncthreads();
+    // the column's loads go out before the twiddles are staged, so that the two latencies
+    // overlap instead of adding up across the barrier; the propagator factors, loaded
+    // before the transform, hide theirs behind it
     float2 fa[E], fb[E];
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
         fa[j] = make_float2(v.x, v.y);
         fb[j] = make_float2(v.z, v.w);
     }
-    // the propagator factors are loaded before the transform hides their latency
+    for (int j = tid; j < N; j += C::THREADS) tw[j] = twg[j];
     const float2 gxp = G[x], gxs = G[2 * N + x];
     float2 gyp[E], gys[E];
 #pragma unroll
@@ -105,6 +108,7 @@ __global__ __launch_bounds__(CW *(N / E)) void k_col_pass(const InstDesc *__rest
         gyp[j] = G[N + t + j * T];
         gys[j] = G[3 * N + t + j * T];
     }
+    __syncthreads();
     auto sync = [] { block_sync(); };
     fft2fe<N, E, +1>(fa, fb, &lds[0][c], &lds[1][c], t, CW, tw, sync);
 #pragma unroll
@@ -138,14 +142,16 @@ __global__ __launch_bounds__(2 * CW *(N / E)) void k_col_fsplit(const InstDesc *
     const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF;
     const int c = r % CW, t = r / CW;
     const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
-    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
-    __syncthreads();
+    // the column's loads go out before the twiddles are staged (the latencies overlap
+    // instead of adding up across the barrier)
     float2 a[E], gy[E];
 #pragma unroll
     for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
     const float2 gx = G[2 * f * N + x];                     // Gx of the field, then its Gy
 #pragma unroll
     for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];
+    __syncthreads();
     auto sync = [] { block_sync(); };
     fft1fe<N, E, +1>(a, &lds[f][c], t, CW, tw, sync);
 #pragma unroll

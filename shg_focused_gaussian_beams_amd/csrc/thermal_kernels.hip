@@ -695,10 +695,10 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
     const int lane = threadIdx.x;
     const int t = lane % TT, rr = lane / TT;
     const int row = blockIdx.x * C::ROWS + rr;
-    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
-    __syncthreads();
     const size_t n2 = (size_t)N * N;
     const size_t base = (size_t)row * N;
+    // the row's loads go out before the twiddles are staged (the latencies overlap
+    // instead of adding up across the barrier)
     float2 fa[E], fb[E];
 #pragma unroll
     for (int j = 0; j < E; j++) {
@@ -706,6 +706,8 @@ __global__ __launch_bounds__(64 * W) void k_therm_row_pass(float4 *__restrict__ 
         fa[j] = make_float2(v.x, v.y);
         fb[j] = make_float2(v.z, v.w);
     }
+    for (int j = lane; j < N; j += C::THREADS) tw[j] = twg[j];
+    __syncthreads();
     auto sync = [] { __syncthreads(); };
     float2 *la = lds[0][rr], *lb = lds[1][rr];
     if (a.flags & TROW_FWD) fft2fe<N, E, -1>(fa, fb, la, lb, t, 1, tw, sync);
