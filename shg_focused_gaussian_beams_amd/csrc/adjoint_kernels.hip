@@ -182,14 +182,16 @@ __global__ __launch_bounds__(2 * AdjColF<N>::HALF) void k_col_tape_f(const InstD
     const auto G = gp(d.G);
     const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF, c = r % CW, t = r / CW;
     const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
-    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
-    __syncthreads();
+    // the column's loads go out before the twiddles are staged (the latencies overlap
+    // instead of adding up across the barrier)
     float2 a[E], gy[E];
 #pragma unroll
     for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
     const float2 gx = G[2 * f * N + x];
 #pragma unroll
     for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];
+    __syncthreads();
     auto sync = [] { __syncthreads(); };
     fft1fe<N, E, +1>(a, &lds[f][c], t, CW, tw, sync);
     if (tp.spec) {
@@ -219,12 +221,14 @@ __global__ __launch_bounds__(2 * AdjColF<N>::HALF) void k_adj_col_f(const InstDe
     const auto G = gp(d.G);
     const int tid = threadIdx.x, f = tid / HALF, r = tid - f * HALF, c = r % CW, t = r / CW;
     const int x = xcd_group(blockIdx.x, gridDim.x) * CW + c;
-    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
-    __syncthreads();
+    // the column's loads go out before the twiddles are staged (the latencies overlap
+    // instead of adding up across the barrier)
     float2 a[E], gy[E], xs[E];
     float q[E];
 #pragma unroll
     for (int j = 0; j < E; j++) a[j] = U[2 * ((size_t)(t + j * T) * N + x) + f];
+    for (int j = tid; j < N; j += 2 * HALF) tw[j] = twg[j];
+    __syncthreads();
     const float2 gx = G[2 * f * N + x];
 #pragma unroll
     for (int j = 0; j < E; j++) gy[j] = G[(2 * f + 1) * N + t + j * T];

@@ -92,30 +92,24 @@ __global__ __launch_bounds__(64 * AdjWide<N>::W) void k_therm_adj_row(const Inst
     const bool cell = tid < N;
     const size_t n2 = (size_t)N * N, base = (size_t)blockIdx.x * N, idx = base + (cell ? tid : 0);
     const int ls = a.cur.step - a.s0;
-    // the cell's inputs, loaded before the transform hides their latency
-    const float2 nn_c = a.NT[(size_t)a.cur.slice * n2 + idx];
-    float2 nn_p = make_float2(0.f, 0.f);
-    float4 A;
-    float dk_c, dk_p = 0.f;
-    if (mode == ADJ_START) {
-        A = a.a2[(size_t)ls * n2 + idx];
-        dk_c = a.dk[(size_t)ls * n2 + idx];
-    } else if (mode == ADJ_END) {
-        A = a.a0[idx];
-        dk_c = a.dk[(size_t)ls * n2 + idx];
-    } else {
-        A = a.a2[(size_t)(ls - 1) * n2 + idx];
-        dk_p = a.dk[(size_t)(ls - 1) * n2 + idx];
-        dk_c = a.dk[(size_t)ls * n2 + idx];
-        nn_p = a.NT[(size_t)a.prev.slice * n2 + idx];
-    }
-    double G = SENS ? a.G[idx] : 0.0, Dp = SENS ? a.Dp[idx] : 0.0;
+    // The row is staged first, then the cell's inputs are loaded, from addresses chosen
+    // per mode (uniform) with no per-lane branches or copies of loaded values: they are
+    // then in flight behind the transform (a copy of a loaded value, or a wait for the row
+    // issued after them, would make the wave wait for them before the row's loads).
     for (int j = tid; j < N; j += THREADS) tw[j] = twg[j];
     if (cell) {
         const float4 v = U[idx];
         lds[0][lds_pad(tid)] = make_float2(v.x, v.y);
         lds[1][lds_pad(tid)] = make_float2(v.z, v.w);
     }
+    const size_t lp1 = (size_t)(ls > 0 ? ls - 1 : 0);
+    const float4 *Ab = mode == ADJ_START ? a.a2 + (size_t)ls * n2 : mode == ADJ_END ? a.a0 : a.a2 + lp1 * n2;
+    const float4 A = Ab[idx];
+    const float2 nn_c = a.NT[(size_t)a.cur.slice * n2 + idx];
+    const float dk_c = a.dk[(size_t)ls * n2 + idx];
+    const float dk_p = a.dk[lp1 * n2 + idx];                  // used in ADJ_MID only
+    const float2 nn_p = mode == ADJ_MID ? a.NT[(size_t)a.prev.slice * n2 + idx] : make_float2(0.f, 0.f);
+    double G = SENS ? a.G[idx] : 0.0, Dp = SENS ? a.Dp[idx] : 0.0;
     __syncthreads();
     if (mode != ADJ_START && f < 2 && lane < T) fft_lds_field<N, E, -1>(lds[f], lane, tw);
     __syncthreads();
