@@ -94,6 +94,10 @@ namespace {
 
 void batch_release(shg_batch *b) {
     if (!b) return;
+    // work an error return left in flight ends before its graph and buffers go
+    for (hipStream_t x : b->xs)
+        if (x) (void)hipStreamSynchronize(x);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     for (hipStream_t x : b->xs)
         if (x) (void)hipStreamDestroy(x);

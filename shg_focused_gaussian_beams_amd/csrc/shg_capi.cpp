@@ -283,6 +283,9 @@ double power_of(const shg_ctx *c, int which, double sum, double temp) {
 
 void release(shg_ctx *c) {
     if (!c) return;
+    // work an error return left in flight ends before its graphs and buffers go
+    if (c->adj.ts) (void)hipStreamSynchronize(c->adj.ts);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (void *ptr : {(void *)c->U, (void *)c->U0, (void *)c->Uout, (void *)c->tw, (void *)c->G, (void *)c->part,
                       (void *)c->d_coef, (void *)c->d_desc,

@@ -98,6 +98,7 @@ namespace {
 
 void tb_release(shg_thermal_batch *b) {
     if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);   // work an error return left in flight
     if (b->gmarch) (void)hipGraphExecDestroy(b->gmarch);
     for (void *ptr : {(void *)b->U, (void *)b->Uout, (void *)b->tw, (void *)b->G, (void *)b->NT, (void *)b->T[0],
                       (void *)b->T[1], (void *)b->Q, (void *)b->DKint, (void *)b->desc, (void *)b->active, (void *)b->ids,
