@@ -212,10 +212,11 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, int off, flo
 // staged by earlier steps), and the second sweep's plane k-4 on the tile (from T1 planes
 // of earlier steps), which it writes: no ring slot is written and read in the same step,
 // so one barrier per step suffices.  Ti plane k+1 and Q plane k-1 (over R1 and over the
-// tile, each thread its own cells) are loaded into registers one step ahead, so that their
-// latency overlaps this step's arithmetic; the tile's values wait two more steps in
-// registers for the second sweep.  Q never goes through LDS, which keeps the footprint to
-// the two rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
+// tile, each thread its own cells) are loaded into registers one step ahead (interior
+// tiles: two steps ahead, jacobi2_march), so that their latency overlaps the step's
+// arithmetic; the tile's values wait two more steps in registers for the second sweep.
+// Loads and stores are buffer operations on a per-plane descriptor.  Q never goes
+// through LDS, which keeps the footprint to the two rings.  T1 never leaves the chip: a pass reads Ti once over R2 (plus the z halo of its
 // chunk) and Q over R1, and writes Tf once, where two single sweeps read and write
 // everything twice.  The halo values of T1 are recomputed by every tile that needs them,
 // with the same formulas, so the result is two single sweeps bit for bit.  With RESID the
@@ -938,6 +939,7 @@ static int j2_slots() {
 
 int jacobi2_plan(int nx, int ny, int hz, int variant, Jacobi2Plan *plan, int count) {
     if (variant < 0 || variant >= kJacobi2VariantCount || !plan || nx < 2 || ny < 2 || hz < 2 || count < 1) return -1;
+    if ((int64_t)nx * ny * 4 > INT32_MAX) return -1;   // a plane is one buffer descriptor's range
     const int shape = variant / 3, waves = kJ2Waves[variant % 3];
     const int tx = kJ2Shape[shape][0], ty = kJ2Shape[shape][1];
     const int slots = shape == 0 ? j2_slots<64, 16>() : shape == 1 ? j2_slots<64, 8>() : j2_slots<128, 8>();
