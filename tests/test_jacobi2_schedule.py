@@ -8,8 +8,10 @@ earlier step (never by the same one).  A toy stencil that reads every in-domain
 neighbour stands in for the Jacobi formulas (which the kernel shares with the
 single-sweep kernel); the schedule must give two plain sweeps exactly, read only staged
 ring entries, and stay inside its rings — on grids that are not multiples of the tile,
-with chunks that do not divide the depth, with both boundary kinds.  The bit-exactness of
-the real formulas is a GPU test (tests/test_gpu_jacobi2.py).
+with chunks that do not divide the depth, with both boundary kinds.  The interior tiles'
+march (unrolled by four in the kernel: fixed ring slots, two register sets of loads two
+steps ahead, the tile's Q held in registers) is restated and checked the same way.  The
+bit-exactness of the real formulas is a GPU test (tests/test_gpu_jacobi2.py).
 """
 import itertools
 
