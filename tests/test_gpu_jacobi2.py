@@ -38,7 +38,10 @@ def _oracle(p, hz, T, Q, n):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,hz,bc", [(64, 12, SHG_BC_TOP_OPEN), (64, 37, SHG_BC_OVEN_SURROUNDED),
                                      (128, 23, SHG_BC_TOP_OPEN), (256, 9, SHG_BC_TOP_OPEN),
-                                     (512, 5, SHG_BC_OVEN_SURROUNDED)])
+                                     (512, 5, SHG_BC_OVEN_SURROUNDED),
+                                     # interior tiles (unrolled march) with chunks that start
+                                     # off a multiple of four planes, both boundary kinds
+                                     (256, 37, SHG_BC_TOP_OPEN), (256, 30, SHG_BC_OVEN_SURROUNDED)])
 def test_every_jacobi_layout_bit_exact(gpu, n, hz, bc):
     p = thermal256_params(nx=n, ny=n, nz=max(hz, 2), heat_nz=hz, lx=2.0 * n, ly=1.5 * n, thermal_bc=bc)
     print(f"\njacobi layouts n={n} hz={hz} bc={bc}", flush=True)
