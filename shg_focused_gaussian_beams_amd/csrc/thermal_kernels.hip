@@ -111,16 +111,18 @@ __device__ __forceinline__ double jacobi_column(float *__restrict__ Tf, const fl
     return r;
 }
 
-// One Jacobi sweep Tf <- J(Ti, Q): a workgroup of 64 x 4 threads, each a column of ZB
+// One Jacobi sweep Tf <- J(Ti, Q): a workgroup of 64 x BY threads, each a column of ZB
 // planes (jacobi_column; ZB = jacobi_planes of the grid).  With RESID the FP64 sum of
-// (Tf - Ti)^2 over the block goes to part[block].  Used for single sweeps (the residual
-// checks of the reference's loop, odd sweep counts) and as a layout of the sweep bursts.
-template <int ZB, bool RESID>
-__global__ __launch_bounds__(256) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
-                                                const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
-                                                double *__restrict__ part) {
+// (Tf - Ti)^2 over the block goes to part[block] (BY = 4 then, so that every caller sums
+// the same partials).  Used for single sweeps (the residual checks of the reference's
+// loop, odd sweep counts) and as a layout of the sweep bursts.
+template <int ZB, bool RESID, int BY = 4>
+__global__ __launch_bounds__(64 * BY) void k_jacobi(float *__restrict__ Tf, const float *__restrict__ Ti,
+                                                    const float *__restrict__ Q, int nx, int ny, int hz, JacobiConst c,
+                                                    double *__restrict__ part) {
+    static_assert(!RESID || BY == 4, "residual partials come from 64 x 4 blocks");
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int y = blockIdx.y * BY + (threadIdx.x >> 6);
     double r = jacobi_column<ZB, RESID>(Tf, Ti, Q, nx, ny, hz, x, y, blockIdx.z * ZB, c);
     if (RESID) {
         for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
@@ -895,14 +897,23 @@ int jacobi_blocks(int nx, int ny, int hz) {
     return (nx / 64) * ((ny + 3) / 4) * ((hz + zb - 1) / zb);
 }
 
+// Sweeps without the residual on grids whose one-plane sweep has fewer than 4096 64 x 4
+// workgroups run 64 x 8 ones: half as many, each twice the work, which such a grid's
+// sweep time (bound by its launch and ramp rather than its bytes) favours — 64x64x200:
+// 4.78 -> 4.40 us per sweep; at 128x128x100 and above no gain.  Every voxel's formula is
+// the same, so the result is too.
 template <int ZB>
 static void jacobi_launch(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
                           double *part, hipStream_t st) {
     const dim3 grid(nx / 64, (ny + 3) / 4, (hz + ZB - 1) / ZB);
-    if (part)
+    if (part) {
         hipLaunchKernelGGL((k_jacobi<ZB, true>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
-    else
+    } else if (ZB == 1 && (long)grid.x * grid.y * grid.z < 4096) {
+        const dim3 grid8(nx / 64, (ny + 7) / 8, hz);
+        hipLaunchKernelGGL((k_jacobi<1, false, 8>), grid8, dim3(512), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    } else {
         hipLaunchKernelGGL((k_jacobi<ZB, false>), grid, dim3(256), 0, st, Tf, Ti, Q, nx, ny, hz, c, part);
+    }
 }
 
 hipError_t launch_jacobi(float *Tf, const float *Ti, const float *Q, int nx, int ny, int hz, const JacobiConst &c,
